@@ -1,0 +1,341 @@
+"""Python access to libapcloud (the MI355X-native AP-Cloud solve-and-move path)
+through its C ABI (include/apcloud.h).  The product path is the HIP library;
+this module only marshals arrays.  Loading fails loudly when the library has
+not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libapcloud.so")
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+_U64 = C.POINTER(C.c_uint64)
+
+APC_OK = 0
+ERRORS = {
+    -1: "APC_ERR_ARG", -2: "APC_ERR_HIP", -3: "APC_ERR_NOMEM", -4: "APC_ERR_NEIGH_MANY",
+    -5: "APC_ERR_NEIGH_FEW", -6: "APC_ERR_NEIGH_INVALID", -7: "APC_ERR_RANK",
+    -8: "APC_ERR_BALANCE", -9: "APC_ERR_UNSUPPORTED", -10: "APC_ERR_NO_DEVICE",
+}
+
+# C ABI entry points declared in include/apcloud.h
+EXPORTS = [
+    "apc_last_error", "apc_version", "apc_device_count", "apc_create", "apc_destroy",
+    "apc_set_particles", "apc_get_particles", "apc_octree_build", "apc_get_sorted",
+    "apc_octree_level", "apc_select", "apc_get_comp_sizes", "apc_get_comp", "apc_neighbors",
+    "apc_solver_set_constants", "apc_build_system", "apc_solver_solve", "apc_differentiate",
+    "apc_get_fields", "apc_set_gradients", "apc_interp_kick", "apc_drift", "apc_max_density",
+    "apc_step", "apc_kernel_times", "apc_reset_kernel_times", "apc_set_timing", "apc_sync",
+]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("bbox", C.c_double * 6), ("depth", C.c_int), ("bc", C.c_int), ("c", C.c_double),
+        ("order", C.c_int), ("mindepth", C.c_int), ("maxdepth", C.c_int),
+        ("leaf_capacity", C.c_int), ("lm", C.c_int), ("dirichlet_point", C.c_int),
+        ("boundary_at_faces", C.c_int), ("bv_kind", C.c_int), ("bv_mass", C.c_double),
+        ("bv_radius", C.c_double), ("bv_center", C.c_double * 3), ("bv_G", C.c_double),
+    ]
+
+
+class KspOpts(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("dtol", C.c_double),
+                ("max_it", C.c_int), ("restart", C.c_int), ("monitor", C.c_int),
+                ("precision", C.c_int)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("its_rho", C.c_int), ("reason_rho", C.c_int), ("rnorm_rho", C.c_double),
+                ("its_phi", C.c_int), ("reason_phi", C.c_int), ("rnorm_phi", C.c_double),
+                ("refine_rho", C.c_int), ("refine_phi", C.c_int)]
+
+
+class CompView(C.Structure):
+    _fields_ = [(n, _D) for n in ("x", "y", "z", "cell_length")] + \
+               [(n, _I) for n in ("np", "first", "type", "first_neigh", "number_neigh", "neigh24")] + \
+               [(n, _D) for n in ("density", "potential", "p_x", "p_y", "p_z", "p_xx", "p_yy",
+                                  "p_zz", "p_xy", "p_xz", "p_yz")]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libapcloud.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libapcloud.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.apc_last_error.restype = C.c_char_p
+        L.apc_last_error.argtypes = [C.c_void_p]
+        L.apc_version.restype = C.c_char_p
+        L.apc_device_count.argtypes = [_I]
+        L.apc_create.argtypes = [C.POINTER(Params), C.POINTER(C.c_void_p)]
+        L.apc_destroy.argtypes = [C.c_void_p]
+        L.apc_set_particles.argtypes = [C.c_void_p, C.c_int] + [_D] * 6
+        L.apc_get_particles.argtypes = [C.c_void_p] + [_D] * 9
+        L.apc_octree_build.argtypes = [C.c_void_p]
+        L.apc_get_sorted.argtypes = [C.c_void_p, _U64, _I]
+        L.apc_octree_level.argtypes = [C.c_void_p, C.c_int, _I, _U64, _I, _I]
+        L.apc_select.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, _I]
+        L.apc_get_comp_sizes.argtypes = [C.c_void_p, _I, _I, _I, _I]
+        L.apc_get_comp.argtypes = [C.c_void_p, C.POINTER(CompView)]
+        L.apc_neighbors.argtypes = [C.c_void_p, _I, _D]
+        L.apc_solver_set_constants.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.apc_build_system.argtypes = [C.c_void_p, _I] + [_D] * 6
+        L.apc_solver_solve.argtypes = [C.c_void_p, C.POINTER(KspOpts), C.POINTER(KspOpts),
+                                       C.POINTER(SolveInfo)]
+        L.apc_differentiate.argtypes = [C.c_void_p, _D]
+        L.apc_get_fields.argtypes = [C.c_void_p, _D, _D, _D, _D]
+        L.apc_set_gradients.argtypes = [C.c_void_p, _D, _D]
+        L.apc_interp_kick.argtypes = [C.c_void_p, C.c_double, C.c_double, _D]
+        L.apc_drift.argtypes = [C.c_void_p, C.c_double, C.c_double, _D]
+        L.apc_max_density.argtypes = [C.c_void_p, _D]
+        L.apc_step.argtypes = [C.c_void_p] + [C.c_double] * 6 + [C.POINTER(KspOpts)] * 2 + \
+            [_D, _D, C.POINTER(SolveInfo), _D]
+        L.apc_kernel_times.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _D, _I, C.c_int]
+        L.apc_reset_kernel_times.argtypes = [C.c_void_p]
+        L.apc_set_timing.argtypes = [C.c_void_p, C.c_int]
+        L.apc_sync.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_D):
+    return a.ctypes.data_as(t)
+
+
+class ApcError(RuntimeError):
+    pass
+
+
+def make_params(bbox=(0, 1, 0, 1, 0, 1), depth=20, bc=1, c=1.0, lm=17, **kw):
+    p = Params()
+    p.bbox[:] = list(bbox)
+    p.depth = depth
+    p.bc = bc
+    p.c = c
+    p.order = kw.get("order", 2)
+    p.mindepth = kw.get("mindepth", 0)
+    p.maxdepth = kw.get("maxdepth", depth)
+    p.leaf_capacity = kw.get("leaf_capacity", 0)
+    p.lm = lm
+    p.dirichlet_point = kw.get("dirichlet_point", 1)
+    p.boundary_at_faces = kw.get("boundary_at_faces", 0)
+    p.bv_kind = kw.get("bv_kind", 0)
+    p.bv_mass = kw.get("bv_mass", 1.0)
+    p.bv_radius = kw.get("bv_radius", 0.5)
+    p.bv_center[:] = list(kw.get("bv_center", (0.5, 0.5, 0.5)))
+    p.bv_G = kw.get("bv_G", 1.0)
+    return p
+
+
+def ksp(rtol=1e-5, atol=1e-50, dtol=1e5, max_it=10000, restart=30, monitor=0, precision=0):
+    o = KspOpts()
+    o.rtol, o.atol, o.dtol = rtol, atol, dtol
+    o.max_it, o.restart, o.monitor, o.precision = max_it, restart, monitor, precision
+    return o
+
+
+def device_count():
+    n = C.c_int()
+    lib().apc_device_count(C.byref(n))
+    return n.value
+
+
+class Context:
+    """One simulation box on the current GPU."""
+
+    def __init__(self, params):
+        self.L = lib()
+        self.p = params
+        h = C.c_void_p()
+        r = self.L.apc_create(C.byref(params), C.byref(h))
+        if r != 0:
+            raise ApcError("apc_create failed: %s" % ERRORS.get(r, r))
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.apc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, r, what):
+        if r != 0:
+            raise ApcError("%s failed (%s): %s" % (what, ERRORS.get(r, r),
+                                                   self.L.apc_last_error(self.h).decode()))
+
+    def set_particles(self, x, y, z, vx=None, vy=None, vz=None):
+        self.n = len(x)
+        arrs = [np.ascontiguousarray(a, np.float64) if a is not None else None
+                for a in (x, y, z, vx, vy, vz)]
+        ptrs = [_p(a) if a is not None else None for a in arrs]
+        self._chk(self.L.apc_set_particles(self.h, self.n, *ptrs), "apc_set_particles")
+
+    def particles(self):
+        out = [np.zeros(self.n) for _ in range(9)]
+        self._chk(self.L.apc_get_particles(self.h, *[_p(a) for a in out]), "apc_get_particles")
+        return out
+
+    def octree_build(self):
+        self._chk(self.L.apc_octree_build(self.h), "apc_octree_build")
+
+    def sorted(self):
+        k = np.zeros(self.n, np.uint64)
+        i = np.zeros(self.n, np.int32)
+        self._chk(self.L.apc_get_sorted(self.h, _p(k, _U64), _p(i, _I)), "apc_get_sorted")
+        return k, i
+
+    def octree_level(self, level):
+        cnt = C.c_int()
+        self._chk(self.L.apc_octree_level(self.h, level, C.byref(cnt), None, None, None),
+                  "apc_octree_level")
+        m = cnt.value
+        key = np.zeros(m, np.uint64)
+        first = np.zeros(m, np.int32)
+        npp = np.zeros(m, np.int32)
+        self._chk(self.L.apc_octree_level(self.h, level, C.byref(cnt), _p(key, _U64),
+                                          _p(first, _I), _p(npp, _I)), "apc_octree_level")
+        return key, first, npp
+
+    def select(self, c=None, order=2, mindepth=0, maxdepth=None):
+        c = self.p.c if c is None else c
+        maxdepth = self.p.depth if maxdepth is None else maxdepth
+        S = C.c_int()
+        self._chk(self.L.apc_select(self.h, c, order, mindepth, maxdepth, C.byref(S)), "apc_select")
+        return S.value
+
+    def comp(self, fields=False):
+        nc, nv, nb, nn = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self._chk(self.L.apc_get_comp_sizes(self.h, C.byref(nc), C.byref(nv), C.byref(nb),
+                                            C.byref(nn)), "apc_get_comp_sizes")
+        S = nc.value + nv.value + nb.value
+        M = nc.value + nv.value
+        d = {k: np.zeros(S) for k in ("x", "y", "z", "cell_length")}
+        i = {k: np.zeros(S, np.int32) for k in ("np", "first", "type", "first_neigh", "number_neigh")}
+        n24 = np.zeros(max(1, nn.value), np.int32)
+        v = CompView()
+        for k, a in d.items():
+            setattr(v, k, _p(a))
+        for k, a in i.items():
+            setattr(v, k, _p(a, _I))
+        v.neigh24 = _p(n24, _I)
+        f = {}
+        if fields:
+            f = {k: np.zeros(M) for k in ("density", "potential", "p_x", "p_y", "p_z", "p_xx",
+                                           "p_yy", "p_zz", "p_xy", "p_xz", "p_yz")}
+            for k, a in f.items():
+                setattr(v, k, _p(a))
+        self._chk(self.L.apc_get_comp(self.h, C.byref(v)), "apc_get_comp")
+        out = dict(nc=nc.value, nv=nv.value, nb=nb.value, x=d["x"], y=d["y"], z=d["z"],
+                   h=d["cell_length"], np=i["np"], first=i["first"], type=i["type"],
+                   fneigh=i["first_neigh"], nneigh=i["number_neigh"], neigh24=n24[:nn.value])
+        out.update(f)
+        return out
+
+    def neighbors(self):
+        nc, nv, nb, nn = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self._chk(self.L.apc_get_comp_sizes(self.h, C.byref(nc), C.byref(nv), C.byref(nb), None),
+                  "apc_get_comp_sizes")
+        M = nc.value + nv.value
+        lm = self.p.lm
+        nbr = np.zeros(M * lm, np.int32)
+        dist = np.zeros(M * lm)
+        self._chk(self.L.apc_neighbors(self.h, _p(nbr, _I), _p(dist)), "apc_neighbors")
+        return nbr.reshape(M, lm), dist.reshape(M, lm)
+
+    def set_constants(self, c_dens, c_pot, c_field):
+        self._chk(self.L.apc_solver_set_constants(self.h, c_dens, c_pot, c_field),
+                  "apc_solver_set_constants")
+
+    def build_system(self):
+        nc, nv, nb = C.c_int(), C.c_int(), C.c_int()
+        self._chk(self.L.apc_get_comp_sizes(self.h, C.byref(nc), C.byref(nv), C.byref(nb), None),
+                  "apc_get_comp_sizes")
+        M = nc.value + nv.value
+        lm = self.p.lm
+        cols = np.zeros(M * lm, np.int32)
+        a1 = np.zeros(M * lm)
+        a2 = np.zeros(M * lm)
+        vec = [np.zeros(M) for _ in range(4)]
+        self._chk(self.L.apc_build_system(self.h, _p(cols, _I), _p(a1), _p(a2),
+                                          *[_p(v) for v in vec]), "apc_build_system")
+        return dict(cols=cols.reshape(M, lm), a1=a1.reshape(M, lm), a2=a2.reshape(M, lm),
+                    d1=vec[0], d2=vec[1], rhs1=vec[2], pbnd=vec[3])
+
+    def solve(self, rho_opts, phi_opts=None):
+        info = SolveInfo()
+        phi_opts = rho_opts if phi_opts is None else phi_opts
+        self._chk(self.L.apc_solver_solve(self.h, C.byref(rho_opts), C.byref(phi_opts),
+                                          C.byref(info)), "apc_solver_solve")
+        return info
+
+    def fields(self):
+        nc, nv, nb = C.c_int(), C.c_int(), C.c_int()
+        self._chk(self.L.apc_get_comp_sizes(self.h, C.byref(nc), C.byref(nv), C.byref(nb), None),
+                  "apc_get_comp_sizes")
+        M = nc.value + nv.value
+        rho, phi, g, H = np.zeros(M), np.zeros(M), np.zeros(3 * M), np.zeros(6 * M)
+        self._chk(self.L.apc_get_fields(self.h, _p(rho), _p(phi), _p(g), _p(H)), "apc_get_fields")
+        return dict(rho=rho, phi=phi, g=g.reshape(M, 3), H=H.reshape(M, 6))
+
+    def differentiate(self, phi):
+        phi = np.ascontiguousarray(phi, np.float64)
+        self._chk(self.L.apc_differentiate(self.h, _p(phi)), "apc_differentiate")
+
+    def set_gradients(self, g, H):
+        g = np.ascontiguousarray(g, np.float64).ravel()
+        H = np.ascontiguousarray(H, np.float64).ravel()
+        self._chk(self.L.apc_set_gradients(self.h, _p(g), _p(H)), "apc_set_gradients")
+
+    def interp_kick(self, cv, dt):
+        out = C.c_double()
+        self._chk(self.L.apc_interp_kick(self.h, cv, dt, C.byref(out)), "apc_interp_kick")
+        return out.value
+
+    def drift(self, cx, dt):
+        out = C.c_double()
+        self._chk(self.L.apc_drift(self.h, cx, dt, C.byref(out)), "apc_drift")
+        return out.value
+
+    def max_density(self):
+        out = C.c_double()
+        self._chk(self.L.apc_max_density(self.h, C.byref(out)), "apc_max_density")
+        return out.value
+
+    def step(self, c_dens, c_pot, c_field, cv, cx, dt, rho_opts, phi_opts=None):
+        phi_opts = rho_opts if phi_opts is None else phi_opts
+        mdv, mdx = C.c_double(), C.c_double()
+        info = SolveInfo()
+        timing = np.zeros(3)
+        self._chk(self.L.apc_step(self.h, c_dens, c_pot, c_field, cv, cx, dt, C.byref(rho_opts),
+                                  C.byref(phi_opts), C.byref(mdv), C.byref(mdx), C.byref(info),
+                                  _p(timing)), "apc_step")
+        return mdv.value, mdx.value, info, timing
+
+    def set_timing(self, on=True):
+        self._chk(self.L.apc_set_timing(self.h, 1 if on else 0), "apc_set_timing")
+
+    def reset_kernel_times(self):
+        self._chk(self.L.apc_reset_kernel_times(self.h), "apc_reset_kernel_times")
+
+    def kernel_times(self):
+        buf = C.create_string_buffer(8192)
+        avg = np.zeros(128)
+        cnt = np.zeros(128, np.int32)
+        n = self.L.apc_kernel_times(self.h, buf, 8192, _p(avg), _p(cnt, _I), 128)
+        names = buf.raw.split(b"\0")[:n]
+        return {nm.decode(): (avg[q], int(cnt[q])) for q, nm in enumerate(names)}
+
+    def sync(self):
+        self._chk(self.L.apc_sync(self.h), "apc_sync")

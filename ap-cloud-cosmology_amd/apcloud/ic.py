@@ -1,0 +1,101 @@
+"""Synthetic initial conditions for the benchmark and test workloads.
+
+All generators use numpy's PCG64 with the workload's seed, so every machine
+regenerates identical particles.  Positions are FP64.
+"""
+import numpy as np
+
+
+def uniform_ball(n, seed=0, radius=0.5, center=(0.5, 0.5, 0.5)):
+    """n points uniform in a ball (r = R U^(1/3), isotropic direction)."""
+    rng = np.random.default_rng(seed)
+    u = rng.random(n)
+    d = rng.standard_normal((3, n))
+    d /= np.linalg.norm(d, axis=0)
+    r = radius * np.cbrt(u)
+    pos = np.asarray(center, np.float64)[:, None] + r * d
+    return pos[0].copy(), pos[1].copy(), pos[2].copy()
+
+
+def lattice(n_side):
+    g = (np.arange(n_side, dtype=np.float64) + 0.5) / n_side
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    return X.ravel(), Y.ravel(), Z.ravel()
+
+
+def zeldovich(n_side, seed, rms, spectral_index=-2.0, a0=0.02):
+    """Perturbed unit-cube lattice: Zel'dovich displacements of a Gaussian
+    field with P(k) ~ k^n (psi_k = i k delta_k / k^2), scaled to the given
+    RMS displacement; velocities are the growing mode in the drift convention
+    dx/da = Cx v with Cx = sqrt(a) (Einstein-de Sitter), v = psi / (a0^1.5).
+    Returns x, y, z, vx, vy, vz (periodically wrapped into [0,1))."""
+    rng = np.random.default_rng(seed)
+    n = n_side
+    w = rng.standard_normal((n, n, n))
+    W = np.fft.rfftn(w)
+    k1 = np.fft.fftfreq(n, d=1.0 / n) * 2 * np.pi
+    kz = np.fft.rfftfreq(n, d=1.0 / n) * 2 * np.pi
+    KX, KY, KZ = np.meshgrid(k1, k1, kz, indexing="ij")
+    k2 = KX * KX + KY * KY + KZ * KZ
+    k2[0, 0, 0] = 1.0
+    amp = np.power(k2, spectral_index / 4.0)  # sqrt(P(k)) = k^(n/2)
+    amp[0, 0, 0] = 0.0
+    D = W * amp
+    psi = []
+    for K in (KX, KY, KZ):
+        psi.append(np.fft.irfftn(1j * K * D / k2, s=(n, n, n), axes=(0, 1, 2)).ravel())
+    psi = np.stack(psi)
+    scale = rms / np.sqrt(np.mean(np.sum(psi * psi, axis=0)))
+    psi *= scale
+    qx, qy, qz = lattice(n)
+    x = np.mod(qx + psi[0], 1.0)
+    y = np.mod(qy + psi[1], 1.0)
+    z = np.mod(qz + psi[2], 1.0)
+    vfac = 1.0 / (a0 * np.sqrt(a0))
+    return x, y, z, psi[0] * vfac, psi[1] * vfac, psi[2] * vfac
+
+
+def plummer_clusters(x, y, z, vx, vy, vz, seed, n_spheres=64, scale=0.005, fraction=0.2,
+                     rmax_factor=30.0):
+    """Moves `fraction` of the particles into n_spheres Plummer spheres
+    (r = b / sqrt(U^(-2/3) - 1), truncated at rmax_factor * b) with uniform
+    centres; moved particles keep zero velocity.  Periodically wrapped."""
+    rng = np.random.default_rng(seed + 1000)
+    n = len(x)
+    k = int(round(fraction * n))
+    sel = rng.choice(n, size=k, replace=False)
+    cen = rng.random((n_spheres, 3))
+    which = rng.integers(0, n_spheres, k)
+    umax = (1.0 + 1.0 / rmax_factor ** 2) ** -1.5
+    u = rng.random(k) * umax
+    r = scale / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
+    d = rng.standard_normal((3, k))
+    d /= np.linalg.norm(d, axis=0)
+    x, y, z, vx, vy, vz = [a.copy() for a in (x, y, z, vx, vy, vz)]
+    x[sel] = np.mod(cen[which, 0] + r * d[0], 1.0)
+    y[sel] = np.mod(cen[which, 1] + r * d[1], 1.0)
+    z[sel] = np.mod(cen[which, 2] + r * d[2], 1.0)
+    vx[sel] = 0.0
+    vy[sel] = 0.0
+    vz[sel] = 0.0
+    return x, y, z, vx, vy, vz
+
+
+def perturbed_lattice(n_side, seed, sigma):
+    """Lattice with Gaussian jitter (sigma in box units), wrapped."""
+    rng = np.random.default_rng(seed)
+    x, y, z = lattice(n_side)
+    n = len(x)
+    return (np.mod(x + sigma * rng.standard_normal(n), 1.0),
+            np.mod(y + sigma * rng.standard_normal(n), 1.0),
+            np.mod(z + sigma * rng.standard_normal(n), 1.0))
+
+
+def clustered(n, seed, n_clusters=4, scale=0.01, fraction=0.5):
+    """Uniform background plus Plummer clusters (small tests)."""
+    rng = np.random.default_rng(seed)
+    x, y, z = rng.random((3, n))
+    z0 = np.zeros(n)
+    x, y, z, _, _, _ = plummer_clusters(x, y, z, z0, z0, z0, seed, n_spheres=n_clusters,
+                                        scale=scale, fraction=fraction)
+    return x, y, z

@@ -1,0 +1,331 @@
+// Context lifetime, particle transfer, solver entry points, per-kernel
+// event timers and the one-call step.
+#include <chrono>
+#include <cstdarg>
+
+#include "apc_internal.h"
+#include "apc_prims.h"
+
+namespace apc {
+
+int fail(apc_ctx *c, int code, const char *fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+int hip_fail(apc_ctx *c, hipError_t e, const char *what) {
+  return fail(c, APC_ERR_HIP, "HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), what);
+}
+
+int ktimer_index(apc_ctx *c, const char *name) {
+  for (size_t q = 0; q < c->timers.size(); ++q)
+    if (c->timers[q].name == name) return (int)q;
+  c->timers.push_back(KTimer());
+  c->timers.back().name = name;
+  return (int)c->timers.size() - 1;
+}
+
+void ktime_begin(apc_ctx *c, int idx) {
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  hipEventRecord(e, c->s);
+  c->timers[idx].ev.push_back(e);
+}
+
+void ktime_end(apc_ctx *c, int idx) {
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  hipEventRecord(e, c->s);
+  c->timers[idx].ev.push_back(e);
+  c->timers[idx].launches++;
+}
+
+static void fold_timers(apc_ctx *c) {
+  hipStreamSynchronize(c->s);
+  for (auto &t : c->timers) {
+    for (size_t q = 0; q + 1 < t.ev.size(); q += 2) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, t.ev[q], t.ev[q + 1]) == hipSuccess) t.total_ms += ms;
+    }
+    for (auto e : t.ev) hipEventDestroy(e);
+    t.ev.clear();
+  }
+}
+
+__global__ void k_scatter_back(int n, const int *__restrict__ id, const double *__restrict__ a,
+                               double *__restrict__ out) {
+  int s = blockIdx.x * 256 + threadIdx.x;
+  if (s < n) out[id[s]] = a[s];
+}
+
+__global__ void k_iota(int n, int *__restrict__ a) {
+  int s = blockIdx.x * 256 + threadIdx.x;
+  if (s < n) a[s] = s;
+}
+
+// (M x 3 / M x 6 row-major) <-> slot-major grad
+__global__ void k_set_grad(int M, const double *__restrict__ g, const double *__restrict__ H,
+                           double *__restrict__ grad) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= M) return;
+  for (int q = 0; q < 3; ++q) grad[(size_t)q * M + i] = g[3 * (size_t)i + q];
+  for (int q = 0; q < 6; ++q) grad[(size_t)(3 + q) * M + i] = H[6 * (size_t)i + q];
+}
+
+}  // namespace apc
+
+using namespace apc;
+
+extern "C" {
+
+const char *apc_last_error(const apc_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+const char *apc_version(void) { return "apcloud-mi355x 0.1 (gfx950)"; }
+
+int apc_device_count(int *n) {
+  int k = 0;
+  hipError_t e = hipGetDeviceCount(&k);
+  if (e != hipSuccess) k = 0;
+  if (n) *n = k;
+  return (k > 0) ? APC_OK : APC_ERR_NO_DEVICE;
+}
+
+int apc_create(const apc_params *p, apc_ctx **out) {
+  if (!p || !out) return APC_ERR_ARG;
+  *out = nullptr;
+  if (p->depth < 1 || p->depth > 21) return APC_ERR_ARG;
+  if (p->lm < 9) return APC_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return APC_ERR_NO_DEVICE;
+  apc_ctx *c = new apc_ctx();
+  c->prm = *p;
+  hipGetDevice(&c->dev);
+  if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return APC_ERR_HIP;
+  }
+  *out = c;
+  return APC_OK;
+}
+
+int apc_destroy(apc_ctx *c) {
+  if (!c) return APC_OK;
+  hipStreamSynchronize(c->s);
+  for (auto &t : c->timers)
+    for (auto e : t.ev) hipEventDestroy(e);
+  free_solve_state(c);
+  apc::DBuf<double> *ds[] = {&c->x, &c->y, &c->z, &c->vx, &c->vy, &c->vz, &c->ax, &c->ay, &c->az,
+                             &c->t0, &c->t1, &c->t2, &c->t3, &c->t4, &c->t5, &c->ncx, &c->ncy,
+                             &c->ncz, &c->cx, &c->cy, &c->cz, &c->ch, &c->dist, &c->sco, &c->d1,
+                             &c->d2, &c->f1, &c->f2, &c->dinv1, &c->dinv2, &c->rhs1, &c->pbnd,
+                             &c->bv, &c->rho, &c->phi, &c->rhs2, &c->grad, &c->nscratch, &c->red_d};
+  for (auto b : ds) b->release();
+  apc::DBuf<int> *is[] = {&c->id, &c->tid, &c->perm, &c->perm2, &c->hist, &c->histx,
+                          &c->scratch_i, &c->nlevel, &c->nfirst, &c->nnp, &c->nstatus, &c->nchild,
+                          &c->nparent, &c->nflag, &c->nrank, &c->nout, &c->dcount, &c->leaf_node,
+                          &c->cnp, &c->cfirst, &c->ctype, &c->n24, &c->n24cnt, &c->n24first,
+                          &c->owner, &c->nbr, &c->nscratch_i, &c->status_dev};
+  for (auto b : is) b->release();
+  c->key.release(); c->key2.release(); c->nprefix.release(); c->red_u64.release();
+  hipStreamDestroy(c->s);
+  delete c;
+  return APC_OK;
+}
+
+int apc_set_particles(apc_ctx *c, int n, const double *x, const double *y, const double *z,
+                      const double *vx, const double *vy, const double *vz) {
+  if (!c || n < 1 || !x || !y || !z) return fail(c, APC_ERR_ARG, "set_particles: bad arguments");
+  c->N = n;
+  APC_ALLOC(c, c->x, n); APC_ALLOC(c, c->y, n); APC_ALLOC(c, c->z, n);
+  APC_ALLOC(c, c->vx, n); APC_ALLOC(c, c->vy, n); APC_ALLOC(c, c->vz, n);
+  APC_ALLOC(c, c->ax, n); APC_ALLOC(c, c->ay, n); APC_ALLOC(c, c->az, n);
+  APC_ALLOC(c, c->id, n);
+  const size_t b = sizeof(double) * n;
+  APC_HIP(c, hipMemcpyAsync(c->x.p, x, b, hipMemcpyHostToDevice, c->s));
+  APC_HIP(c, hipMemcpyAsync(c->y.p, y, b, hipMemcpyHostToDevice, c->s));
+  APC_HIP(c, hipMemcpyAsync(c->z.p, z, b, hipMemcpyHostToDevice, c->s));
+  const double *vs[3] = {vx, vy, vz};
+  double *vd[3] = {c->vx.p, c->vy.p, c->vz.p};
+  for (int q = 0; q < 3; ++q) {
+    if (vs[q]) APC_HIP(c, hipMemcpyAsync(vd[q], vs[q], b, hipMemcpyHostToDevice, c->s));
+    else APC_HIP(c, hipMemsetAsync(vd[q], 0, b, c->s));
+  }
+  APC_HIP(c, hipMemsetAsync(c->ax.p, 0, b, c->s));
+  APC_HIP(c, hipMemsetAsync(c->ay.p, 0, b, c->s));
+  APC_HIP(c, hipMemsetAsync(c->az.p, 0, b, c->s));
+  hipLaunchKernelGGL(k_iota, dim3(nblocks(n, 256)), dim3(256), 0, c->s, n, c->id.p);
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  c->sorted = false;
+  c->have_comp = false;
+  c->have_nbr = c->have_sys = c->have_fields = 0;
+  return APC_OK;
+}
+
+int apc_get_particles(apc_ctx *c, double *x, double *y, double *z, double *vx, double *vy,
+                      double *vz, double *ax, double *ay, double *az) {
+  if (!c || c->N < 1) return fail(c, APC_ERR_ARG, "get_particles: no particles");
+  const int n = c->N;
+  APC_ALLOC(c, c->t0, n);
+  double *dst[9] = {x, y, z, vx, vy, vz, ax, ay, az};
+  const double *src[9] = {c->x.p, c->y.p, c->z.p, c->vx.p, c->vy.p, c->vz.p, c->ax.p, c->ay.p, c->az.p};
+  for (int q = 0; q < 9; ++q) {
+    if (!dst[q]) continue;
+    hipLaunchKernelGGL(k_scatter_back, dim3(nblocks(n, 256)), dim3(256), 0, c->s, n, c->id.p,
+                       src[q], c->t0.p);
+    APC_HIP(c, hipMemcpyAsync(dst[q], c->t0.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+    APC_HIP(c, hipStreamSynchronize(c->s));
+  }
+  return APC_OK;
+}
+
+int apc_solver_set_constants(apc_ctx *c, double c_dens, double c_pot, double c_field) {
+  if (!c) return APC_ERR_ARG;
+  c->c_dens = c_dens;
+  c->c_pot = c_pot;
+  c->c_field = c_field;
+  c->have_sys = 0;  // rhs1 carries c_dens
+  return APC_OK;
+}
+
+int apc_solver_solve(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_opts *o2,
+                     apc_solve_info *info) {
+  if (!c || !o1 || !o2) return APC_ERR_ARG;
+  if (!c->have_comp) return fail(c, APC_ERR_ARG, "solve: no computational particles");
+  int r = build_stencils(c);
+  if (r) return r;
+  r = solve_density_potential(c, o1, o2, info);
+  if (r) return r;
+  return differentiate(c);
+}
+
+int apc_differentiate(apc_ctx *c, const double *phi) {
+  if (!c || !phi) return APC_ERR_ARG;
+  if (!c->have_comp) return fail(c, APC_ERR_ARG, "differentiate: no computational particles");
+  if (!c->have_sys) {
+    int r = build_stencils(c);
+    if (r) return r;
+  }
+  const int M = c->nc + c->nv;
+  APC_ALLOC(c, c->phi, M + 1);
+  APC_HIP(c, hipMemcpyAsync(c->phi.p, phi, sizeof(double) * M, hipMemcpyHostToDevice, c->s));
+  return differentiate(c);
+}
+
+int apc_get_fields(apc_ctx *c, double *rho, double *phi, double *g, double *H) {
+  if (!c) return APC_ERR_ARG;
+  if (!c->have_fields) return fail(c, APC_ERR_ARG, "get_fields: no fields");
+  const int M = c->nc + c->nv;
+  std::vector<double> gr((size_t)9 * M);
+  if (rho && c->rho.p) APC_HIP(c, hipMemcpy(rho, c->rho.p, 8ull * M, hipMemcpyDeviceToHost));
+  if (phi) APC_HIP(c, hipMemcpy(phi, c->phi.p, 8ull * M, hipMemcpyDeviceToHost));
+  APC_HIP(c, hipMemcpy(gr.data(), c->grad.p, 8ull * 9 * M, hipMemcpyDeviceToHost));
+  for (int i = 0; i < M; ++i) {
+    if (g)
+      for (int q = 0; q < 3; ++q) g[3 * (size_t)i + q] = gr[(size_t)q * M + i];
+    if (H)
+      for (int q = 0; q < 6; ++q) H[6 * (size_t)i + q] = gr[(size_t)(3 + q) * M + i];
+  }
+  return APC_OK;
+}
+
+int apc_set_gradients(apc_ctx *c, const double *g, const double *H) {
+  if (!c || !g || !H) return APC_ERR_ARG;
+  if (!c->have_comp) return fail(c, APC_ERR_ARG, "set_gradients: no computational particles");
+  const int M = c->nc + c->nv;
+  APC_ALLOC(c, c->grad, (size_t)9 * M + 1);
+  apc::DBuf<double> tg, th;
+  if (tg.ensure((size_t)3 * M + 1) || th.ensure((size_t)6 * M + 1)) {
+    tg.release(); th.release();
+    return fail(c, APC_ERR_NOMEM, "set_gradients");
+  }
+  hipError_t e = hipMemcpy(tg.p, g, 8ull * 3 * M, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(th.p, H, 8ull * 6 * M, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_set_grad, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, tg.p, th.p, c->grad.p);
+    e = hipStreamSynchronize(c->s);
+  }
+  tg.release(); th.release();
+  if (e != hipSuccess) return hip_fail(c, e, "set_gradients");
+  c->have_fields = 1;
+  return APC_OK;
+}
+
+int apc_step(apc_ctx *c, double c_dens, double c_pot, double c_field, double Cv, double Cx,
+             double dt, const apc_ksp_opts *o1, const apc_ksp_opts *o2, double *max_dv,
+             double *max_dx, apc_solve_info *info, double *timing) {
+  if (!c) return APC_ERR_ARG;
+  using clk = std::chrono::steady_clock;
+  auto t0 = clk::now();
+  int r = apc_octree_build(c);
+  if (r) return r;
+  int S = 0;
+  r = apc_select(c, c->prm.c, c->prm.order, c->prm.mindepth, c->prm.maxdepth, &S);
+  if (r) return r;
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  auto t1 = clk::now();
+  r = apc_solver_set_constants(c, c_dens, c_pot, c_field);
+  if (r) return r;
+  r = apc_solver_solve(c, o1, o2, info);
+  if (r) return r;
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  auto t2 = clk::now();
+  r = interp_kick(c, Cv, dt, max_dv);
+  if (r) return r;
+  r = drift(c, Cx, dt, max_dx);
+  if (r) return r;
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  auto t3 = clk::now();
+  if (timing) {
+    timing[0] = std::chrono::duration<double>(t1 - t0).count();
+    timing[1] = std::chrono::duration<double>(t2 - t1).count();
+    timing[2] = std::chrono::duration<double>(t3 - t2).count();
+  }
+  return APC_OK;
+}
+
+int apc_set_timing(apc_ctx *c, int on) {
+  if (!c) return APC_ERR_ARG;
+  c->timing = on != 0;
+  return APC_OK;
+}
+
+int apc_reset_kernel_times(apc_ctx *c) {
+  if (!c) return APC_ERR_ARG;
+  fold_timers(c);
+  for (auto &t : c->timers) { t.total_ms = 0; t.launches = 0; }
+  return APC_OK;
+}
+
+int apc_kernel_times(apc_ctx *c, char *names, int names_len, double *avg_ms, int *launches, int cap) {
+  if (!c) return APC_ERR_ARG;
+  fold_timers(c);
+  int n = 0, pos = 0;
+  for (auto &t : c->timers) {
+    if (n >= cap) break;
+    if (names) {
+      int L = (int)t.name.size() + 1;
+      if (pos + L > names_len) break;
+      memcpy(names + pos, t.name.c_str(), L);
+      pos += L;
+    }
+    if (avg_ms) avg_ms[n] = t.launches ? t.total_ms / t.launches : 0.0;
+    if (launches) launches[n] = t.launches;
+    ++n;
+  }
+  return n;
+}
+
+int apc_sync(apc_ctx *c) {
+  if (!c) return APC_ERR_ARG;
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  return APC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,190 @@
+// Internal state of an apcloud context and helpers shared by the .hip files.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "apcloud.h"
+
+namespace apc {
+
+// Grow-only device buffer.
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= cap) return hipSuccess;
+    size_t want = n + n / 4 + 256;
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc((void **)&p, want * sizeof(T));
+    if (e != hipSuccess) { p = nullptr; return e; }
+    cap = want;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// Node status values of the selection tree.
+enum : int { ST_LEAF_TD = 1, ST_LEAF_VAC = 2, ST_OPENED = 3, ST_CLAIMED = 4 };
+
+struct KTimer {
+  std::string name;
+  std::vector<hipEvent_t> ev;  // start/stop pairs not yet folded in
+  double total_ms = 0;
+  int launches = 0;
+};
+
+struct SolveState;  // apc_gmres.hip
+
+}  // namespace apc
+
+struct apc_ctx {
+  apc_params prm;
+  hipStream_t s = nullptr;
+  int dev = 0;
+  std::string err;
+  int N = 0;
+  // particles, current (key-sorted) order; id = caller index
+  apc::DBuf<double> x, y, z, vx, vy, vz, ax, ay, az;
+  apc::DBuf<int> id;
+  apc::DBuf<double> t0, t1, t2, t3, t4, t5;  // permutation scratch
+  apc::DBuf<int> tid;
+  // sort
+  apc::DBuf<uint64_t> key, key2;
+  apc::DBuf<int> perm, perm2;
+  apc::DBuf<int> hist, histx, scratch_i;
+  bool sorted = false;
+  // selection tree
+  int T = 0, T_td = 0, ml = 0;
+  std::vector<int> lvl_start;  // top-down level ranges
+  apc::DBuf<uint64_t> nprefix;
+  apc::DBuf<int> nlevel, nfirst, nnp, nstatus, nchild, nparent, nflag, nrank, nout;
+  apc::DBuf<double> ncx, ncy, ncz;
+  apc::DBuf<int> dcount;  // small device counters
+  int *hcount = nullptr;  // pinned mirror
+  // computational particles: S = nc+nv+nb, M = nc+nv
+  int nc = 0, nv = 0, nb = 0, nneigh = 0;
+  bool have_comp = false;
+  apc::DBuf<int> leaf_node;            // M: node id of each output leaf
+  apc::DBuf<double> cx, cy, cz, ch;    // S
+  apc::DBuf<int> cnp, cfirst, ctype;   // S
+  apc::DBuf<int> n24, n24cnt, n24first;  // M*24 padded, M counts, S firsts
+  apc::DBuf<int> owner;                // N: output leaf of each sorted particle
+  double h0 = 0;
+  std::vector<double> hlev, hxl, hyl, hzl;
+  // solver
+  double c_dens = 1, c_pot = 1, c_field = 1;
+  int have_nbr = 0, have_sys = 0, have_fields = 0;
+  apc::DBuf<int> nbr;      // lm x M (slot-major)
+  apc::DBuf<double> dist;  // lm x M
+  apc::DBuf<double> sco;   // lm x M Laplacian weights s_ij
+  apc::DBuf<double> d1, d2, f1, f2, dinv1, dinv2, rhs1, pbnd, bv, rho, phi, rhs2;
+  apc::DBuf<double> grad;  // 9 x M: gx gy gz hxx hyy hzz hxy hxz hyz
+  apc::DBuf<double> nscratch;  // neighbour search scratch
+  apc::DBuf<int> nscratch_i;
+  apc::DBuf<int> status_dev;
+  apc::SolveState *gm = nullptr;
+  // reductions
+  apc::DBuf<unsigned long long> red_u64;
+  apc::DBuf<double> red_d;
+  // timing
+  bool timing = false;
+  std::vector<apc::KTimer> timers;
+};
+
+namespace apc {
+
+int fail(apc_ctx *c, int code, const char *fmt, ...);
+int hip_fail(apc_ctx *c, hipError_t e, const char *what);
+
+#define APC_HIP(ctx, call)                                    \
+  do {                                                        \
+    hipError_t e__ = (call);                                  \
+    if (e__ != hipSuccess) return apc::hip_fail(ctx, e__, #call); \
+  } while (0)
+
+#define APC_ALLOC(ctx, buf, n)                                          \
+  do {                                                                  \
+    if ((buf).ensure(n) != hipSuccess)                                  \
+      return apc::fail(ctx, APC_ERR_NOMEM, "hipMalloc failed for %s (%zu elements)", #buf, (size_t)(n)); \
+  } while (0)
+
+// Kernel timing: wrap a launch in ktime_begin/ktime_end.
+int ktimer_index(apc_ctx *c, const char *name);
+void ktime_begin(apc_ctx *c, int idx);
+void ktime_end(apc_ctx *c, int idx);
+
+struct KScope {
+  apc_ctx *c;
+  int idx;
+  KScope(apc_ctx *c_, const char *name) : c(c_), idx(-1) {
+    if (c->timing) {
+      idx = ktimer_index(c, name);
+      ktime_begin(c, idx);
+    }
+  }
+  ~KScope() {
+    if (idx >= 0) ktime_end(c, idx);
+  }
+};
+
+// Scan / reduction helpers (apc_scan.hip).  Exclusive scan of n ints; the
+// total is written to *total_dev (device) if non-null.
+int scan_exclusive(apc_ctx *c, const int *in, int *out, int n, int *total_dev);
+// Max of n non-negative doubles (device pointer in), result to host.
+int reduce_max_nonneg(apc_ctx *c, const double *in, int n, double *out_host);
+
+// Stage entry points implemented across files.
+int sort_particles(apc_ctx *c);                  // apc_sort.hip
+int select_tree(apc_ctx *c, double cc, int order, int mindepth, int maxdepth);  // apc_select.hip
+int search_neighbors(apc_ctx *c);                // apc_neigh.hip
+int build_stencils(apc_ctx *c);                  // apc_stencil.hip
+int differentiate(apc_ctx *c);                   // apc_stencil.hip
+int solve_density_potential(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_opts *o2,
+                            apc_solve_info *info);  // apc_gmres.hip
+void free_solve_state(apc_ctx *c);              // apc_gmres.hip
+int interp_kick(apc_ctx *c, double Cv, double dt, double *maxdv);  // apc_mover.hip
+int drift(apc_ctx *c, double Cx, double dt, double *maxdx);        // apc_mover.hip
+
+inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Morton helpers on 3-bit-per-level prefixes.
+__host__ __device__ inline uint64_t spread3(uint32_t v) {
+  uint64_t x = v & 0x1fffff;
+  x = (x | (x << 32)) & 0x1f00000000ffffULL;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffULL;
+  x = (x | (x << 8)) & 0x100f00f00f00f00fULL;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ULL;
+  x = (x | (x << 2)) & 0x1249249249249249ULL;
+  return x;
+}
+__host__ __device__ inline uint32_t compact3(uint64_t x) {
+  x &= 0x1249249249249249ULL;
+  x = (x | (x >> 2)) & 0x10c30c30c30c30c3ULL;
+  x = (x | (x >> 4)) & 0x100f00f00f00f00fULL;
+  x = (x | (x >> 8)) & 0x1f0000ff0000ffULL;
+  x = (x | (x >> 16)) & 0x1f00000000ffffULL;
+  x = (x | (x >> 32)) & 0x1fffffULL;
+  return (uint32_t)x;
+}
+// prefix digit order: x = bit 2, y = bit 1, z = bit 0 of every 3-bit group
+__host__ __device__ inline uint64_t morton_encode(uint32_t ix, uint32_t iy, uint32_t iz) {
+  return (spread3(ix) << 2) | (spread3(iy) << 1) | spread3(iz);
+}
+__host__ __device__ inline void morton_decode(uint64_t m, uint32_t &ix, uint32_t &iy, uint32_t &iz) {
+  ix = compact3(m >> 2);
+  iy = compact3(m >> 1);
+  iz = compact3(m);
+}
+
+}  // namespace apc

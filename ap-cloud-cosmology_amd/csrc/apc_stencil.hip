@@ -1,0 +1,323 @@
+// GFD weighted-least-squares stencils (solver.cpp:24-180) and the gradient /
+// Hessian fit (solver.cpp:295-362), one thread per computational/vacuum row.
+//
+// The reference solves each row's lm x 9 weighted least-squares problem with
+// LAPACKE_dgels (QR).  Here every row streams its lm weighted basis rows
+// through Givens rotations into the 9x9 triangular factor R (kept in
+// registers), so B^T B = R^T R without forming the normal matrix:
+//   Laplacian weights  s_j = w_j^2 * basis_j . y,  y = R^-1 R^-T (e3+e4+e5)
+//   (the reference's X3+X4+X5 column sums of X = (WB)^+ W), and
+//   gradient/Hessian   c = R^-1 Q^T b with b carried as a tenth column.
+// Matrices are stored slot-major (slot j of all rows contiguous) so the
+// SpMV and the stencil writes coalesce.
+#include "apc_internal.h"
+#include "apc_prims.h"
+
+namespace apc {
+
+__device__ inline double pwrap_s(double x, double len, int bc) {
+  if (bc == 1 && isfinite(x)) {  // a non-finite value would never leave the loop
+    while (x >= (0.5 * len)) x -= len;
+    while (x < -(0.5 * len)) x += len;
+  }
+  return x;
+}
+
+constexpr int RI(int k, int m) { return 9 * k - (k * (k - 1)) / 2 + (m - k); }
+
+struct Geo {
+  double len[3];
+  int bc;
+};
+
+// Row basis (solver.cpp:87-100): [p1,p2,p3,p1^2/2,p2^2/2,p3^2/2,p1p2,p1p3,p2p3]
+__device__ inline void basis_row(double p1, double p2, double p3, double *r) {
+  r[0] = p1; r[1] = p2; r[2] = p3;
+  r[3] = p1 * p1 / 2.0; r[4] = p2 * p2 / 2.0; r[5] = p3 * p3 / 2.0;
+  r[6] = p1 * p2; r[7] = p1 * p3; r[8] = p2 * p3;
+}
+
+// Rotate row r (with optional RHS rb) into R (and Qtb).
+template <bool RHS>
+__device__ inline void givens_add(double *R, double *Qtb, double *r, double rb) {
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    double a = R[RI(k, k)], b = r[k];
+    if (b != 0.0) {
+      double rr = sqrt(a * a + b * b);
+      double cs = a / rr, sn = b / rr;
+      R[RI(k, k)] = rr;
+#pragma unroll
+      for (int m = k + 1; m < 9; ++m) {
+        double u = R[RI(k, m)], v = r[m];
+        R[RI(k, m)] = cs * u + sn * v;
+        r[m] = cs * v - sn * u;
+      }
+      if (RHS) {
+        double u = Qtb[k];
+        Qtb[k] = cs * u + sn * rb;
+        rb = cs * rb - sn * u;
+      }
+    }
+  }
+}
+
+// Gaussian weights of solver.cpp:77-85 (maxdis = 0.7 * max distance).
+__device__ inline double row_maxdis(const double *__restrict__ dist, int M, int i, int lm) {
+  double maxdis = 0;
+  for (int j = 0; j < lm; ++j) {
+    double d = dist[(size_t)j * M + i];
+    if (d > maxdis) maxdis = d;
+  }
+  return maxdis * 0.7;
+}
+__device__ inline double weight_of(double d, double maxdis) {
+  return (exp(-d * d / maxdis / maxdis) - exp(-4.0)) / (1 - exp(-4.0));
+}
+
+__global__ void __launch_bounds__(256) k_stencil(
+    int M, int lm, Geo G, int pin_row0, double c_dens, const double *__restrict__ cx,
+    const double *__restrict__ cy, const double *__restrict__ cz, const double *__restrict__ ch,
+    const int *__restrict__ cnp, const int *__restrict__ nbr, const double *__restrict__ dist,
+    const double *__restrict__ bv, double *__restrict__ sco, double *__restrict__ d1,
+    double *__restrict__ d2, double *__restrict__ f1, double *__restrict__ f2,
+    double *__restrict__ dinv1, double *__restrict__ dinv2, double *__restrict__ rhs1,
+    double *__restrict__ pbnd, int *__restrict__ status) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= M) return;
+  const double xi = cx[i], yi = cy[i], zi = cz[i], h = ch[i];
+  const double maxdis = row_maxdis(dist, M, i, lm);
+  double R[45];
+#pragma unroll
+  for (int q = 0; q < 45; ++q) R[q] = 0.0;
+  for (int j = 0; j < lm; ++j) {
+    int m = nbr[(size_t)j * M + i];
+    double w = weight_of(dist[(size_t)j * M + i], maxdis);
+    double r[9];
+    basis_row(pwrap_s(cx[m] - xi, G.len[0], G.bc) / h, pwrap_s(cy[m] - yi, G.len[1], G.bc) / h,
+              pwrap_s(cz[m] - zi, G.len[2], G.bc) / h, r);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) r[k] *= w;
+    givens_add<false>(R, nullptr, r, 0.0);
+  }
+  bool deficient = false;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) deficient |= (R[RI(k, k)] == 0.0);
+  if (deficient) { status[0] = APC_ERR_RANK; status[1] = i; return; }
+  // y = R^-1 R^-T u, u = e3+e4+e5
+  double z[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    double s = (k >= 3 && k <= 5) ? 1.0 : 0.0;
+#pragma unroll
+    for (int q = 0; q < k; ++q) s -= R[RI(q, k)] * z[q];
+    z[k] = s / R[RI(k, k)];
+  }
+  double y[9];
+#pragma unroll
+  for (int k = 8; k >= 0; --k) {
+    double s = z[k];
+#pragma unroll
+    for (int q = k + 1; q < 9; ++q) s -= R[RI(k, q)] * y[q];
+    y[k] = s / R[RI(k, k)];
+  }
+  // coefficients, A1 / A2 rows (solver.cpp:116-172)
+  double p2a = 0.0, p2b = 0.0, p3 = 0.0;
+  const double ih2 = 1.0 / h / h;
+  const bool pinned = pin_row0 && i == 0;
+  for (int j = 0; j < lm; ++j) {
+    int m = nbr[(size_t)j * M + i];
+    double w = weight_of(dist[(size_t)j * M + i], maxdis);
+    double r[9];
+    basis_row(pwrap_s(cx[m] - xi, G.len[0], G.bc) / h, pwrap_s(cy[m] - yi, G.len[1], G.bc) / h,
+              pwrap_s(cz[m] - zi, G.len[2], G.bc) / h, r);
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) s += r[k] * y[k];
+    s *= w * w;
+    sco[(size_t)j * M + i] = s;
+    p2a = p2a - s / 24.0;
+    double a2 = s / h / h;
+    p2b = p2b - a2;
+    if (m >= M) p3 = p3 - bv[m - M] * a2;
+  }
+  double dd1 = p2a + 1;
+  double dd2 = pinned ? 1.0 : p2b;
+  d1[i] = dd1;
+  d2[i] = dd2;
+  f1[i] = 1.0 / 24.0;
+  f2[i] = pinned ? 0.0 : ih2;
+  dinv1[i] = (dd1 == 0.0) ? 1.0 : 1.0 / dd1;
+  dinv2[i] = (dd2 == 0.0) ? 1.0 : 1.0 / dd2;
+  rhs1[i] = 1.0 * cnp[i] / h / h / h * c_dens;
+  pbnd[i] = pinned ? 0.0 : p3;
+}
+
+__global__ void __launch_bounds__(256) k_gradient(
+    int M, int lm, Geo G, double c_field, const double *__restrict__ cx,
+    const double *__restrict__ cy, const double *__restrict__ cz, const double *__restrict__ ch,
+    const int *__restrict__ nbr, const double *__restrict__ dist, const double *__restrict__ phi,
+    const double *__restrict__ bv, double *__restrict__ grad, int *__restrict__ status) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= M) return;
+  const double xi = cx[i], yi = cy[i], zi = cz[i], h = ch[i];
+  const double maxdis = row_maxdis(dist, M, i, lm);
+  const double p4 = phi[i];
+  double R[45], Qtb[9];
+#pragma unroll
+  for (int q = 0; q < 45; ++q) R[q] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) Qtb[q] = 0.0;
+  for (int j = 0; j < lm; ++j) {
+    int m = nbr[(size_t)j * M + i];
+    double w = weight_of(dist[(size_t)j * M + i], maxdis);
+    double r[9];
+    basis_row(pwrap_s(cx[m] - xi, G.len[0], G.bc) / h, pwrap_s(cy[m] - yi, G.len[1], G.bc) / h,
+              pwrap_s(cz[m] - zi, G.len[2], G.bc) / h, r);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) r[k] *= w;
+    double v = (m < M) ? phi[m] : bv[m - M];
+    double rb = w * (v - p4) * c_field;
+    givens_add<true>(R, Qtb, r, rb);
+  }
+  bool deficient = false;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) deficient |= (R[RI(k, k)] == 0.0);
+  if (deficient) { status[0] = APC_ERR_RANK; status[1] = i; return; }
+  double cf[9];
+#pragma unroll
+  for (int k = 8; k >= 0; --k) {
+    double s = Qtb[k];
+#pragma unroll
+    for (int q = k + 1; q < 9; ++q) s -= R[RI(k, q)] * cf[q];
+    cf[k] = s / R[RI(k, k)];
+  }
+  grad[i] = cf[0] / h;
+  grad[(size_t)M + i] = cf[1] / h;
+  grad[2 * (size_t)M + i] = cf[2] / h;
+#pragma unroll
+  for (int q = 3; q < 9; ++q) grad[(size_t)q * M + i] = cf[q] / h / h;
+}
+
+// Boundary values at the boundary nodes (bv_kind 1: uniform sphere).
+__global__ void k_bv(int M, int nb, int kind, double GM, double R0, double ccx, double ccy,
+                     double ccz, const double *__restrict__ cx, const double *__restrict__ cy,
+                     const double *__restrict__ cz, double *__restrict__ bv) {
+  int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= nb) return;
+  double v = 0.0;
+  if (kind == 1) {
+    int k = M + q;
+    double dx = cx[k] - ccx, dy = cy[k] - ccy, dz = cz[k] - ccz;
+    double r = sqrt(dx * dx + dy * dy + dz * dz);
+    v = (r < R0) ? -GM * (3.0 * R0 * R0 - r * r) / (2.0 * R0 * R0 * R0) : -GM / r;
+  }
+  bv[q] = v;
+}
+
+static Geo geo_of(apc_ctx *c) {
+  Geo G;
+  const double *b = c->prm.bbox;
+  G.len[0] = b[1] - b[0];
+  G.len[1] = b[3] - b[2];
+  G.len[2] = b[5] - b[4];
+  G.bc = c->prm.bc;
+  return G;
+}
+
+static int check_status(apc_ctx *c, const char *what) {
+  int st[2] = {0, 0};
+  APC_HIP(c, hipMemcpyAsync(st, c->status_dev.p, sizeof(st), hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  if (st[0]) return fail(c, st[0], "%s: rank-deficient least-squares fit at row %d", what, st[1]);
+  return 0;
+}
+
+int build_stencils(apc_ctx *c) {
+  if (!c->have_nbr) {
+    int r = search_neighbors(c);
+    if (r) return r;
+  }
+  const int M = c->nc + c->nv, lm = c->prm.lm, nb = c->nb;
+  APC_ALLOC(c, c->sco, (size_t)lm * M + 1);
+  APC_ALLOC(c, c->d1, M + 1); APC_ALLOC(c, c->d2, M + 1);
+  APC_ALLOC(c, c->f1, M + 1); APC_ALLOC(c, c->f2, M + 1);
+  APC_ALLOC(c, c->dinv1, M + 1); APC_ALLOC(c, c->dinv2, M + 1);
+  APC_ALLOC(c, c->rhs1, M + 1); APC_ALLOC(c, c->pbnd, M + 1);
+  APC_ALLOC(c, c->bv, nb + 1);
+  APC_ALLOC(c, c->status_dev, 4);
+  APC_HIP(c, hipMemsetAsync(c->status_dev.p, 0, 4 * sizeof(int), c->s));
+  const apc_params &p = c->prm;
+  if (nb > 0)
+    hipLaunchKernelGGL(k_bv, dim3(nblocks(nb, 256)), dim3(256), 0, c->s, M, nb, p.bv_kind,
+                       p.bv_G * p.bv_mass, p.bv_radius, p.bv_center[0], p.bv_center[1],
+                       p.bv_center[2], c->cx.p, c->cy.p, c->cz.p, c->bv.p);
+  int pin = (p.bc == 1 && p.dirichlet_point) ? 1 : 0;
+  {
+    KScope ks(c, "stencil");
+    hipLaunchKernelGGL(k_stencil, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, lm, geo_of(c), pin,
+                       c->c_dens, c->cx.p, c->cy.p, c->cz.p, c->ch.p, c->cnp.p, c->nbr.p,
+                       c->dist.p, c->bv.p, c->sco.p, c->d1.p, c->d2.p, c->f1.p, c->f2.p,
+                       c->dinv1.p, c->dinv2.p, c->rhs1.p, c->pbnd.p, c->status_dev.p);
+  }
+  APC_HIP(c, hipGetLastError());
+  int r = check_status(c, "stencil");
+  if (r) return r;
+  c->have_sys = 1;
+  return 0;
+}
+
+int differentiate(apc_ctx *c) {
+  const int M = c->nc + c->nv, lm = c->prm.lm;
+  APC_ALLOC(c, c->grad, (size_t)9 * M + 1);
+  APC_ALLOC(c, c->status_dev, 4);
+  APC_HIP(c, hipMemsetAsync(c->status_dev.p, 0, 4 * sizeof(int), c->s));
+  {
+    KScope ks(c, "gradient");
+    hipLaunchKernelGGL(k_gradient, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, lm, geo_of(c),
+                       c->c_field, c->cx.p, c->cy.p, c->cz.p, c->ch.p, c->nbr.p, c->dist.p,
+                       c->phi.p, c->bv.p, c->grad.p, c->status_dev.p);
+  }
+  APC_HIP(c, hipGetLastError());
+  int r = check_status(c, "gradient");
+  if (r) return r;
+  c->have_fields = 1;
+  return 0;
+}
+
+}  // namespace apc
+
+using namespace apc;
+
+extern "C" int apc_build_system(apc_ctx *c, int *cols, double *a1, double *a2, double *d1,
+                                double *d2, double *rhs1, double *pbnd) {
+  if (!c) return APC_ERR_ARG;
+  if (!c->have_comp) return fail(c, APC_ERR_ARG, "build_system: no computational particles");
+  int r = build_stencils(c);
+  if (r) return r;
+  const int M = c->nc + c->nv, lm = c->prm.lm;
+  std::vector<int> tn((size_t)M * lm);
+  std::vector<double> ts((size_t)M * lm), h(M), vd1(M), vd2(M), vr(M), vp(M), vf2(M);
+  APC_HIP(c, hipMemcpy(tn.data(), c->nbr.p, 4ull * M * lm, hipMemcpyDeviceToHost));
+  APC_HIP(c, hipMemcpy(ts.data(), c->sco.p, 8ull * M * lm, hipMemcpyDeviceToHost));
+  APC_HIP(c, hipMemcpy(h.data(), c->ch.p, 8ull * M, hipMemcpyDeviceToHost));
+  APC_HIP(c, hipMemcpy(vd1.data(), c->d1.p, 8ull * M, hipMemcpyDeviceToHost));
+  APC_HIP(c, hipMemcpy(vd2.data(), c->d2.p, 8ull * M, hipMemcpyDeviceToHost));
+  APC_HIP(c, hipMemcpy(vr.data(), c->rhs1.p, 8ull * M, hipMemcpyDeviceToHost));
+  APC_HIP(c, hipMemcpy(vp.data(), c->pbnd.p, 8ull * M, hipMemcpyDeviceToHost));
+  APC_HIP(c, hipMemcpy(vf2.data(), c->f2.p, 8ull * M, hipMemcpyDeviceToHost));
+  for (int i = 0; i < M; ++i) {
+    for (int q = 0; q < lm; ++q) {
+      int j = tn[(size_t)q * M + i];
+      double s = ts[(size_t)q * M + i];
+      if (cols) cols[(size_t)i * lm + q] = j;
+      if (a1) a1[(size_t)i * lm + q] = (j < M) ? s / 24.0 : 0.0;
+      if (a2) a2[(size_t)i * lm + q] = (vf2[i] == 0.0 || j >= M) ? 0.0 : s / h[i] / h[i];
+    }
+    if (d1) d1[i] = vd1[i];
+    if (d2) d2[i] = vd2[i];
+    if (rhs1) rhs1[i] = vr[i];
+    if (pbnd) pbnd[i] = vp[i];
+  }
+  return 0;
+}

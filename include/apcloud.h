@@ -1,0 +1,171 @@
+/*
+ * apcloud.h — C ABI of the MI355X-native AP-Cloud solve-and-move path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status
+ * (APC_OK or a negative APC_ERR_* code; apc_last_error() gives the text).  No
+ * exception crosses the ABI.  A context owns all device memory and one HIP
+ * stream; caller pointers are read or written only during the call.  One host
+ * thread per context.
+ *
+ * The reference has no C ABI: its "operator API" is the C++ class surface of
+ * Octree / Solver / Mover / Controller.  Each function below names the
+ * reference member (file:line) whose work it performs; the C++ classes in
+ * ap-cloud-cosmology_amd/host/ keep the reference's signatures and call these.
+ */
+#ifndef APCLOUD_H
+#define APCLOUD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  APC_OK = 0,
+  APC_ERR_ARG = -1,          /* bad argument / state */
+  APC_ERR_HIP = -2,          /* HIP runtime error */
+  APC_ERR_NOMEM = -3,        /* device allocation failed */
+  APC_ERR_NEIGH_MANY = -4,   /* solver.cpp:535-539 "Too many neighbours in an octant" */
+  APC_ERR_NEIGH_FEW = -5,    /* solver.cpp:544-550 "Not enough neighbours" */
+  APC_ERR_NEIGH_INVALID = -6,/* solver.cpp:511-518 "Invalid neighbour" */
+  APC_ERR_RANK = -7,         /* solver.cpp:107-115,343-348 rank-deficient WLSQ */
+  APC_ERR_BALANCE = -8,      /* 2:1 balance violated while emitting Neigh24 (internal) */
+  APC_ERR_UNSUPPORTED = -9,  /* input outside what this build implements */
+  APC_ERR_NO_DEVICE = -10
+};
+
+/* Parameters of one simulation box (the .param tags of parameter.cpp:45-149
+ * that the path reads, plus the optional tags this build adds). */
+typedef struct {
+  double bbox[6];          /* BoundingBox_min_x, _max_x, _min_y, _max_y, _min_z, _max_z */
+  int depth;               /* MaxOctreeLevel (1..21) */
+  int bc;                  /* Periodic: 1 periodic, 0 non-periodic */
+  double c;                /* CellSizeParameter */
+  int order;               /* criterion order, reference mover.cpp:86 passes 2 */
+  int mindepth;            /* reference mover.cpp:86 passes 0 */
+  int maxdepth;            /* reference mover.cpp:86 passes MaxOctreeLevel */
+  int leaf_capacity;       /* >0: open a cell iff n_p > leaf_capacity (added tag); 0: n*h^(2o-2) > c */
+  int lm;                  /* NumberOfNeighbors */
+  int dirichlet_point;     /* solver.cpp:17: pin row 0 of the Laplacian when periodic */
+  int boundary_at_faces;   /* added tag: 0 = reference (+-1, octree.cpp:2772...), 1 = box faces */
+  int bv_kind;             /* added tag: 0 = reference bv=0, 1 = uniform-sphere potential */
+  double bv_mass, bv_radius, bv_center[3], bv_G;
+} apc_params;
+
+/* PETSc KSP options honoured by the GMRES (-ksp_rtol, -ksp_atol,
+ * -ksp_divtol, -ksp_max_it, -ksp_gmres_restart, -ksp_monitor). */
+typedef struct {
+  double rtol, atol, dtol;
+  int max_it, restart;
+  int monitor;             /* print "%3d KSP Residual norm %14.12e" like -ksp_monitor */
+  int precision;           /* 0: FP64 GMRES; 1: FP32 inner GMRES in FP64 iterative refinement */
+} apc_ksp_opts;
+
+typedef struct {
+  int its_rho, reason_rho; double rnorm_rho;
+  int its_phi, reason_phi; double rnorm_phi;
+  int refine_rho, refine_phi;   /* outer refinement sweeps (precision form) */
+} apc_solve_info;
+
+/* Host view of the computational particles (particle.h:23-53), filled by
+ * apc_get_comp.  Arrays are caller-allocated with the sizes from
+ * apc_get_comp_sizes; NULL members are skipped. */
+typedef struct {
+  double *x, *y, *z, *cell_length;
+  int *np, *first, *type, *first_neigh, *number_neigh, *neigh24;
+  double *density, *potential;
+  double *p_x, *p_y, *p_z, *p_xx, *p_yy, *p_zz, *p_xy, *p_xz, *p_yz;
+} apc_comp_view;
+
+typedef struct apc_ctx apc_ctx;
+
+const char *apc_last_error(const apc_ctx *ctx);
+const char *apc_version(void);
+int apc_device_count(int *n);
+
+/* Octree::Octree + Solver::Solver + Mover::Mover (octree.h:224,
+ * solver.h:53, mover.h:31): allocate a context on the current HIP device. */
+int apc_create(const apc_params *p, apc_ctx **out);
+int apc_destroy(apc_ctx *ctx);
+
+/* initialization.cpp:41-91 hands the path a Physical_particle; this copies
+ * the caller's SoA arrays (particle.h:5-21) to HBM. */
+int apc_set_particles(apc_ctx *ctx, int n, const double *x, const double *y, const double *z,
+                      const double *vx, const double *vy, const double *vz);
+/* Copies positions, velocities and accelerations back in the caller's
+ * original particle order.  NULL pointers are skipped. */
+int apc_get_particles(apc_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy,
+                      double *vz, double *ax, double *ay, double *az);
+
+/* Octree::buildOctree (octree.cpp:111-494): Morton keys (computeKey64,
+ * octree.cpp:676-852), stable device radix sort and the move of the particle
+ * arrays into key order.  The sorted (key, original index) pairs are
+ * readable with apc_get_sorted. */
+int apc_octree_build(apc_ctx *ctx);
+int apc_get_sorted(apc_ctx *ctx, uint64_t *keys, int *index);
+/* The octree level arrays of buildOctree for one level: node count, and per
+ * node the masked key, first sorted particle and particle count
+ * (octree.cpp:177-313).  Pass NULL arrays to get the count only. */
+int apc_octree_level(apc_ctx *ctx, int level, int *count, uint64_t *key, int *first, int *np);
+
+/* Octree::select_computational_particles_2to1_3d (octree.cpp:1050-3001):
+ * top-down refinement, 2:1 face balance and Neigh24.  *S_out receives
+ * nc+nv+nb.  Output order: computational leaves in the reference's order,
+ * vacuum leaves by (level, Morton), boundary nodes in leaf order. */
+int apc_select(apc_ctx *ctx, double c, int order, int mindepth, int maxdepth, int *S_out);
+int apc_get_comp_sizes(apc_ctx *ctx, int *nc, int *nv, int *nb, int *nneigh);
+int apc_get_comp(apc_ctx *ctx, const apc_comp_view *view);
+
+/* Solver::searchneighboroctant (solver.cpp:456-570) for every computational
+ * and vacuum node: lm neighbour indices and distances, row-major (M x lm). */
+int apc_neighbors(apc_ctx *ctx, int *nbr, double *dist);
+
+/* Solver::Set_Constant_For_{Density_Estimator,Potential_Solver,Field_Solver}
+ * (solver.h:57-59). */
+int apc_solver_set_constants(apc_ctx *ctx, double c_dens, double c_pot, double c_field);
+/* Solver::Build_Linear_System (solver.cpp:24-180).  Optional export of the
+ * rows (M x lm, neighbour order): cols, A1 and A2 off-diagonal values (both
+ * 0 on boundary columns, whose A2 weights go into the boundary RHS term
+ * pbnd), diagonals, rhs1 and pbnd. */
+int apc_build_system(apc_ctx *ctx, int *cols, double *a1, double *a2, double *d1, double *d2,
+                     double *rhs1, double *pbnd);
+/* Solver::solve (solver.cpp:382-454): system build, Solve_Density,
+ * Solve_Potential, Differentiate. */
+int apc_solver_solve(apc_ctx *ctx, const apc_ksp_opts *rho, const apc_ksp_opts *phi,
+                     apc_solve_info *info);
+/* Solver::Differentiate alone on a caller-supplied potential (M values). */
+int apc_differentiate(apc_ctx *ctx, const double *phi);
+/* Node fields after a solve: rho, phi (M), g (M x 3), H (M x 6: xx yy zz xy xz yz). */
+int apc_get_fields(apc_ctx *ctx, double *rho, double *phi, double *g, double *H);
+/* Overwrites g and H (same layout) — lets tests drive the mover alone. */
+int apc_set_gradients(apc_ctx *ctx, const double *g, const double *H);
+
+/* Mover::move_velocity (mover.cpp:51-89) with the interpolation of
+ * octree.cpp:3003-3031: a = g + H dx, v += a Cv dt; *max_dv = max |a| Cv dt. */
+int apc_interp_kick(apc_ctx *ctx, double Cv, double dt, double *max_dv);
+/* Mover::move_position (mover.cpp:91-153): x = wrap(x + v Cx dt);
+ * *max_dx = max |v| Cx dt. */
+int apc_drift(apc_ctx *ctx, double Cx, double dt, double *max_dx);
+/* max over computational particles of n_p / h^3 (controller.cpp:146-162). */
+int apc_max_density(apc_ctx *ctx, double *max_density);
+
+/* One whole solve-and-move step (controller.cpp:119-208 without output):
+ * octree + selection, solve with the given coefficients, kick with Cv, drift
+ * with Cx.  Reports max changes and, if timing != NULL, per-phase seconds
+ * [select, solve, move]. */
+int apc_step(apc_ctx *ctx, double c_dens, double c_pot, double c_field, double Cv, double Cx,
+             double dt, const apc_ksp_opts *rho, const apc_ksp_opts *phi, double *max_dv,
+             double *max_dx, apc_solve_info *info, double *timing);
+
+/* Per-kernel device time of the last calls (ms, averaged per launch) for
+ * the roofline report; names are NUL-separated.  Returns the count. */
+int apc_kernel_times(apc_ctx *ctx, char *names, int names_len, double *avg_ms, int *launches,
+                     int cap);
+int apc_reset_kernel_times(apc_ctx *ctx);
+int apc_set_timing(apc_ctx *ctx, int on);
+int apc_sync(apc_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
