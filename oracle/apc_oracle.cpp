@@ -350,6 +350,8 @@ struct Selector {
     j += 8;
   }
 
+  int leaf_capacity = 0;
+
   int run(Comp &out, double c, int order, int mindepth, int maxdepth, bool faces) {
     ensure(1024);
     int i = 0;
@@ -378,7 +380,8 @@ struct Selector {
       double h = h0;
       for (int i1 = 0; i1 < level[i]; ++i1) h = 0.5 * h;
       for (int i1 = 0; i1 < (2 * order - 2); ++i1) temp = temp * h;
-      if (((temp > c) || (level[i] < mindepth)) && (level[i] < maxdepth)) {
+      bool crit = (leaf_capacity > 0) ? (n_p[i] > leaf_capacity) : (temp > c);
+      if ((crit || (level[i] < mindepth)) && (level[i] < maxdepth)) {
         open(i, false);
       } else {
         leaf[i] = 1;
@@ -1169,6 +1172,7 @@ int orc_select(OrcSim *s, double c, int order, int mindepth, int maxdepth) {
   if (!s->have_tree) return -1;
   if (maxdepth > s->p.depth) return -1;
   Selector sel(s->T, s->x.data(), s->y.data(), s->z.data(), s->p.bc);
+  sel.leaf_capacity = s->p.leaf_capacity;
   int S = sel.run(s->cp, c, order, mindepth, maxdepth, s->p.boundary_at_faces != 0);
   s->have_comp = true;
   return S;
@@ -1254,9 +1258,9 @@ int orc_solve(OrcSim *s, OrcSolveInfo *info) {
 
 int orc_comp_fields(OrcSim *s, double *rho, double *phi, double *g, double *H) {
   const int M = s->cp.nc + s->cp.nv;
-  if ((int)s->rho.size() < M || (int)s->g.size() < 3 * M) return -1;
-  if (rho) std::memcpy(rho, s->rho.data(), sizeof(double) * M);
-  if (phi) std::memcpy(phi, s->phi.data(), sizeof(double) * M);
+  if ((int)s->g.size() < 3 * M) return -1;
+  if (rho && (int)s->rho.size() >= M) std::memcpy(rho, s->rho.data(), sizeof(double) * M);
+  if (phi && (int)s->phi.size() >= M) std::memcpy(phi, s->phi.data(), sizeof(double) * M);
   if (g) std::memcpy(g, s->g.data(), sizeof(double) * 3 * M);
   if (H) std::memcpy(H, s->H.data(), sizeof(double) * 6 * M);
   return 0;

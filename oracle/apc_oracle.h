@@ -37,6 +37,8 @@ typedef struct {
   /* GMRES (PETSc semantics) for rho and phi */
   double rtol, atol, dtol; int max_it, restart;
   int nthreads;            /* OpenMP threads for the parallel loops (0 = default) */
+  int leaf_capacity;       /* >0: open a cell iff n_p > leaf_capacity (the build's added
+                              criterion); 0: the reference's n*h^(2o-2) > c */
 } OrcParams;
 
 typedef struct {

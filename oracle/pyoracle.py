@@ -25,7 +25,7 @@ class OrcParams(C.Structure):
         ("unit_mass", C.c_double), ("unit_length", C.c_double),
         ("mass_per_particle", C.c_double), ("adaptive_dt", C.c_int), ("rtol", C.c_double),
         ("atol", C.c_double), ("dtol", C.c_double), ("max_it", C.c_int), ("restart", C.c_int),
-        ("nthreads", C.c_int),
+        ("nthreads", C.c_int), ("leaf_capacity", C.c_int),
     ]
 
 
@@ -117,6 +117,7 @@ def make_params(bbox=(0, 1, 0, 1, 0, 1), depth=20, bc=1, c=1.0, lm=17, **kw):
     p.max_it = kw.get("max_it", 10000)
     p.restart = kw.get("restart", 30)
     p.nthreads = kw.get("nthreads", 0)
+    p.leaf_capacity = kw.get("leaf_capacity", 0)
     return p
 
 
