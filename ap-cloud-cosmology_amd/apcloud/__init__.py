@@ -27,7 +27,8 @@ EXPORTS = [
     "apc_last_error", "apc_version", "apc_device_count", "apc_create", "apc_destroy",
     "apc_set_particles", "apc_get_particles", "apc_octree_build", "apc_get_sorted",
     "apc_octree_level", "apc_select", "apc_get_comp_sizes", "apc_get_comp", "apc_neighbors",
-    "apc_solver_set_constants", "apc_build_system", "apc_solver_solve", "apc_differentiate",
+    "apc_solver_set_neighbors", "apc_solver_set_constants", "apc_build_system",
+    "apc_solver_solve", "apc_differentiate",
     "apc_get_fields", "apc_set_gradients", "apc_interp_kick", "apc_drift", "apc_max_density",
     "apc_step", "apc_kernel_times", "apc_reset_kernel_times", "apc_set_timing", "apc_sync",
 ]
@@ -87,6 +88,7 @@ def lib():
         L.apc_get_comp_sizes.argtypes = [C.c_void_p, _I, _I, _I, _I]
         L.apc_get_comp.argtypes = [C.c_void_p, C.POINTER(CompView)]
         L.apc_neighbors.argtypes = [C.c_void_p, _I, _D]
+        L.apc_solver_set_neighbors.argtypes = [C.c_void_p, C.c_int]
         L.apc_solver_set_constants.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
         L.apc_build_system.argtypes = [C.c_void_p, _I] + [_D] * 6
         L.apc_solver_solve.argtypes = [C.c_void_p, C.POINTER(KspOpts), C.POINTER(KspOpts),

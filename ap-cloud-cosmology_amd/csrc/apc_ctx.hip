@@ -185,6 +185,16 @@ int apc_get_particles(apc_ctx *c, double *x, double *y, double *z, double *vx, d
   return APC_OK;
 }
 
+int apc_solver_set_neighbors(apc_ctx *c, int lm) {
+  if (!c) return APC_ERR_ARG;
+  if (lm < 9) return fail(c, APC_ERR_ARG, "NumberOfNeighbors must be >= 9 (got %d)", lm);
+  if (lm != c->prm.lm) {
+    c->prm.lm = lm;
+    c->have_nbr = c->have_sys = c->have_fields = 0;
+  }
+  return APC_OK;
+}
+
 int apc_solver_set_constants(apc_ctx *c, double c_dens, double c_pot, double c_field) {
   if (!c) return APC_ERR_ARG;
   c->c_dens = c_dens;

@@ -120,6 +120,8 @@ int apc_get_comp(apc_ctx *ctx, const apc_comp_view *view);
  * and vacuum node: lm neighbour indices and distances, row-major (M x lm). */
 int apc_neighbors(apc_ctx *ctx, int *nbr, double *dist);
 
+/* Solver::Set_NumberOfGFDNeighbors (solver.h:56): lm >= 9. */
+int apc_solver_set_neighbors(apc_ctx *ctx, int lm);
 /* Solver::Set_Constant_For_{Density_Estimator,Potential_Solver,Field_Solver}
  * (solver.h:57-59). */
 int apc_solver_set_constants(apc_ctx *ctx, double c_dens, double c_pot, double c_field);
