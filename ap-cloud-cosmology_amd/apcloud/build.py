@@ -56,8 +56,21 @@ def sources():
     hip = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
     host = []
     if os.path.isdir(HOST):
-        host = sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp"))
+        host = sorted(os.path.join(HOST, f) for f in os.listdir(HOST)
+                      if f.endswith(".cpp") and f != MAIN)
     return hip + host
+
+
+MAIN = "apcloud_main.cpp"
+EXE = os.path.join(PKG, "apcloud")
+
+
+def build_driver():
+    """The `apcloud <file.param> [-ksp_* ...]` executable over libapcloud.so."""
+    obj = _compile(os.path.join(HOST, MAIN))
+    tmp = EXE + ".tmp"
+    _run([HIPCC, "-o", tmp, obj, f"-L{PKG}", "-lapcloud", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, EXE)
 
 
 def build_library(jobs=None, verbose=True):
@@ -69,8 +82,9 @@ def build_library(jobs=None, verbose=True):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, LIB)
+    build_driver()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", EXE)
     return LIB
 
 

@@ -99,3 +99,42 @@ def clustered(n, seed, n_clusters=4, scale=0.01, fraction=0.5):
     x, y, z, _, _, _ = plummer_clusters(x, y, z, z0, z0, z0, seed, n_spheres=n_clusters,
                                         scale=scale, fraction=fraction)
     return x, y, z
+
+
+def write_vtk(path, x, y, z, vx, vy, vz, ids=None, unit_velocity=1.0):
+    """Writes particles in the reference's IC layout (initialization.cpp:34-83):
+    4 header lines, POINTS, then v_x, v_y, v_z and ID blocks, velocities
+    multiplied by UnitVelocity_in_cm_per_s (the reader divides by it).  Values
+    are written with 17 significant digits so they read back exactly."""
+    n = len(x)
+    ids = np.arange(n) if ids is None else ids
+    with open(path, "w") as f:
+        f.write("# vtk DataFile Version 3.0\nsynthetic initial conditions\nASCII\nDATASET POLYDATA\n")
+        f.write("POINTS %d double\n" % n)
+        np.savetxt(f, np.stack([x, y, z], 1), fmt="%.17g")
+        f.write("\nPOINT_DATA %d\n" % n)
+        for name, v in (("v_x", vx), ("v_y", vy), ("v_z", vz)):
+            if name != "v_x":
+                f.write("\n")
+            f.write("SCALARS %s double\nLOOKUP_TABLE default\n" % name)
+            np.savetxt(f, np.asarray(v) * unit_velocity, fmt="%.17g")
+        f.write("\nSCALARS ID int\nLOOKUP_TABLE default\n")
+        np.savetxt(f, ids, fmt="%d")
+
+
+PARAM_TAGS = [
+    "Omega", "OmegaLambda", "Redshift", "HubbleParam", "GravitionalConstant", "UnitLength_in_cm",
+    "UnitMass_in_g", "UnitVelocity_in_cm_per_s", "ICFile", "ICFileType", "FileBase", "OutputFile",
+    "BoundingBox_min_x", "BoundingBox_max_x", "BoundingBox_min_y", "BoundingBox_max_y",
+    "BoundingBox_min_z", "BoundingBox_max_z", "Periodic", "DeltaA", "EndRedShift",
+    "MassPerParticle", "MaxOctreeLevel", "CellSizeParameter", "NumberOfNeighbors",
+    "DensityEstimator",
+]
+
+
+def write_param(path, **tags):
+    """A .param file in the reference's Tag value %comment format."""
+    with open(path, "w") as f:
+        f.write("% written by apcloud.ic.write_param\n")
+        for k, v in tags.items():
+            f.write("%-28s %s\n" % (k, repr(v) if isinstance(v, float) else v))

@@ -30,6 +30,7 @@ double BoundaryMass = 1.0, BoundaryRadius = 0.5;
 double BoundaryCenter_x = 0.5, BoundaryCenter_y = 0.5, BoundaryCenter_z = 0.5;
 int WriteInterval = 10;
 int GmresPrecision = 0;
+char FinalStateFile[200] = "";
 
 namespace {
 enum Kind { KFLOAT, KSTRING, KINT };
@@ -84,6 +85,7 @@ int read_parameterfile(const char *fname) {
       {"BoundaryCenter_z", &BoundaryCenter_z, KFLOAT, false, false},
       {"WriteInterval", &WriteInterval, KINT, false, false},
       {"GmresPrecision", &GmresPrecision, KINT, false, false},
+      {"FinalStateFile", FinalStateFile, KSTRING, false, false},
   };
   int errorFlag = 0;
   FILE *fd = fopen(fname, "r");

@@ -32,5 +32,6 @@ extern int BoundaryValue;     // 0: bv = 0 (reference), 1: uniform-sphere potent
 extern double BoundaryMass, BoundaryRadius, BoundaryCenter_x, BoundaryCenter_y, BoundaryCenter_z;
 extern int WriteInterval;     // reference: 10 (test.cpp:66); 0 disables output
 extern int GmresPrecision;    // 0 FP64, 1 FP32 inner / FP64 refinement
+extern char FinalStateFile[200];  // raw FP64 dump of x,y,z,v,a after the run (empty: none)
 
 #endif

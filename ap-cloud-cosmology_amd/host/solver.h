@@ -24,6 +24,7 @@ class Solver {
   // -ksp_* options parsed from argv (both solves, like a global
   // KSPSetFromOptions); -ksp_gmres_precision selects the mixed-precision form
   apc_ksp_opts ksp_options() const { return opts; }
+  void set_precision(int p) { opts.precision = p; }
   apc_solve_info last_info() const { return info; }
   int export_fields = 1;  // fill Density/Potential/p_* of the computational particles
 
