@@ -31,6 +31,7 @@ EXPORTS = [
     "apc_solver_solve", "apc_differentiate",
     "apc_get_fields", "apc_set_gradients", "apc_interp_kick", "apc_drift", "apc_max_density",
     "apc_step", "apc_kernel_times", "apc_reset_kernel_times", "apc_set_timing", "apc_sync",
+    "apc_stream_copy",
 ]
 
 
@@ -53,7 +54,8 @@ class KspOpts(C.Structure):
 class SolveInfo(C.Structure):
     _fields_ = [("its_rho", C.c_int), ("reason_rho", C.c_int), ("rnorm_rho", C.c_double),
                 ("its_phi", C.c_int), ("reason_phi", C.c_int), ("rnorm_phi", C.c_double),
-                ("refine_rho", C.c_int), ("refine_phi", C.c_int)]
+                ("refine_rho", C.c_int), ("refine_phi", C.c_int), ("nc", C.c_int),
+                ("nv", C.c_int), ("nb", C.c_int)]
 
 
 class CompView(C.Structure):
@@ -101,7 +103,8 @@ def lib():
         L.apc_max_density.argtypes = [C.c_void_p, _D]
         L.apc_step.argtypes = [C.c_void_p] + [C.c_double] * 6 + [C.POINTER(KspOpts)] * 2 + \
             [_D, _D, C.POINTER(SolveInfo), _D]
-        L.apc_kernel_times.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _D, _I, C.c_int]
+        L.apc_kernel_times.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _D, _I, _D, C.c_int]
+        L.apc_stream_copy.argtypes = [C.c_void_p, C.c_double, C.c_int, _D]
         L.apc_reset_kernel_times.argtypes = [C.c_void_p]
         L.apc_set_timing.argtypes = [C.c_void_p, C.c_int]
         L.apc_sync.argtypes = [C.c_void_p]
@@ -335,9 +338,15 @@ class Context:
         buf = C.create_string_buffer(8192)
         avg = np.zeros(128)
         cnt = np.zeros(128, np.int32)
-        n = self.L.apc_kernel_times(self.h, buf, 8192, _p(avg), _p(cnt, _I), 128)
+        byt = np.zeros(128)
+        n = self.L.apc_kernel_times(self.h, buf, 8192, _p(avg), _p(cnt, _I), _p(byt), 128)
         names = buf.raw.split(b"\0")[:n]
-        return {nm.decode(): (avg[q], int(cnt[q])) for q, nm in enumerate(names)}
+        return {nm.decode(): (avg[q], int(cnt[q]), byt[q]) for q, nm in enumerate(names)}
+
+    def stream_copy(self, nbytes, reps=10):
+        out = C.c_double()
+        self._chk(self.L.apc_stream_copy(self.h, float(nbytes), reps, C.byref(out)), "apc_stream_copy")
+        return out.value
 
     def sync(self):
         self._chk(self.L.apc_sync(self.h), "apc_sync")

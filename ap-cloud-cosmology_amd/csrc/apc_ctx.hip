@@ -32,20 +32,33 @@ int ktimer_index(apc_ctx *c, const char *name) {
   return (int)c->timers.size() - 1;
 }
 
+static hipEvent_t pooled_event(apc_ctx *c) {
+  if (!c->evpool.empty()) {
+    hipEvent_t e = c->evpool.back();
+    c->evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
 void ktime_begin(apc_ctx *c, int idx) {
-  hipEvent_t e;
-  if (hipEventCreate(&e) != hipSuccess) return;
+  hipEvent_t e = pooled_event(c);
+  if (!e) return;
   hipEventRecord(e, c->s);
   c->timers[idx].ev.push_back(e);
 }
 
 void ktime_end(apc_ctx *c, int idx) {
-  hipEvent_t e;
-  if (hipEventCreate(&e) != hipSuccess) return;
+  hipEvent_t e = pooled_event(c);
+  if (!e) return;
   hipEventRecord(e, c->s);
   c->timers[idx].ev.push_back(e);
   c->timers[idx].launches++;
 }
+
+void ktime_bytes(apc_ctx *c, int idx, double bytes) { c->timers[idx].bytes += bytes; }
 
 static void fold_timers(apc_ctx *c) {
   hipStreamSynchronize(c->s);
@@ -54,7 +67,7 @@ static void fold_timers(apc_ctx *c) {
       float ms = 0;
       if (hipEventElapsedTime(&ms, t.ev[q], t.ev[q + 1]) == hipSuccess) t.total_ms += ms;
     }
-    for (auto e : t.ev) hipEventDestroy(e);
+    for (auto e : t.ev) c->evpool.push_back(e);
     t.ev.clear();
   }
 }
@@ -63,6 +76,13 @@ __global__ void k_scatter_back(int n, const int *__restrict__ id, const double *
                                double *__restrict__ out) {
   int s = blockIdx.x * 256 + threadIdx.x;
   if (s < n) out[id[s]] = a[s];
+}
+
+// plain streaming copy (HBM bandwidth reference)
+__global__ void __launch_bounds__(256) k_copy2(size_t n, const double2 *__restrict__ a,
+                                               double2 *__restrict__ b) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+    b[i] = a[i];
 }
 
 __global__ void k_iota(int n, int *__restrict__ a) {
@@ -120,6 +140,7 @@ int apc_destroy(apc_ctx *c) {
   hipStreamSynchronize(c->s);
   for (auto &t : c->timers)
     for (auto e : t.ev) hipEventDestroy(e);
+  for (auto e : c->evpool) hipEventDestroy(e);
   free_solve_state(c);
   apc::DBuf<double> *ds[] = {&c->x, &c->y, &c->z, &c->vx, &c->vy, &c->vz, &c->ax, &c->ay, &c->az,
                              &c->t0, &c->t1, &c->t2, &c->t3, &c->t4, &c->t5, &c->ncx, &c->ncy,
@@ -309,11 +330,12 @@ int apc_set_timing(apc_ctx *c, int on) {
 int apc_reset_kernel_times(apc_ctx *c) {
   if (!c) return APC_ERR_ARG;
   fold_timers(c);
-  for (auto &t : c->timers) { t.total_ms = 0; t.launches = 0; }
+  for (auto &t : c->timers) { t.total_ms = 0; t.launches = 0; t.bytes = 0; }
   return APC_OK;
 }
 
-int apc_kernel_times(apc_ctx *c, char *names, int names_len, double *avg_ms, int *launches, int cap) {
+int apc_kernel_times(apc_ctx *c, char *names, int names_len, double *avg_ms, int *launches,
+                     double *avg_bytes, int cap) {
   if (!c) return APC_ERR_ARG;
   fold_timers(c);
   int n = 0, pos = 0;
@@ -326,10 +348,42 @@ int apc_kernel_times(apc_ctx *c, char *names, int names_len, double *avg_ms, int
       pos += L;
     }
     if (avg_ms) avg_ms[n] = t.launches ? t.total_ms / t.launches : 0.0;
+    if (avg_bytes) avg_bytes[n] = t.launches ? t.bytes / t.launches : 0.0;
     if (launches) launches[n] = t.launches;
     ++n;
   }
   return n;
+}
+
+int apc_stream_copy(apc_ctx *c, double bytes, int reps, double *gbs) {
+  if (!c || bytes < 1024 || reps < 1 || !gbs) return APC_ERR_ARG;
+  size_t n = (size_t)(bytes / 16);  // double2 elements
+  apc::DBuf<double> a, b;
+  if (a.ensure(2 * n) != hipSuccess || b.ensure(2 * n) != hipSuccess) {
+    a.release(); b.release();
+    return apc::fail(c, APC_ERR_NOMEM, "stream_copy");
+  }
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 65535u * 16u);
+  hipLaunchKernelGGL(k_copy2, dim3(nb), dim3(256), 0, c->s, n, (const double2 *)a.p, (double2 *)b.p);
+  hipEventRecord(e0, c->s);
+  for (int r = 0; r < reps; ++r) {
+    KScope ks(c, "stream_copy", 2.0 * 16.0 * n);
+    hipLaunchKernelGGL(k_copy2, dim3(nb), dim3(256), 0, c->s, n, (const double2 *)a.p, (double2 *)b.p);
+  }
+  hipEventRecord(e1, c->s);
+  hipError_t e = hipEventSynchronize(e1);
+  float ms = 0;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  a.release();
+  b.release();
+  if (e != hipSuccess) return apc::hip_fail(c, e, "stream_copy");
+  *gbs = 2.0 * 16.0 * n * reps / (ms * 1e-3) / 1e9;
+  return APC_OK;
 }
 
 int apc_sync(apc_ctx *c) {

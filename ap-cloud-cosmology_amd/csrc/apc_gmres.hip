@@ -2,158 +2,327 @@
 //
 // Semantics follow PETSc's KSPSolve_GMRES: left Jacobi preconditioning, zero
 // initial guess, convergence on the preconditioned residual with
-// KSPConvergedDefault (rtol relative to the first residual norm, atol,
-// dtol), classical Gram-Schmidt (PETSc's default orthogonalisation),
-// Hessenberg updated with Givens rotations, true residual at each restart.
+// KSPConvergedDefault (rtol relative to the first residual norm, atol, dtol),
+// classical Gram-Schmidt (PETSc's default orthogonalisation), Givens update of
+// the Hessenberg matrix, true residual at every restart, -ksp_max_it counted
+// across restarts.
 //
-// Device side: one SpMV kernel over the slot-major GFD matrix
-// (A x)_i = d_i x_i + f_i sum_j s_ij x_nbr(i,j), a multi-vector dot kernel
-// and a fused "subtract projections + norm" kernel per Arnoldi step, all
-// with fixed-order block reductions (deterministic).  The (j+2)-value
-// Hessenberg column comes back to the host, which runs the Givens update.
+// Device-driven: an Arnoldi step is three launches — preconditioned SpMV,
+// multi-dot (h = V^T w), and "subtract projections + norm + Hessenberg
+// update" — whose reductions finish in the last block to arrive (fixed
+// summation order, deterministic).  The Givens rotations and the stopping
+// test run on the device, so the host only launches and checks a status word
+// every few steps.  Basis vectors are kept unnormalised with scale factors
+// sig[q] (v_q = sig[q] w_q), which removes the normalisation pass.
+//
+// Precision form: the same cycle with an FP32 basis and FP32 copies of the
+// matrix, inside FP64 iterative refinement — every restart computes the true
+// residual with the FP64 operator, the FP32 cycle solves for the correction
+// and the correction is accumulated into the FP64 solution.
 #include "apc_internal.h"
 #include "apc_prims.h"
 
 namespace apc {
 
-struct PMat {
+template <class T>
+struct Op {
   const int *nbr;
-  const double *s, *d, *f, *dinv;
+  const T *s, *d, *f, *dinv;
   int M, lm;
 };
 
+struct GState {
+  int it, its, reason, cycle_done;
+  int m, max_it, fp32, monitor;
+  double res, rnorm0, ttol, cyc_res0;
+  double rtol, atol, dtol;
+  unsigned counter;
+};
+
+enum { KSP_CONVERGED_RTOL = 2, KSP_CONVERGED_ATOL = 3, KSP_DIVERGED_NULL = -2,
+       KSP_DIVERGED_ITS = -3, KSP_DIVERGED_DTOL = -4, KSP_DIVERGED_BREAKDOWN = -5,
+       KSP_DIVERGED_NANORINF = -9 };
+
+struct GDev {
+  GState *st;
+  double *HH, *cc, *ss, *grs, *sig, *h, *coef, *part, *hist;
+};
+
 struct SolveState {
-  DBuf<double> V, part, hd, work;
-  double *hpin = nullptr;
-  int hcap = 0;
+  DBuf<double> V64, dscratch, r64, part;
+  DBuf<float> V32, s32, d32, f32, dinv32;
+  DBuf<double> hess;  // HH, cc, ss, grs, sig, h, coef, hist
+  DBuf<GState> st;
+  GState *hst = nullptr;  // pinned mirror
+  int mcap = 0, histcap = 0;
   ~SolveState() {
-    V.release(); part.release(); hd.release(); work.release();
-    if (hpin) hipHostFree(hpin);
+    V64.release(); dscratch.release(); r64.release(); part.release();
+    V32.release(); s32.release(); d32.release(); f32.release(); dinv32.release();
+    hess.release(); st.release();
+    if (hst) hipHostFree(hst);
   }
 };
 
-constexpr int RED_BLOCKS = 1024;
+__device__ inline int converged(GState *st, int nit, double rn) {
+  if (nit == 0) {
+    st->rnorm0 = rn;
+    st->ttol = fmax(st->rtol * rn, st->atol);
+  }
+  if (isnan(rn) || isinf(rn)) return KSP_DIVERGED_NANORINF;
+  if (rn <= st->ttol) return (rn < st->atol) ? KSP_CONVERGED_ATOL : KSP_CONVERGED_RTOL;
+  if (rn >= st->dtol * st->rnorm0) return KSP_DIVERGED_DTOL;
+  return 0;
+}
 
-__device__ inline double row_apply(const PMat &A, const double *__restrict__ x, int i) {
-  double acc = 0.0;
+__device__ inline bool halted(const GState *st) { return st->reason != 0 || st->cycle_done != 0; }
+
+// Fixed-order sum of nb partials by the calling block; result to all threads.
+__device__ inline double sum_partials(const double *part, int nb, double *ws) {
+  double a = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) a += part[b];
+  return block_sum256(a, ws);
+}
+
+// Last block to arrive?  (partials written before the ticket)
+__device__ inline bool last_block(unsigned *counter, int *flag) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) *flag = (atomicAdd(counter, 1u) == gridDim.x - 1);
+  __syncthreads();
+  return *flag != 0;
+}
+
+template <class T>
+__device__ inline double row_apply(const Op<T> &A, const T *__restrict__ x, int i) {
+  T acc = 0;
   for (int j = 0; j < A.lm; ++j) {
     int m = A.nbr[(size_t)j * A.M + i];
     if (m < A.M) acc += A.s[(size_t)j * A.M + i] * x[m];
   }
-  return A.d[i] * x[i] + A.f[i] * acc;
+  return (double)(A.d[i] * x[i] + A.f[i] * acc);
 }
 
-__global__ void __launch_bounds__(256) k_spmv_pc(PMat A, const double *__restrict__ x,
-                                                 double *__restrict__ y) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= A.M) return;
-  y[i] = A.dinv[i] * row_apply(A, x, i);
-}
-
-__global__ void __launch_bounds__(256) k_pc_resid(PMat A, const double *__restrict__ x,
-                                                  const double *__restrict__ b,
-                                                  double *__restrict__ r, int zero_guess) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= A.M) return;
-  r[i] = zero_guess ? A.dinv[i] * b[i] : A.dinv[i] * (b[i] - row_apply(A, x, i));
-}
-
-// part[q*nb + block] = sum over the block's rows of w_i * V_q,i, q < nvec
-__global__ void __launch_bounds__(256) k_mdot(int M, int nvec, const double *__restrict__ V,
-                                              size_t ld, const double *__restrict__ w,
-                                              double *__restrict__ part) {
+// ---- cycle start: v0 = D^-1 (b - A x) in FP64, ||v0||, PETSc's check ----
+template <class T>
+__global__ void __launch_bounds__(256) k_resid(Op<double> A, const double *__restrict__ x,
+                                               const double *__restrict__ b, int zero_guess,
+                                               T *__restrict__ v0, GDev g) {
   __shared__ double ws[4];
-  const int nb = gridDim.x;
-  for (int q0 = 0; q0 < nvec; q0 += 4) {
-    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-    const int nq = min(4, nvec - q0);
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < M; i += (long)nb * 256) {
-      double wi = w[i];
-      a0 += wi * V[(size_t)q0 * ld + i];
-      if (nq > 1) a1 += wi * V[(size_t)(q0 + 1) * ld + i];
-      if (nq > 2) a2 += wi * V[(size_t)(q0 + 2) * ld + i];
-      if (nq > 3) a3 += wi * V[(size_t)(q0 + 3) * ld + i];
-    }
-    double r0 = block_sum256(a0, ws);
-    double r1 = (nq > 1) ? block_sum256(a1, ws) : 0.0;
-    double r2 = (nq > 2) ? block_sum256(a2, ws) : 0.0;
-    double r3 = (nq > 3) ? block_sum256(a3, ws) : 0.0;
-    if (threadIdx.x == 0) {
-      part[(size_t)q0 * nb + blockIdx.x] = r0;
-      if (nq > 1) part[(size_t)(q0 + 1) * nb + blockIdx.x] = r1;
-      if (nq > 2) part[(size_t)(q0 + 2) * nb + blockIdx.x] = r2;
-      if (nq > 3) part[(size_t)(q0 + 3) * nb + blockIdx.x] = r3;
-    }
+  __shared__ int flag;
+  GState *st = g.st;
+  if (st->reason) return;
+  double acc = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < A.M; i += gridDim.x * 256) {
+    double r = zero_guess ? A.dinv[i] * b[i] : A.dinv[i] * (b[i] - row_apply(A, x, i));
+    v0[i] = (T)r;
+    acc += r * r;
+  }
+  acc = block_sum256(acc, ws);
+  if (threadIdx.x == 0) g.part[blockIdx.x] = acc;
+  if (!last_block(&st->counter, &flag)) return;
+  double nrm2 = sum_partials(g.part, gridDim.x, ws);
+  if (threadIdx.x == 0) {
+    st->counter = 0;
+    double res = sqrt(nrm2);
+    st->res = res;
+    st->it = 0;
+    st->cycle_done = 0;
+    st->cyc_res0 = res;
+    g.grs[0] = res;
+    g.sig[0] = (res != 0.0) ? 1.0 / res : 1.0;
+    if (st->monitor) g.hist[st->its] = res;
+    if (res == 0.0) st->reason = KSP_CONVERGED_ATOL;
+    else st->reason = converged(st, st->its, res);
+    if (!st->reason && st->its >= st->max_it) st->cycle_done = 1;
   }
 }
 
-// out[q] = sum_b part[q*nb + b]; one block per q
-__global__ void __launch_bounds__(256) k_reduce_cols(const double *__restrict__ part, int nb,
-                                                     double *__restrict__ out) {
-  __shared__ double ws[4];
-  const int q = blockIdx.x;
-  double a = 0.0;
-  for (int b = threadIdx.x; b < nb; b += 256) a += part[(size_t)q * nb + b];
-  a = block_sum256(a, ws);
-  if (threadIdx.x == 0) out[q] = a;
+// ---- w = sig_j D^-1 A w_j ----
+template <class T>
+__global__ void __launch_bounds__(256) k_spmv(Op<T> A, const T *__restrict__ vj,
+                                              T *__restrict__ w, int j, GDev g) {
+  if (halted(g.st)) return;
+  const T s = (T)g.sig[j];
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= A.M) return;
+  w[i] = (T)(s * (T)(A.dinv[i] * (T)row_apply(A, vj, i)));
 }
 
-// w -= sum_q h_q V_q ; part[block] = sum w^2
-__global__ void __launch_bounds__(256) k_maxpy_norm(int M, int nvec, const double *__restrict__ V,
-                                                    size_t ld, const double *__restrict__ h,
-                                                    double *__restrict__ w,
-                                                    double *__restrict__ part) {
-  __shared__ double hs[256];
+// ---- h_q = sig_q (w_q . w), q = 0..nv-1 ----
+template <class T, int R>
+__global__ void __launch_bounds__(256) k_mdot(int M, int nv, const T *__restrict__ V, size_t ld,
+                                              const T *__restrict__ w, GDev g) {
+  extern __shared__ double lds[];  // [4][nv]
   __shared__ double ws[4];
-  double acc = 0.0;
-  for (long i0 = (long)blockIdx.x * 256; i0 < M; i0 += (long)gridDim.x * 256) {
-    long i = i0 + threadIdx.x;
-    double wi = (i < M) ? w[i] : 0.0;
-    for (int q0 = 0; q0 < nvec; q0 += 256) {
-      int nq = min(256, nvec - q0);
-      __syncthreads();
-      if (threadIdx.x < nq) hs[threadIdx.x] = h[q0 + threadIdx.x];
-      __syncthreads();
-      if (i < M)
-        for (int q = 0; q < nq; ++q) wi -= hs[q] * V[(size_t)(q0 + q) * ld + i];
+  __shared__ int flag;
+  GState *st = g.st;
+  if (halted(st)) return;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const long base = (long)blockIdx.x * 256 * R;
+  double wr[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    long i = base + r * 256 + t;
+    wr[r] = (i < M) ? (double)w[i] : 0.0;
+  }
+  for (int q = 0; q < nv; ++q) {
+    const T *vq = V + (size_t)q * ld;
+    double a = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      long i = base + r * 256 + t;
+      if (i < M) a += wr[r] * (double)vq[i];
     }
+    a = wave_sum(a);
+    if (lane == 0) lds[wave * nv + q] = a;
+  }
+  __syncthreads();
+  const int nb = gridDim.x;
+  for (int q = t; q < nv; q += 256)
+    g.part[(size_t)q * nb + blockIdx.x] =
+        (lds[q] + lds[nv + q]) + (lds[2 * nv + q] + lds[3 * nv + q]);
+  if (!last_block(&st->counter, &flag)) return;
+  for (int q = 0; q < nv; ++q) {
+    double s = sum_partials(g.part + (size_t)q * nb, nb, ws);
+    if (t == 0) g.h[q] = g.sig[q] * s;
+  }
+  if (t == 0) st->counter = 0;
+}
+
+// ---- w -= sum_q h_q sig_q w_q ; ||w|| ; Hessenberg column j ----
+template <class T, int R>
+__global__ void __launch_bounds__(256) k_maxpy_hess(int M, int nv, const T *__restrict__ V,
+                                                    size_t ld, T *__restrict__ w, GDev g) {
+  extern __shared__ double cq[];  // nv coefficients
+  __shared__ double ws[4];
+  __shared__ int flag;
+  GState *st = g.st;
+  if (halted(st)) return;
+  const int t = threadIdx.x;
+  for (int q = t; q < nv; q += 256) cq[q] = g.h[q] * g.sig[q];
+  __syncthreads();
+  const long base = (long)blockIdx.x * 256 * R;
+  double acc = 0.0;
+  T wr[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    long i = base + r * 256 + t;
+    wr[r] = (i < M) ? w[i] : (T)0;
+  }
+  for (int q = 0; q < nv; ++q) {
+    const T *vq = V + (size_t)q * ld;
+    const T c = (T)cq[q];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      long i = base + r * 256 + t;
+      if (i < M) wr[r] -= c * vq[i];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    long i = base + r * 256 + t;
     if (i < M) {
-      w[i] = wi;
-      acc += wi * wi;
+      w[i] = wr[r];
+      acc += (double)wr[r] * (double)wr[r];
     }
   }
   acc = block_sum256(acc, ws);
-  if (threadIdx.x == 0) part[blockIdx.x] = acc;
-}
-
-__global__ void k_scale(int M, double *__restrict__ w, const double *__restrict__ sc) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < M) w[i] *= sc[0];
-}
-
-__global__ void k_scale_host(int M, double *__restrict__ w, double s) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < M) w[i] *= s;
-}
-
-// x += sum_q c_q V_q
-__global__ void __launch_bounds__(256) k_axpy_sol(int M, int nvec, const double *__restrict__ V,
-                                                  size_t ld, const double *__restrict__ cf,
-                                                  double *__restrict__ x) {
-  __shared__ double hs[256];
-  for (long i0 = (long)blockIdx.x * 256; i0 < M; i0 += (long)gridDim.x * 256) {
-    long i = i0 + threadIdx.x;
-    double s = 0.0;
-    for (int q0 = 0; q0 < nvec; q0 += 256) {
-      int nq = min(256, nvec - q0);
-      __syncthreads();
-      if (threadIdx.x < nq) hs[threadIdx.x] = cf[q0 + threadIdx.x];
-      __syncthreads();
-      if (i < M)
-        for (int q = 0; q < nq; ++q) s += hs[q] * V[(size_t)(q0 + q) * ld + i];
-    }
-    if (i < M) x[i] += s;
+  if (t == 0) g.part[blockIdx.x] = acc;
+  if (!last_block(&st->counter, &flag)) return;
+  double nrm2 = sum_partials(g.part, gridDim.x, ws);
+  if (t != 0) return;
+  st->counter = 0;
+  // KSPGMRESCycle after the orthogonalisation (CGS coefficients in h)
+  const int m = st->m;
+  const int j = st->it;
+  double tt = sqrt(nrm2);
+  double *hh = g.HH + (size_t)j * (m + 1);
+  for (int q = 0; q <= j; ++q) hh[q] = g.h[q];
+  hh[j + 1] = tt;
+  g.sig[j + 1] = (tt != 0.0) ? 1.0 / tt : 1.0;
+  double hapbnd = fabs(tt / g.grs[j]);
+  if (hapbnd > 1.0e-30) hapbnd = 1.0e-30;
+  const bool hapend = tt < hapbnd;
+  for (int q = 1; q <= j; ++q) {  // KSPGMRESUpdateHessenberg
+    double t0 = hh[q - 1];
+    hh[q - 1] = g.cc[q - 1] * t0 + g.ss[q - 1] * hh[q];
+    hh[q] = g.cc[q - 1] * hh[q] - g.ss[q - 1] * t0;
   }
+  double res;
+  if (!hapend) {
+    double t1 = sqrt(hh[j] * hh[j] + hh[j + 1] * hh[j + 1]);
+    if (t1 == 0.0) {
+      st->reason = KSP_DIVERGED_NULL;
+      return;
+    }
+    g.cc[j] = hh[j] / t1;
+    g.ss[j] = hh[j + 1] / t1;
+    g.grs[j + 1] = -(g.ss[j] * g.grs[j]);
+    g.grs[j] = g.cc[j] * g.grs[j];
+    hh[j] = g.cc[j] * hh[j] + g.ss[j] * hh[j + 1];
+    res = fabs(g.grs[j + 1]);
+  } else {
+    res = 0.0;
+  }
+  st->it = j + 1;
+  st->its += 1;
+  st->res = res;
+  if (st->monitor) g.hist[st->its] = res;
+  int reason = converged(st, st->its, res);
+  if (hapend && !reason) reason = KSP_DIVERGED_BREAKDOWN;
+  if (st->fp32 && (reason == KSP_CONVERGED_RTOL || reason == KSP_CONVERGED_ATOL || hapend)) {
+    // an FP32 estimate only ends the cycle; the FP64 residual at the next
+    // restart decides convergence
+    reason = 0;
+    st->cycle_done = 1;
+  }
+  st->reason = reason;
+  if (!reason) {
+    // FP32 cycles stop at their precision floor and hand back to the FP64
+    // refinement; FP64 cycles run to the restart length like PETSc
+    if (st->fp32 && res <= 2e-6 * st->cyc_res0) st->cycle_done = 1;
+    if (st->it >= m || st->its >= st->max_it) st->cycle_done = 1;
+  }
+}
+
+// ---- KSPGMRESBuildSoln: coef_q = y_q sig_q ----
+__global__ void k_buildsoln(GDev g, int top) {
+  if (threadIdx.x != 0) return;
+  GState *st = g.st;
+  const int m = st->m;
+  auto H = [&](int r, int c) { return g.HH[(size_t)c * (m + 1) + r]; };
+  double *nrs = g.coef;
+  if (H(top, top) == 0.0) { st->reason = KSP_DIVERGED_BREAKDOWN; top = -1; }
+  else nrs[top] = g.grs[top] / H(top, top);
+  for (int ii = 1; top >= 0 && ii <= top; ++ii) {
+    int k = top - ii;
+    double tt = g.grs[k];
+    for (int q = k + 1; q <= top; ++q) tt = tt - H(k, q) * nrs[q];
+    if (H(k, k) == 0.0) { st->reason = KSP_DIVERGED_BREAKDOWN; top = -1; break; }
+    nrs[k] = tt / H(k, k);
+  }
+  for (int q = 0; q <= top; ++q) nrs[q] *= g.sig[q];
+  for (int q = (top < 0 ? 0 : top + 1); q < m; ++q) nrs[q] = 0.0;
+}
+
+// ---- x += sum_q coef_q w_q ----
+template <class T>
+__global__ void __launch_bounds__(256) k_axpy_sol(int M, int nv, const T *__restrict__ V, size_t ld,
+                                                  const double *__restrict__ coef,
+                                                  double *__restrict__ x) {
+  extern __shared__ double cs[];
+  for (int q = threadIdx.x; q < nv; q += 256) cs[q] = coef[q];
+  __syncthreads();
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < M; i += (long)gridDim.x * 256) {
+    double s = 0.0;
+    for (int q = 0; q < nv; ++q) s += cs[q] * (double)V[(size_t)q * ld + i];
+    x[i] += s;
+  }
+}
+
+__global__ void k_to_float(size_t n, const double *__restrict__ a, float *__restrict__ b) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+    b[i] = (float)a[i];
 }
 
 // sums of rho*h^3 and h^3 (solver.cpp:244-252), block partials
@@ -175,12 +344,17 @@ __global__ void __launch_bounds__(256) k_mass_sums(int M, const double *__restri
   }
 }
 
-__global__ void k_rhs2_periodic(int M, const double *__restrict__ rho, const double *__restrict__ sums,
-                                double c_pot, double *__restrict__ rhs2) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= M) return;
-  double avg = sums[0] / sums[1];
-  rhs2[i] = (rho[i] - avg) * c_pot;
+__global__ void __launch_bounds__(256) k_rhs2_periodic(int M, const double *__restrict__ rho,
+                                                       const double *__restrict__ part, int nb,
+                                                       double c_pot, double *__restrict__ rhs2) {
+  __shared__ double ws[4];
+  double a = 0.0, b = 0.0;
+  for (int q = threadIdx.x; q < nb; q += 256) { a += part[q]; b += part[nb + q]; }
+  a = block_sum256(a, ws);
+  b = block_sum256(b, ws);
+  double avg = a / b;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < M; i += (long)gridDim.x * 256)
+    rhs2[i] = (rho[i] - avg) * c_pot;
 }
 
 __global__ void k_rhs2_bounded(int M, const double *__restrict__ rho, const double *__restrict__ pbnd,
@@ -190,182 +364,215 @@ __global__ void k_rhs2_bounded(int M, const double *__restrict__ rho, const doub
   rhs2[i] = rho[i] * c_pot + pbnd[i];
 }
 
-enum { KSP_CONVERGED_RTOL = 2, KSP_CONVERGED_ATOL = 3, KSP_DIVERGED_NULL = -2,
-       KSP_DIVERGED_ITS = -3, KSP_DIVERGED_DTOL = -4, KSP_DIVERGED_BREAKDOWN = -5,
-       KSP_DIVERGED_NANORINF = -9 };
+constexpr int RED_BLOCKS = 1024;
+constexpr int MAX_RESTART = 1000;
 
-static int reduce_norm2(apc_ctx *c, SolveState *S, const double *v, int M, double *out) {
-  const int nb = (int)std::min<long>(RED_BLOCKS, nblocks(M, 256));
-  hipLaunchKernelGGL(k_mdot, dim3(nb), dim3(256), 0, c->s, M, 1, v, (size_t)M, v, S->part.p);
-  hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, c->s, S->part.p, nb, S->hd.p);
-  APC_HIP(c, hipMemcpyAsync(S->hpin, S->hd.p, sizeof(double), hipMemcpyDeviceToHost, c->s));
-  APC_HIP(c, hipStreamSynchronize(c->s));
-  *out = S->hpin[0];
+template <class T>
+struct Cfg;
+template <>
+struct Cfg<double> { static constexpr int R = 4; };
+template <>
+struct Cfg<float> { static constexpr int R = 8; };
+
+static int ensure_state(apc_ctx *c, int m, int max_it, int M, bool fp32) {
+  if (!c->gm) c->gm = new SolveState();
+  SolveState *S = c->gm;
+  if (fp32) APC_ALLOC(c, S->V32, (size_t)(m + 1) * M + 1);
+  else APC_ALLOC(c, S->V64, (size_t)(m + 1) * M + 1);
+  const int nbm = (int)nblocks(M, 256) + 8;
+  APC_ALLOC(c, S->part, (size_t)(m + 2) * nbm + RED_BLOCKS * 2 + 64);
+  size_t hsz = (size_t)(m + 1) * m + 6 * (size_t)(m + 2) + (size_t)max_it + 8;
+  APC_ALLOC(c, S->hess, hsz);
+  APC_ALLOC(c, S->st, 1);
+  if (!S->hst) APC_HIP(c, hipHostMalloc((void **)&S->hst, sizeof(GState)));
+  S->mcap = m;
   return 0;
 }
 
-// Solve A x = b (x overwritten).  Returns 0 and fills its/reason/rnorm.
-static int gmres(apc_ctx *c, const PMat &A, const double *b, double *x, const apc_ksp_opts &o,
-                 const char *tag, int *its_out, int *reason_out, double *rnorm_out) {
-  const int M = A.M;
-  const int m = std::max(1, o.restart);
-  if (!c->gm) c->gm = new SolveState();
+static GDev gdev_of(SolveState *S, int m) {
+  GDev g;
+  g.st = S->st.p;
+  double *p = S->hess.p;
+  g.HH = p; p += (size_t)(m + 1) * m;
+  g.cc = p; p += m + 2;
+  g.ss = p; p += m + 2;
+  g.grs = p; p += m + 2;
+  g.sig = p; p += m + 2;
+  g.h = p; p += m + 2;
+  g.coef = p; p += m + 2;
+  g.hist = p;
+  g.part = S->part.p;
+  return g;
+}
+
+static int read_state(apc_ctx *c, SolveState *S) {
+  APC_HIP(c, hipMemcpyAsync(S->hst, S->st.p, sizeof(GState), hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  return 0;
+}
+
+template <class T>
+static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, const double *b,
+                      double *x, const apc_ksp_opts &o, bool fp32, int *its_out, int *reason_out,
+                      double *rnorm_out, int *sweeps) {
+  const int M = A64.M;
+  const int m = std::min(std::max(1, o.restart), MAX_RESTART);
   SolveState *S = c->gm;
-  const int nb = (int)std::min<long>(RED_BLOCKS, std::max<long>(1, nblocks(M, 256)));
-  APC_ALLOC(c, S->V, (size_t)(m + 1) * M + 1);
-  APC_ALLOC(c, S->part, (size_t)(m + 2) * nb + 16);
-  APC_ALLOC(c, S->hd, (size_t)m + 8);
-  if (S->hcap < m + 8) {
-    if (S->hpin) hipHostFree(S->hpin);
-    APC_HIP(c, hipHostMalloc((void **)&S->hpin, sizeof(double) * (m + 8)));
-    S->hcap = m + 8;
-  }
-  std::vector<double> HH((size_t)(m + 1) * m, 0.0), cc(m), ss(m), grs(m + 1), nrs(m);
-  const unsigned gb = nblocks(M, 256);
+  GDev g = gdev_of(S, m);
+  GState h0;
+  memset(&h0, 0, sizeof(h0));
+  h0.m = m;
+  h0.max_it = o.max_it;
+  h0.fp32 = fp32 ? 1 : 0;
+  h0.monitor = o.monitor;
+  h0.rtol = o.rtol;
+  h0.atol = o.atol;
+  h0.dtol = o.dtol;
+  *S->hst = h0;
+  APC_HIP(c, hipMemcpyAsync(S->st.p, S->hst, sizeof(GState), hipMemcpyHostToDevice, c->s));
   APC_HIP(c, hipMemsetAsync(x, 0, sizeof(double) * M, c->s));
-  int its = 0, reason = 0, itcount = 0;
-  double rnorm0 = 0, ttol = 0, res = 0;
-  bool guess_zero = true;
-  auto converged = [&](int nit, double rn) {
-    if (nit == 0) {
-      rnorm0 = rn;
-      ttol = std::max(o.rtol * rnorm0, o.atol);
-    }
-    if (std::isnan(rn) || std::isinf(rn)) return (int)KSP_DIVERGED_NANORINF;
-    if (rn <= ttol) return (rn < o.atol) ? (int)KSP_CONVERGED_ATOL : (int)KSP_CONVERGED_RTOL;
-    if (rn >= o.dtol * rnorm0) return (int)KSP_DIVERGED_DTOL;
+  constexpr int R = Cfg<T>::R;
+  const size_t ld = (size_t)M;
+  const unsigned gb = nblocks(M, 256);
+  const unsigned gR = nblocks(M, 256 * R);
+  const unsigned gres = (unsigned)std::min<long>(RED_BLOCKS, gb);
+  const double nnz = (double)M * A64.lm;
+  const double bspmv = fp32 ? 8.0 * nnz + 20.0 * M : 12.0 * nnz + 40.0 * M;
+  int itcount = 0, reason = 0, ncyc = 0, printed = 0;
+  bool zero_guess = true;
+  auto monitor_flush = [&](int its) {
+    if (!o.monitor) return 0;
+    std::vector<double> hist(its + 1);
+    APC_HIP(c, hipMemcpy(hist.data(), g.hist, sizeof(double) * (its + 1), hipMemcpyDeviceToHost));
+    for (int k = printed; k <= its; ++k) printf("%3d KSP Residual norm %14.12e \n", k, hist[k]);
+    printed = its + 1;
     return 0;
   };
-  double *V = S->V.p;
-  const size_t ld = (size_t)M;
-  while (!reason) {
+  for (;;) {
     {
-      KScope ks(c, "gmres_spmv");
-      hipLaunchKernelGGL(k_pc_resid, dim3(gb), dim3(256), 0, c->s, A, x, b, V, guess_zero ? 1 : 0);
+      KScope ks(c, fp32 ? "gmres_resid64" : "gmres_resid", 12.0 * nnz + 48.0 * M + sizeof(T) * M);
+      hipLaunchKernelGGL(k_resid<T>, dim3(gres), dim3(256), 0, c->s, A64, x, b, zero_guess ? 1 : 0,
+                         V, g);
     }
-    double nn2;
-    int r = reduce_norm2(c, S, V, M, &nn2);
+    ++ncyc;
+    int it_launched = 0;
+    const int budget = o.max_it - itcount;
+    int r = read_state(c, S);
     if (r) return r;
-    double res_norm = sqrt(nn2);
-    res = res_norm;
-    grs[0] = res_norm;
-    int it = 0;
-    if (o.monitor) printf("%3d KSP Residual norm %14.12e \n", its, res);
-    if (res == 0.0) { reason = KSP_CONVERGED_ATOL; break; }
-    hipLaunchKernelGGL(k_scale_host, dim3(gb), dim3(256), 0, c->s, M, V, 1.0 / res_norm);
-    reason = converged(its, res);
-    while (!reason && it < m && its < o.max_it) {
-      double *vit = V + (size_t)it * ld, *vn = V + (size_t)(it + 1) * ld;
+    if (S->hst->reason) { reason = S->hst->reason; monitor_flush(S->hst->its); break; }
+    const int batch = 16;
+    for (int j = 0; j < m && j < budget; ++j) {
+      const T *vj = V + (size_t)j * ld;
+      T *w = V + (size_t)(j + 1) * ld;
       {
-        KScope ks(c, "gmres_spmv");
-        hipLaunchKernelGGL(k_spmv_pc, dim3(gb), dim3(256), 0, c->s, A, vit, vn);
+        KScope ks(c, fp32 ? "gmres_spmv32" : "gmres_spmv", bspmv);
+        hipLaunchKernelGGL(k_spmv<T>, dim3(gb), dim3(256), 0, c->s, AT, vj, w, j, g);
+      }
+      const int nv = j + 1;
+      {
+        KScope ks(c, fp32 ? "gmres_mdot32" : "gmres_mdot", (double)sizeof(T) * M * (nv + 1));
+        hipLaunchKernelGGL((k_mdot<T, R>), dim3(gR), dim3(256), 4 * nv * sizeof(double), c->s, M,
+                           nv, V, ld, w, g);
       }
       {
-        KScope ks(c, "gmres_orth");
-        hipLaunchKernelGGL(k_mdot, dim3(nb), dim3(256), 0, c->s, M, it + 1, V, ld, vn, S->part.p);
-        hipLaunchKernelGGL(k_reduce_cols, dim3(it + 1), dim3(256), 0, c->s, S->part.p, nb, S->hd.p);
-        hipLaunchKernelGGL(k_maxpy_norm, dim3(nb), dim3(256), 0, c->s, M, it + 1, V, ld, S->hd.p, vn,
-                           S->part.p + (size_t)(m + 1) * nb);
-        hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, c->s,
-                           S->part.p + (size_t)(m + 1) * nb, nb, S->hd.p + it + 1);
+        KScope ks(c, fp32 ? "gmres_maxpy32" : "gmres_maxpy", (double)sizeof(T) * M * (nv + 2));
+        hipLaunchKernelGGL((k_maxpy_hess<T, R>), dim3(gR), dim3(256), nv * sizeof(double), c->s,
+                           M, nv, V, ld, w, g);
       }
-      APC_HIP(c, hipMemcpyAsync(S->hpin, S->hd.p, sizeof(double) * (it + 2), hipMemcpyDeviceToHost, c->s));
-      APC_HIP(c, hipStreamSynchronize(c->s));
-      double *hh = &HH[(size_t)it * (m + 1)];
-      for (int q = 0; q <= it; ++q) hh[q] = S->hpin[q];
-      double tt = sqrt(S->hpin[it + 1]);
-      if (tt != 0.0) hipLaunchKernelGGL(k_scale_host, dim3(gb), dim3(256), 0, c->s, M, vn, 1.0 / tt);
-      hh[it + 1] = tt;
-      double hapbnd = fabs(tt / grs[it]);
-      if (hapbnd > 1.0e-30) hapbnd = 1.0e-30;
-      bool hapend = tt < hapbnd;
-      for (int q = 1; q <= it; ++q) {
-        double t0 = hh[q - 1];
-        hh[q - 1] = cc[q - 1] * t0 + ss[q - 1] * hh[q];
-        hh[q] = cc[q - 1] * hh[q] - ss[q - 1] * t0;
+      ++it_launched;
+      if ((j + 1) % batch == 0) {
+        r = read_state(c, S);
+        if (r) return r;
+        if (S->hst->reason || S->hst->cycle_done) break;
       }
-      if (!hapend) {
-        double t1 = sqrt(hh[it] * hh[it] + hh[it + 1] * hh[it + 1]);
-        if (t1 == 0.0) { reason = KSP_DIVERGED_NULL; break; }
-        cc[it] = hh[it] / t1;
-        ss[it] = hh[it + 1] / t1;
-        grs[it + 1] = -(ss[it] * grs[it]);
-        grs[it] = cc[it] * grs[it];
-        hh[it] = cc[it] * hh[it] + ss[it] * hh[it + 1];
-        res = fabs(grs[it + 1]);
-      } else {
-        res = 0.0;
-      }
-      it++;
-      its++;
-      if (o.monitor) printf("%3d KSP Residual norm %14.12e \n", its, res);
-      reason = converged(its, res);
-      if (hapend && !reason) reason = KSP_DIVERGED_BREAKDOWN;
     }
-    int top = it - 1;
-    if (top >= 0) {
-      auto H = [&](int rr, int cl) { return HH[(size_t)cl * (m + 1) + rr]; };
-      bool ok = true;
-      if (H(top, top) != 0.0) nrs[top] = grs[top] / H(top, top);
-      else ok = false;
-      for (int ii = 1; ok && ii <= top; ++ii) {
-        int k = top - ii;
-        double tt = grs[k];
-        for (int q = k + 1; q <= top; ++q) tt = tt - H(k, q) * nrs[q];
-        if (H(k, k) == 0.0) { ok = false; break; }
-        nrs[k] = tt / H(k, k);
-      }
-      if (!ok) { reason = KSP_DIVERGED_BREAKDOWN; break; }
-      for (int q = 0; q <= top; ++q) S->hpin[q] = nrs[q];
-      APC_HIP(c, hipMemcpyAsync(S->hd.p, S->hpin, sizeof(double) * (top + 1), hipMemcpyHostToDevice, c->s));
+    r = read_state(c, S);
+    if (r) return r;
+    const int it = S->hst->it;
+    reason = S->hst->reason;
+    monitor_flush(S->hst->its);
+    if (it > 0) {
+      hipLaunchKernelGGL(k_buildsoln, dim3(1), dim3(64), 0, c->s, g, it - 1);
       {
-        KScope ks(c, "gmres_update");
-        hipLaunchKernelGGL(k_axpy_sol, dim3(nb), dim3(256), 0, c->s, M, top + 1, V, ld, S->hd.p, x);
+        KScope ks(c, fp32 ? "gmres_update32" : "gmres_update", (double)sizeof(T) * M * it + 16.0 * M);
+        hipLaunchKernelGGL(k_axpy_sol<T>, dim3(gres), dim3(256), it * sizeof(double), c->s, M, it,
+                           V, ld, g.coef, x);
       }
-      APC_HIP(c, hipStreamSynchronize(c->s));  // hpin is reused next cycle
+      r = read_state(c, S);
+      if (r) return r;
+      if (!reason) reason = S->hst->reason;
     }
     itcount += it;
-    if (itcount >= o.max_it) {
-      if (!reason) reason = KSP_DIVERGED_ITS;
-      break;
-    }
-    guess_zero = false;
+    (void)it_launched;
+    if (reason) break;
+    if (itcount >= o.max_it) { reason = KSP_DIVERGED_ITS; break; }
+    zero_guess = false;
   }
-  APC_HIP(c, hipGetLastError());
-  (void)tag;
-  *its_out = its;
+  *its_out = S->hst->its;
   *reason_out = reason;
-  *rnorm_out = res;
+  *rnorm_out = S->hst->res;
+  if (sweeps) *sweeps = ncyc;
+  APC_HIP(c, hipGetLastError());
   return 0;
+}
+
+// Solve A x = b (x overwritten) with the matrix (nbr, s, d, f, dinv).
+static int gmres(apc_ctx *c, const Op<double> &A, const double *b, double *x,
+                 const apc_ksp_opts &o, int *its, int *reason, double *rnorm, int *sweeps) {
+  const int M = A.M;
+  const int m = std::min(std::max(1, o.restart), MAX_RESTART);
+  const bool fp32 = o.precision == 1;
+  int r = ensure_state(c, m, std::max(1, o.max_it), M, fp32);
+  if (r) return r;
+  SolveState *S = c->gm;
+  if (!fp32) return run_cycles<double>(c, A, A, S->V64.p, b, x, o, false, its, reason, rnorm, sweeps);
+  // FP32 copies of the operator for the inner cycles
+  const size_t nnz = (size_t)M * A.lm;
+  APC_ALLOC(c, S->s32, nnz + 1);
+  APC_ALLOC(c, S->d32, M + 1);
+  APC_ALLOC(c, S->f32, M + 1);
+  APC_ALLOC(c, S->dinv32, M + 1);
+  hipLaunchKernelGGL(k_to_float, dim3(1024), dim3(256), 0, c->s, nnz, A.s, S->s32.p);
+  hipLaunchKernelGGL(k_to_float, dim3(256), dim3(256), 0, c->s, (size_t)M, A.d, S->d32.p);
+  hipLaunchKernelGGL(k_to_float, dim3(256), dim3(256), 0, c->s, (size_t)M, A.f, S->f32.p);
+  hipLaunchKernelGGL(k_to_float, dim3(256), dim3(256), 0, c->s, (size_t)M, A.dinv, S->dinv32.p);
+  Op<float> A32{A.nbr, S->s32.p, S->d32.p, S->f32.p, S->dinv32.p, M, A.lm};
+  return run_cycles<float>(c, A, A32, S->V32.p, b, x, o, true, its, reason, rnorm, sweeps);
 }
 
 int solve_density_potential(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_opts *o2,
                             apc_solve_info *info) {
   const int M = c->nc + c->nv;
-  if (o1->precision != 0 || o2->precision != 0)
-    return fail(c, APC_ERR_UNSUPPORTED, "mixed-precision GMRES is not built in this library");
+  if (o1->precision < 0 || o1->precision > 1 || o2->precision < 0 || o2->precision > 1)
+    return fail(c, APC_ERR_ARG, "GMRES precision must be 0 (FP64) or 1 (FP32 + FP64 refinement)");
   APC_ALLOC(c, c->rho, M + 1);
   APC_ALLOC(c, c->phi, M + 1);
   APC_ALLOC(c, c->rhs2, M + 1);
-  PMat A1{c->nbr.p, c->sco.p, c->d1.p, c->f1.p, c->dinv1.p, M, c->prm.lm};
-  PMat A2{c->nbr.p, c->sco.p, c->d2.p, c->f2.p, c->dinv2.p, M, c->prm.lm};
-  apc_solve_info inf{};
-  int r = gmres(c, A1, c->rhs1.p, c->rho.p, *o1, "rho", &inf.its_rho, &inf.reason_rho, &inf.rnorm_rho);
+  Op<double> A1{c->nbr.p, c->sco.p, c->d1.p, c->f1.p, c->dinv1.p, M, c->prm.lm};
+  Op<double> A2{c->nbr.p, c->sco.p, c->d2.p, c->f2.p, c->dinv2.p, M, c->prm.lm};
+  apc_solve_info inf;
+  memset(&inf, 0, sizeof(inf));
+  int r = gmres(c, A1, c->rhs1.p, c->rho.p, *o1, &inf.its_rho, &inf.reason_rho, &inf.rnorm_rho,
+                &inf.refine_rho);
   if (r) return r;
   // right-hand side of the Poisson solve (solver.cpp:233-261)
   if (c->prm.bc == 1) {
     const int nb = (int)std::min<long>(RED_BLOCKS, std::max<long>(1, nblocks(M, 256)));
     SolveState *S = c->gm;
-    hipLaunchKernelGGL(k_mass_sums, dim3(nb), dim3(256), 0, c->s, M, c->rho.p, c->ch.p, S->part.p);
-    hipLaunchKernelGGL(k_reduce_cols, dim3(2), dim3(256), 0, c->s, S->part.p, nb, S->hd.p);
-    hipLaunchKernelGGL(k_rhs2_periodic, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, c->rho.p,
-                       S->hd.p, c->c_pot, c->rhs2.p);
+    double *part = S->part.p;
+    hipLaunchKernelGGL(k_mass_sums, dim3(nb), dim3(256), 0, c->s, M, c->rho.p, c->ch.p, part);
+    hipLaunchKernelGGL(k_rhs2_periodic, dim3(nb), dim3(256), 0, c->s, M, c->rho.p, part, nb,
+                       c->c_pot, c->rhs2.p);
   } else {
     hipLaunchKernelGGL(k_rhs2_bounded, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, c->rho.p,
                        c->pbnd.p, c->c_pot, c->rhs2.p);
   }
-  r = gmres(c, A2, c->rhs2.p, c->phi.p, *o2, "phi", &inf.its_phi, &inf.reason_phi, &inf.rnorm_phi);
+  r = gmres(c, A2, c->rhs2.p, c->phi.p, *o2, &inf.its_phi, &inf.reason_phi, &inf.rnorm_phi,
+            &inf.refine_phi);
   if (r) return r;
+  inf.nc = c->nc;
+  inf.nv = c->nv;
+  inf.nb = c->nb;
   if (info) *info = inf;
   return 0;
 }

@@ -42,6 +42,7 @@ struct KTimer {
   std::vector<hipEvent_t> ev;  // start/stop pairs not yet folded in
   double total_ms = 0;
   int launches = 0;
+  double bytes = 0;  // algorithmic bytes summed over the launches
 };
 
 struct SolveState;  // apc_gmres.hip
@@ -100,6 +101,7 @@ struct apc_ctx {
   // timing
   bool timing = false;
   std::vector<apc::KTimer> timers;
+  std::vector<hipEvent_t> evpool;
 };
 
 namespace apc {
@@ -123,14 +125,18 @@ int hip_fail(apc_ctx *c, hipError_t e, const char *what);
 int ktimer_index(apc_ctx *c, const char *name);
 void ktime_begin(apc_ctx *c, int idx);
 void ktime_end(apc_ctx *c, int idx);
+void ktime_bytes(apc_ctx *c, int idx, double bytes);
 
 struct KScope {
   apc_ctx *c;
   int idx;
-  KScope(apc_ctx *c_, const char *name) : c(c_), idx(-1) {
+  // `bytes`: algorithmic HBM bytes the scoped launch moves (each input read
+  // once, each output written once, neighbour gathers counted per use)
+  KScope(apc_ctx *c_, const char *name, double bytes = 0.0) : c(c_), idx(-1) {
     if (c->timing) {
       idx = ktimer_index(c, name);
       ktime_begin(c, idx);
+      ktime_bytes(c, idx, bytes);
     }
   }
   ~KScope() {

@@ -94,7 +94,9 @@ int interp_kick(apc_ctx *c, double Cv, double dt, double *maxdv) {
   APC_ALLOC(c, c->red_u64, 4);
   APC_HIP(c, hipMemsetAsync(c->red_u64.p, 0, sizeof(unsigned long long), c->s));
   {
-    KScope ks(c, "interp_kick");
+    // per particle: owner 4, x 24, v 24 read + 24 written, a 24 written;
+    // per leaf: centre 24 + g,H 72 read
+    KScope ks(c, "interp_kick", 100.0 * N + 96.0 * M);
     hipLaunchKernelGGL(k_interp_kick, dim3(nblocks(N, 256)), dim3(256), 0, c->s, N, M, c->owner.p,
                        c->cx.p, c->cy.p, c->cz.p, c->grad.p, c->x.p, c->y.p, c->z.p, c->vx.p,
                        c->vy.p, c->vz.p, c->ax.p, c->ay.p, c->az.p, Cv, dt, c->red_u64.p);
@@ -110,7 +112,7 @@ int drift(apc_ctx *c, double Cx, double dt, double *maxdx) {
   APC_ALLOC(c, c->red_u64, 4);
   APC_HIP(c, hipMemsetAsync(c->red_u64.p, 0, sizeof(unsigned long long), c->s));
   {
-    KScope ks(c, "drift");
+    KScope ks(c, "drift", 72.0 * N);
     hipLaunchKernelGGL(k_drift, dim3(nblocks(N, 256)), dim3(256), 0, c->s, N, c->x.p, c->y.p,
                        c->z.p, c->vx.p, c->vy.p, c->vz.p, Cx, dt, b[0], b[1], b[2], b[3], b[4],
                        b[5], c->prm.bc, c->red_u64.p);

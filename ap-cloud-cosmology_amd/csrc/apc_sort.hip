@@ -111,13 +111,13 @@ int radix_sort_pairs(apc_ctx *c, uint64_t *k, int *v, uint64_t *k2, int *v2, int
   for (int p = 0; p < passes; ++p) {
     int sh = 8 * p;
     {
-      KScope ks(c, "radix_hist");
+      KScope ks(c, "radix_hist", 8.0 * n);
       hipLaunchKernelGGL(k_rx_hist, dim3(nblk), dim3(256), 0, c->s, ka, n, sh, c->hist.p, nblk);
     }
     int r = scan_exclusive(c, c->hist.p, c->histx.p, 256 * nblk, nullptr);
     if (r) return r;
     {
-      KScope ks(c, "radix_scatter");
+      KScope ks(c, "radix_scatter", 24.0 * n);
       hipLaunchKernelGGL(k_rx_scatter, dim3(nblk), dim3(256), 0, c->s, ka, va, kb, vb, n, sh,
                          c->histx.p, nblk);
     }
@@ -161,7 +161,7 @@ int sort_particles(apc_ctx *c) {
   APC_ALLOC(c, c->perm, n);
   APC_ALLOC(c, c->perm2, n);
   {
-    KScope ks(c, "keys");
+    KScope ks(c, "keys", 36.0 * n);
     hipLaunchKernelGGL(k_keys, dim3(nblocks(n, 256)), dim3(256), 0, c->s, n, c->x.p, c->y.p,
                        c->z.p, p.bbox[0], p.bbox[1], p.bbox[2], p.bbox[3], p.bbox[4], p.bbox[5],
                        p.depth, c->key.p, c->perm.p);
@@ -172,7 +172,7 @@ int sort_particles(apc_ctx *c) {
   APC_ALLOC(c, c->t3, n); APC_ALLOC(c, c->t4, n); APC_ALLOC(c, c->t5, n);
   APC_ALLOC(c, c->tid, n);
   {
-    KScope ks(c, "permute");
+    KScope ks(c, "permute", 108.0 * n);
     hipLaunchKernelGGL(k_permute, dim3(nblocks(n, 256)), dim3(256), 0, c->s, n, c->perm.p, c->x.p,
                        c->y.p, c->z.p, c->vx.p, c->vy.p, c->vz.p, c->id.p, c->t0.p, c->t1.p,
                        c->t2.p, c->t3.p, c->t4.p, c->t5.p, c->tid.p);

@@ -254,7 +254,9 @@ int build_stencils(apc_ctx *c) {
                        p.bv_center[2], c->cx.p, c->cy.p, c->cz.p, c->bv.p);
   int pin = (p.bc == 1 && p.dirichlet_point) ? 1 : 0;
   {
-    KScope ks(c, "stencil");
+    // per slot: index 4 + distance 8 + neighbour xyz 24 read, weight 8 written;
+    // per row: own xyz, h, n_p 36 read, eight row arrays 64 written
+    KScope ks(c, "stencil", (double)M * (44.0 * lm + 100.0));
     hipLaunchKernelGGL(k_stencil, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, lm, geo_of(c), pin,
                        c->c_dens, c->cx.p, c->cy.p, c->cz.p, c->ch.p, c->cnp.p, c->nbr.p,
                        c->dist.p, c->bv.p, c->sco.p, c->d1.p, c->d2.p, c->f1.p, c->f2.p,
@@ -273,7 +275,8 @@ int differentiate(apc_ctx *c) {
   APC_ALLOC(c, c->status_dev, 4);
   APC_HIP(c, hipMemsetAsync(c->status_dev.p, 0, 4 * sizeof(int), c->s));
   {
-    KScope ks(c, "gradient");
+    // per slot: index 4 + distance 8 + xyz 24 + phi 8; per row: own 40 read, 72 written
+    KScope ks(c, "gradient", (double)M * (44.0 * lm + 112.0));
     hipLaunchKernelGGL(k_gradient, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, lm, geo_of(c),
                        c->c_field, c->cx.p, c->cy.p, c->cz.p, c->ch.p, c->nbr.p, c->dist.p,
                        c->phi.p, c->bv.p, c->grad.p, c->status_dev.p);

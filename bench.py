@@ -51,26 +51,6 @@ def eds(a):
     return 1.5 / a, -1.0, cx, cx / a / a
 
 
-def algorithmic_bytes(name, M, N, lm):
-    """Compulsory HBM bytes of one launch of a kernel (each input read once,
-    each output written once; neighbour gathers counted per use)."""
-    nnz = M * lm
-    if name == "gmres_spmv":
-        # s (8) + col (4) per slot, x_i + y_i (16) + d, f, dinv (24) per row
-        return 12 * nnz + 40 * M
-    if name == "stencil":
-        # neighbour xyz (24) + index (4) + distance (8) per slot; own xyz,h,np;
-        # weights out (8 per slot) + d1 d2 f1 f2 dinv1 dinv2 rhs1 pbnd
-        return 36 * nnz + 8 * nnz + 36 * M + 64 * M
-    if name == "gradient":
-        return 44 * nnz + 40 * M + 72 * M
-    if name == "interp_kick":
-        return N * (4 + 24 + 48 + 24) + 96 * M
-    if name == "drift":
-        return N * 72
-    return None
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +62,7 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--restart", type=int, default=200)
     ap.add_argument("--max-it", type=int, default=1000)
+    ap.add_argument("--precision", type=int, default=0, help="0 FP64 GMRES, 1 FP32 + FP64 refinement")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -101,7 +82,8 @@ def main():
                                  leaf_capacity=8)
     ctx = apcloud.Context(params)
     ctx.set_particles(x, y, z, vx, vy, vz)
-    opts = apcloud.ksp(rtol=args.rtol, restart=args.restart, max_it=args.max_it)
+    opts = apcloud.ksp(rtol=args.rtol, restart=args.restart, max_it=args.max_it,
+                       precision=args.precision)
     a_start, a_end = 0.02, 0.1
     da = (a_end - a_start) / nsteps_total
     t = a_start
@@ -133,25 +115,26 @@ def main():
     wall = time.perf_counter() - t0
     kt = ctx.kernel_times()
     ctx.set_timing(False)
-    comp = ctx.comp()
-    M = comp["nc"] + comp["nv"]
+    M = info.nc + info.nv
 
-    # roofline of the dominant kernel (largest total device time)
+    # roofline of the dominant kernel: largest total device time among the
+    # kernels whose launches carry algorithmic byte counts
     best = None
-    for name, (avg_ms, cnt) in kt.items():
-        b = algorithmic_bytes(name, M, N, params.lm)
-        if b is None or cnt == 0:
+    for name, (avg_ms, cnt, avg_bytes) in kt.items():
+        if cnt == 0 or avg_bytes <= 0 or name == "stream_copy":
             continue
         tot = avg_ms * cnt
         if best is None or tot > best[3]:
-            best = (name, b, avg_ms, tot)
+            best = (name, avg_bytes, avg_ms, tot, cnt)
     roof = None
+    copy_gbs = ctx.stream_copy(2 ** 30, reps=10)
     if best:
-        name, b, avg_ms, _ = best
+        name, b, avg_ms, _, cnt = best
         ach = b / (avg_ms * 1e-3) / 1e9
         roof = {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_launch": int(b), "avg_launch_ms": round(avg_ms, 4)}
+                "bytes_per_launch": int(b), "avg_launch_ms": round(avg_ms, 4), "launches": cnt,
+                "measured_copy_GBs": round(copy_gbs, 1)}
 
     value = N * K / wall
     out = {
@@ -175,7 +158,9 @@ def main():
         "cpu_baseline": None,
     }
     if args.verbose:
-        out["kernels_ms"] = {k: [round(v[0], 4), v[1]] for k, v in kt.items()}
+        out["kernels"] = {k: {"avg_ms": round(v[0], 4), "launches": v[1],
+                              "GBs": round(v[2] / (v[0] * 1e6), 1) if v[0] > 0 else None}
+                          for k, v in kt.items()}
     ctx.close()
 
     if not args.no_cpu:

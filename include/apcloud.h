@@ -65,6 +65,7 @@ typedef struct {
   int its_rho, reason_rho; double rnorm_rho;
   int its_phi, reason_phi; double rnorm_phi;
   int refine_rho, refine_phi;   /* outer refinement sweeps (precision form) */
+  int nc, nv, nb;               /* sizes of the system that was solved */
 } apc_solve_info;
 
 /* Host view of the computational particles (particle.h:23-53), filled by
@@ -159,10 +160,15 @@ int apc_step(apc_ctx *ctx, double c_dens, double c_pot, double c_field, double C
              double dt, const apc_ksp_opts *rho, const apc_ksp_opts *phi, double *max_dv,
              double *max_dx, apc_solve_info *info, double *timing);
 
-/* Per-kernel device time of the last calls (ms, averaged per launch) for
- * the roofline report; names are NUL-separated.  Returns the count. */
+/* Per-kernel device time of the timed calls (HIP events on the context's
+ * stream): average ms and algorithmic bytes per launch, launch counts; names
+ * are NUL-separated.  Returns the number of kernels. */
 int apc_kernel_times(apc_ctx *ctx, char *names, int names_len, double *avg_ms, int *launches,
-                     int cap);
+                     double *avg_bytes, int cap);
+/* Copies `bytes` bytes device-to-device with a plain streaming kernel `reps`
+ * times and reports the achieved GB/s (read + write), the measured HBM
+ * reference beside the 8 TB/s datasheet peak. */
+int apc_stream_copy(apc_ctx *ctx, double bytes, int reps, double *gbs);
 int apc_reset_kernel_times(apc_ctx *ctx);
 int apc_set_timing(apc_ctx *ctx, int on);
 int apc_sync(apc_ctx *ctx);

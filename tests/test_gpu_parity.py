@@ -239,3 +239,25 @@ def test_full_step_against_oracle():
     for a_, b_ in zip(P_g[:3], P_o[:3]):
         assert np.abs(a_ - b_).max() < 1e-12
     assert abs(dv_g - dv_o) <= 1e-6 * dv_o
+
+
+@pytest.mark.parametrize("kind,n,bc", [("plummer", 20000, 1), ("ball", 20000, 0)])
+def test_mixed_precision_gmres_matches_fp64(kind, n, bc):
+    # FP32 cycles inside FP64 iterative refinement converge to the FP64 answer
+    x, y, z = cloud(kind, n, 18)
+    kw = dict(bv_kind=1, boundary_at_faces=1) if bc == 0 else {}
+    res = []
+    for prec in (0, 1):
+        g, _ = pair(x, y, z, bc=bc, c=0.01, lm=27, **kw)
+        g.octree_build()
+        g.select()
+        g.set_constants(1.0 / n, 4 * np.pi if bc == 0 else 75.0, -1.0)
+        info = g.solve(apcloud.ksp(rtol=1e-11, restart=60, max_it=4000, precision=prec))
+        assert info.reason_rho > 0 and info.reason_phi > 0, (prec, info.reason_rho, info.reason_phi)
+        res.append(g.fields())
+        if prec == 1:
+            assert info.refine_phi >= 2
+    for f, tol in (("rho", 1e-9), ("phi", 1e-8), ("g", 1e-7)):
+        ref = res[0][f]
+        err = np.abs(res[1][f] - ref).max() / np.abs(ref).max()
+        assert err < tol, (f, err)
