@@ -149,48 +149,65 @@ __global__ void __launch_bounds__(256) k_spmv(Op<T> A, const T *__restrict__ vj,
 }
 
 // ---- h_q = sig_q (w_q . w), q = 0..nv-1 ----
-template <class T, int R>
+// Grid-stride over rows, QG vectors per pass (QG + 1 independent loads per
+// row), one block reduction per pass; the last block reduces the per-block
+// partials one vector per wave.
+template <class T, int QG>
 __global__ void __launch_bounds__(256) k_mdot(int M, int nv, const T *__restrict__ V, size_t ld,
                                               const T *__restrict__ w, GDev g) {
-  extern __shared__ double lds[];  // [4][nv]
-  __shared__ double ws[4];
+  __shared__ double ws[4][QG];
   __shared__ int flag;
   GState *st = g.st;
   if (halted(st)) return;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-  const long base = (long)blockIdx.x * 256 * R;
-  double wr[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    long i = base + r * 256 + t;
-    wr[r] = (i < M) ? (double)w[i] : 0.0;
-  }
-  for (int q = 0; q < nv; ++q) {
-    const T *vq = V + (size_t)q * ld;
-    double a = 0.0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      long i = base + r * 256 + t;
-      if (i < M) a += wr[r] * (double)vq[i];
-    }
-    a = wave_sum(a);
-    if (lane == 0) lds[wave * nv + q] = a;
-  }
-  __syncthreads();
   const int nb = gridDim.x;
-  for (int q = t; q < nv; q += 256)
-    g.part[(size_t)q * nb + blockIdx.x] =
-        (lds[q] + lds[nv + q]) + (lds[2 * nv + q] + lds[3 * nv + q]);
+  const long stride = (long)nb * 256;
+  for (int q0 = 0; q0 < nv; q0 += QG) {
+    const int nq = min(QG, nv - q0);
+    const T *v0 = V + (size_t)q0 * ld;
+    double acc[QG];
+#pragma unroll
+    for (int k = 0; k < QG; ++k) acc[k] = 0.0;
+    if (nq == QG) {
+      for (long i = (long)blockIdx.x * 256 + t; i < M; i += stride) {
+        const double wi = (double)w[i];
+#pragma unroll
+        for (int k = 0; k < QG; ++k) acc[k] += wi * (double)v0[(size_t)k * ld + i];
+      }
+    } else {
+      for (long i = (long)blockIdx.x * 256 + t; i < M; i += stride) {
+        const double wi = (double)w[i];
+#pragma unroll
+        for (int k = 0; k < QG; ++k)
+          if (k < nq) acc[k] += wi * (double)v0[(size_t)k * ld + i];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < QG; ++k) {
+      double a = wave_sum(acc[k]);
+      if (lane == 0) ws[wave][k] = a;
+    }
+    __syncthreads();
+    if (t < nq)
+      g.part[(size_t)(q0 + t) * nb + blockIdx.x] = (ws[0][t] + ws[1][t]) + (ws[2][t] + ws[3][t]);
+    __syncthreads();
+  }
   if (!last_block(&st->counter, &flag)) return;
-  for (int q = 0; q < nv; ++q) {
-    double s = sum_partials(g.part + (size_t)q * nb, nb, ws);
-    if (t == 0) g.h[q] = g.sig[q] * s;
+  for (int q = wave; q < nv; q += 4) {
+    const double *pq = g.part + (size_t)q * nb;
+    double a = 0.0;
+    for (int b = lane; b < nb; b += 64) a += pq[b];
+    a = wave_sum(a);
+    if (lane == 0) g.h[q] = g.sig[q] * a;
   }
   if (t == 0) st->counter = 0;
 }
 
 // ---- w -= sum_q h_q sig_q w_q ; ||w|| ; Hessenberg column j ----
-template <class T, int R>
+// Grid-stride over rows; the update is accumulated in FP64 whatever the
+// storage type, four basis vectors per step (four independent loads in
+// flight per row).
+template <class T>
 __global__ void __launch_bounds__(256) k_maxpy_hess(int M, int nv, const T *__restrict__ V,
                                                     size_t ld, T *__restrict__ w, GDev g) {
   extern __shared__ double cq[];  // nv coefficients
@@ -201,30 +218,25 @@ __global__ void __launch_bounds__(256) k_maxpy_hess(int M, int nv, const T *__re
   const int t = threadIdx.x;
   for (int q = t; q < nv; q += 256) cq[q] = g.h[q] * g.sig[q];
   __syncthreads();
-  const long base = (long)blockIdx.x * 256 * R;
+  const long stride = (long)gridDim.x * 256;
+  const int nv4 = nv & ~3;
   double acc = 0.0;
-  T wr[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    long i = base + r * 256 + t;
-    wr[r] = (i < M) ? w[i] : (T)0;
-  }
-  for (int q = 0; q < nv; ++q) {
-    const T *vq = V + (size_t)q * ld;
-    const T c = (T)cq[q];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      long i = base + r * 256 + t;
-      if (i < M) wr[r] -= c * vq[i];
+  for (long i = (long)blockIdx.x * 256 + t; i < M; i += stride) {
+    double s = (double)w[i];
+    const T *vi = V + i;
+    int q = 0;
+    for (; q < nv4; q += 4) {
+      const double a0 = (double)vi[(size_t)q * ld], a1 = (double)vi[(size_t)(q + 1) * ld];
+      const double a2 = (double)vi[(size_t)(q + 2) * ld], a3 = (double)vi[(size_t)(q + 3) * ld];
+      s -= cq[q] * a0;
+      s -= cq[q + 1] * a1;
+      s -= cq[q + 2] * a2;
+      s -= cq[q + 3] * a3;
     }
-  }
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    long i = base + r * 256 + t;
-    if (i < M) {
-      w[i] = wr[r];
-      acc += (double)wr[r] * (double)wr[r];
-    }
+    for (; q < nv; ++q) s -= cq[q] * (double)vi[(size_t)q * ld];
+    const T sv = (T)s;
+    w[i] = sv;
+    acc += (double)sv * (double)sv;
   }
   acc = block_sum256(acc, ws);
   if (t == 0) g.part[blockIdx.x] = acc;
@@ -367,12 +379,15 @@ __global__ void k_rhs2_bounded(int M, const double *__restrict__ rho, const doub
 constexpr int RED_BLOCKS = 1024;
 constexpr int MAX_RESTART = 1000;
 
+// basis vectors per multi-dot pass: QG + 1 independent row loads in flight
 template <class T>
 struct Cfg;
 template <>
-struct Cfg<double> { static constexpr int R = 4; };
+struct Cfg<double> { static constexpr int QG = 8; };
 template <>
-struct Cfg<float> { static constexpr int R = 8; };
+struct Cfg<float> { static constexpr int QG = 16; };
+constexpr int MDOT_BLOCKS = 1024;   // partials per vector reduced by one block
+constexpr int MAXPY_BLOCKS = 4096;
 
 static int ensure_state(apc_ctx *c, int m, int max_it, int M, bool fp32) {
   if (!c->gm) c->gm = new SolveState();
@@ -431,10 +446,11 @@ static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, 
   *S->hst = h0;
   APC_HIP(c, hipMemcpyAsync(S->st.p, S->hst, sizeof(GState), hipMemcpyHostToDevice, c->s));
   APC_HIP(c, hipMemsetAsync(x, 0, sizeof(double) * M, c->s));
-  constexpr int R = Cfg<T>::R;
+  constexpr int QG = Cfg<T>::QG;
   const size_t ld = (size_t)M;
   const unsigned gb = nblocks(M, 256);
-  const unsigned gR = nblocks(M, 256 * R);
+  const unsigned gdot = (unsigned)std::min<long>(MDOT_BLOCKS, gb);
+  const unsigned gaxpy = (unsigned)std::min<long>(MAXPY_BLOCKS, gb);
   const unsigned gres = (unsigned)std::min<long>(RED_BLOCKS, gb);
   const double nnz = (double)M * A64.lm;
   const double bspmv = fp32 ? 8.0 * nnz + 20.0 * M : 12.0 * nnz + 40.0 * M;
@@ -471,12 +487,11 @@ static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, 
       const int nv = j + 1;
       {
         KScope ks(c, fp32 ? "gmres_mdot32" : "gmres_mdot", (double)sizeof(T) * M * (nv + 1));
-        hipLaunchKernelGGL((k_mdot<T, R>), dim3(gR), dim3(256), 4 * nv * sizeof(double), c->s, M,
-                           nv, V, ld, w, g);
+        hipLaunchKernelGGL((k_mdot<T, QG>), dim3(gdot), dim3(256), 0, c->s, M, nv, V, ld, w, g);
       }
       {
         KScope ks(c, fp32 ? "gmres_maxpy32" : "gmres_maxpy", (double)sizeof(T) * M * (nv + 2));
-        hipLaunchKernelGGL((k_maxpy_hess<T, R>), dim3(gR), dim3(256), nv * sizeof(double), c->s,
+        hipLaunchKernelGGL((k_maxpy_hess<T>), dim3(gaxpy), dim3(256), nv * sizeof(double), c->s,
                            M, nv, V, ld, w, g);
       }
       ++it_launched;
