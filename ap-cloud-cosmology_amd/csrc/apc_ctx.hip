@@ -152,7 +152,8 @@ int apc_destroy(apc_ctx *c) {
                           &c->scratch_i, &c->nlevel, &c->nfirst, &c->nnp, &c->nstatus, &c->nchild,
                           &c->nparent, &c->nflag, &c->nrank, &c->nout, &c->dcount, &c->leaf_node,
                           &c->cnp, &c->cfirst, &c->ctype, &c->n24, &c->n24cnt, &c->n24first,
-                          &c->owner, &c->nbr, &c->nscratch_i, &c->status_dev};
+                          &c->owner, &c->nbr, &c->nscratch_i, &c->nscratch_f, &c->novf,
+                          &c->status_dev};
   for (auto b : is) b->release();
   c->key.release(); c->key2.release(); c->nprefix.release(); c->red_u64.release();
   hipStreamDestroy(c->s);

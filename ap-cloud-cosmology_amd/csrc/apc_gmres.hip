@@ -7,18 +7,24 @@
 // the Hessenberg matrix, true residual at every restart, -ksp_max_it counted
 // across restarts.
 //
-// Device-driven: an Arnoldi step is three launches — preconditioned SpMV,
-// multi-dot (h = V^T w), and "subtract projections + norm + Hessenberg
-// update" — whose reductions finish in the last block to arrive (fixed
-// summation order, deterministic).  The Givens rotations and the stopping
-// test run on the device, so the host only launches and checks a status word
-// every few steps.  Basis vectors are kept unnormalised with scale factors
-// sig[q] (v_q = sig[q] w_q), which removes the normalisation pass.
+// Device-driven: an Arnoldi step is five launches — preconditioned SpMV,
+// multi-dot partials (h = V^T w), their reduction, "subtract projections +
+// norm partials", and a one-block Hessenberg/Givens/stopping-test kernel.
+// Every reduction is two-stage with a fixed summation order (deterministic).
+// The reductions end in kernels of their own rather than in the last block
+// to arrive: on this part a device-scope release fence per block writes back
+// that XCD's L2 and the arrival counter serialises every block on one
+// address, which cost more than the extra launches.  The host only launches
+// and checks a status word every few steps.  Basis vectors are kept
+// unnormalised with scale factors sig[q] (v_q = sig[q] w_q), which removes
+// the normalisation pass.
 //
 // Precision form: the same cycle with an FP32 basis and FP32 copies of the
 // matrix, inside FP64 iterative refinement — every restart computes the true
 // residual with the FP64 operator, the FP32 cycle solves for the correction
 // and the correction is accumulated into the FP64 solution.
+#include <cstdlib>
+
 #include "apc_internal.h"
 #include "apc_prims.h"
 
@@ -36,7 +42,6 @@ struct GState {
   int m, max_it, fp32, monitor;
   double res, rnorm0, ttol, cyc_res0;
   double rtol, atol, dtol;
-  unsigned counter;
 };
 
 enum { KSP_CONVERGED_RTOL = 2, KSP_CONVERGED_ATOL = 3, KSP_DIVERGED_NULL = -2,
@@ -83,14 +88,6 @@ __device__ inline double sum_partials(const double *part, int nb, double *ws) {
   return block_sum256(a, ws);
 }
 
-// Last block to arrive?  (partials written before the ticket)
-__device__ inline bool last_block(unsigned *counter, int *flag) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) *flag = (atomicAdd(counter, 1u) == gridDim.x - 1);
-  __syncthreads();
-  return *flag != 0;
-}
 
 template <class T>
 __device__ inline double row_apply(const Op<T> &A, const T *__restrict__ x, int i) {
@@ -108,9 +105,7 @@ __global__ void __launch_bounds__(256) k_resid(Op<double> A, const double *__res
                                                const double *__restrict__ b, int zero_guess,
                                                T *__restrict__ v0, GDev g) {
   __shared__ double ws[4];
-  __shared__ int flag;
-  GState *st = g.st;
-  if (st->reason) return;
+  if (g.st->reason) return;
   double acc = 0.0;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < A.M; i += gridDim.x * 256) {
     double r = zero_guess ? A.dinv[i] * b[i] : A.dinv[i] * (b[i] - row_apply(A, x, i));
@@ -119,10 +114,15 @@ __global__ void __launch_bounds__(256) k_resid(Op<double> A, const double *__res
   }
   acc = block_sum256(acc, ws);
   if (threadIdx.x == 0) g.part[blockIdx.x] = acc;
-  if (!last_block(&st->counter, &flag)) return;
-  double nrm2 = sum_partials(g.part, gridDim.x, ws);
+}
+
+// one block: ||v0|| from the partials, cycle state, PETSc's first check
+__global__ void __launch_bounds__(256) k_resid_fin(int nb, GDev g) {
+  __shared__ double ws[4];
+  GState *st = g.st;
+  if (st->reason) return;
+  double nrm2 = sum_partials(g.part, nb, ws);
   if (threadIdx.x == 0) {
-    st->counter = 0;
     double res = sqrt(nrm2);
     st->res = res;
     st->it = 0;
@@ -149,89 +149,110 @@ __global__ void __launch_bounds__(256) k_spmv(Op<T> A, const T *__restrict__ vj,
 }
 
 // ---- h_q = sig_q (w_q . w), q = 0..nv-1 ----
-// Grid-stride over rows, QG vectors per pass (QG + 1 independent loads per
-// row), one block reduction per pass; the last block reduces the per-block
-// partials one vector per wave.
-template <class T, int QG>
+// Grid-stride over row tiles of 256*R rows.  A thread keeps its R entries of
+// w in registers and walks every basis vector over the tile, U vectors at a
+// time (R*U independent loads in flight); per-vector wave sums accumulate in
+// LDS per wave, in tile order.  Partials go to part[q*nb + block].
+template <class T, int R, int U>
 __global__ void __launch_bounds__(256) k_mdot(int M, int nv, const T *__restrict__ V, size_t ld,
                                               const T *__restrict__ w, GDev g) {
-  __shared__ double ws[4][QG];
-  __shared__ int flag;
-  GState *st = g.st;
-  if (halted(st)) return;
+  extern __shared__ double lacc[];  // [4][nv]
+  if (halted(g.st)) return;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   const int nb = gridDim.x;
-  const long stride = (long)nb * 256;
-  for (int q0 = 0; q0 < nv; q0 += QG) {
-    const int nq = min(QG, nv - q0);
-    const T *v0 = V + (size_t)q0 * ld;
-    double acc[QG];
+  for (int q = t; q < 4 * nv; q += 256) lacc[q] = 0.0;
+  __syncthreads();
+  double *la = lacc + wave * nv;
+  const long ntile = ((long)M + 256 * R - 1) / (256 * R);
+  for (long tile = blockIdx.x; tile < ntile; tile += nb) {
+    const long base = tile * 256 * R + t;
+    double wr[R];
+    if ((tile + 1) * 256 * R <= M) {
 #pragma unroll
-    for (int k = 0; k < QG; ++k) acc[k] = 0.0;
-    if (nq == QG) {
-      for (long i = (long)blockIdx.x * 256 + t; i < M; i += stride) {
-        const double wi = (double)w[i];
+      for (int r = 0; r < R; ++r) wr[r] = (double)w[base + r * 256];
+      int q = 0;
+      for (; q + U <= nv; q += U) {
+        double a[U];
 #pragma unroll
-        for (int k = 0; k < QG; ++k) acc[k] += wi * (double)v0[(size_t)k * ld + i];
+        for (int u = 0; u < U; ++u) {
+          const T *vq = V + (size_t)(q + u) * ld + base;
+          a[u] = 0.0;
+#pragma unroll
+          for (int r = 0; r < R; ++r) a[u] += wr[r] * (double)vq[r * 256];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const double sum = wave_sum(a[u]);
+          if (lane == 0) la[q + u] += sum;
+        }
       }
-    } else {
-      for (long i = (long)blockIdx.x * 256 + t; i < M; i += stride) {
-        const double wi = (double)w[i];
+      for (; q < nv; ++q) {
+        const T *vq = V + (size_t)q * ld + base;
+        double a = 0.0;
 #pragma unroll
-        for (int k = 0; k < QG; ++k)
-          if (k < nq) acc[k] += wi * (double)v0[(size_t)k * ld + i];
+        for (int r = 0; r < R; ++r) a += wr[r] * (double)vq[r * 256];
+        a = wave_sum(a);
+        if (lane == 0) la[q] += a;
+      }
+    } else {  // ragged last tile
+#pragma unroll
+      for (int r = 0; r < R; ++r) wr[r] = (base + r * 256 < M) ? (double)w[base + r * 256] : 0.0;
+      for (int q = 0; q < nv; ++q) {
+        const T *vq = V + (size_t)q * ld;
+        double a = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (base + r * 256 < M) a += wr[r] * (double)vq[base + r * 256];
+        a = wave_sum(a);
+        if (lane == 0) la[q] += a;
       }
     }
-#pragma unroll
-    for (int k = 0; k < QG; ++k) {
-      double a = wave_sum(acc[k]);
-      if (lane == 0) ws[wave][k] = a;
-    }
-    __syncthreads();
-    if (t < nq)
-      g.part[(size_t)(q0 + t) * nb + blockIdx.x] = (ws[0][t] + ws[1][t]) + (ws[2][t] + ws[3][t]);
-    __syncthreads();
   }
-  if (!last_block(&st->counter, &flag)) return;
-  for (int q = wave; q < nv; q += 4) {
-    const double *pq = g.part + (size_t)q * nb;
-    double a = 0.0;
-    for (int b = lane; b < nb; b += 64) a += pq[b];
-    a = wave_sum(a);
-    if (lane == 0) g.h[q] = g.sig[q] * a;
-  }
-  if (t == 0) st->counter = 0;
+  __syncthreads();
+  for (int q = t; q < nv; q += 256)
+    g.part[(size_t)q * nb + blockIdx.x] =
+        (lacc[q] + lacc[nv + q]) + (lacc[2 * nv + q] + lacc[3 * nv + q]);
 }
 
-// ---- w -= sum_q h_q sig_q w_q ; ||w|| ; Hessenberg column j ----
+// h_q = sig_q * sum of the nb partials of vector q; one wave per vector
+__global__ void __launch_bounds__(256) k_mdot_fin(int nv, int nb, GDev g) {
+  if (halted(g.st)) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int q = blockIdx.x * 4 + wave;
+  if (q >= nv) return;
+  const double *pq = g.part + (size_t)q * nb;
+  double a = 0.0;
+  for (int b = lane; b < nb; b += 64) a += pq[b];
+  a = wave_sum(a);
+  if (lane == 0) g.h[q] = g.sig[q] * a;
+}
+
+// ---- w -= sum_q h_q sig_q w_q ; ||w||^2 partials ----
 // Grid-stride over rows; the update is accumulated in FP64 whatever the
-// storage type, four basis vectors per step (four independent loads in
-// flight per row).
-template <class T>
-__global__ void __launch_bounds__(256) k_maxpy_hess(int M, int nv, const T *__restrict__ V,
-                                                    size_t ld, T *__restrict__ w, GDev g) {
+// storage type, U basis vectors per step (U independent loads in flight per
+// row).
+template <class T, int U>
+__global__ void __launch_bounds__(256) k_maxpy(int M, int nv, const T *__restrict__ V, size_t ld,
+                                               T *__restrict__ w, GDev g) {
   extern __shared__ double cq[];  // nv coefficients
   __shared__ double ws[4];
-  __shared__ int flag;
-  GState *st = g.st;
-  if (halted(st)) return;
+  if (halted(g.st)) return;
   const int t = threadIdx.x;
   for (int q = t; q < nv; q += 256) cq[q] = g.h[q] * g.sig[q];
   __syncthreads();
   const long stride = (long)gridDim.x * 256;
-  const int nv4 = nv & ~3;
+  const int nvU = nv - nv % U;
   double acc = 0.0;
   for (long i = (long)blockIdx.x * 256 + t; i < M; i += stride) {
     double s = (double)w[i];
     const T *vi = V + i;
     int q = 0;
-    for (; q < nv4; q += 4) {
-      const double a0 = (double)vi[(size_t)q * ld], a1 = (double)vi[(size_t)(q + 1) * ld];
-      const double a2 = (double)vi[(size_t)(q + 2) * ld], a3 = (double)vi[(size_t)(q + 3) * ld];
-      s -= cq[q] * a0;
-      s -= cq[q + 1] * a1;
-      s -= cq[q + 2] * a2;
-      s -= cq[q + 3] * a3;
+    for (; q < nvU; q += U) {
+      double a[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) a[u] = (double)vi[(size_t)(q + u) * ld];
+#pragma unroll
+      for (int u = 0; u < U; ++u) s -= cq[q + u] * a[u];
     }
     for (; q < nv; ++q) s -= cq[q] * (double)vi[(size_t)q * ld];
     const T sv = (T)s;
@@ -240,61 +261,86 @@ __global__ void __launch_bounds__(256) k_maxpy_hess(int M, int nv, const T *__re
   }
   acc = block_sum256(acc, ws);
   if (t == 0) g.part[blockIdx.x] = acc;
-  if (!last_block(&st->counter, &flag)) return;
-  double nrm2 = sum_partials(g.part, gridDim.x, ws);
-  if (t != 0) return;
-  st->counter = 0;
-  // KSPGMRESCycle after the orthogonalisation (CGS coefficients in h)
+}
+
+// ---- one block: ||w||, Hessenberg column j, Givens, stopping test ----
+// The column and the previous rotations are staged in LDS by the whole
+// block; the rotation chain is sequential and runs on one thread out of LDS.
+__global__ void __launch_bounds__(256) k_hess(int nb, GDev g) {
+  extern __shared__ double sh[];  // column [j+2], cc [j+1], ss [j+1]
+  __shared__ double ws[4];
+  GState *st = g.st;
+  if (halted(st)) return;
+  const double nrm2 = sum_partials(g.part, nb, ws);
   const int m = st->m;
   const int j = st->it;
-  double tt = sqrt(nrm2);
-  double *hh = g.HH + (size_t)j * (m + 1);
-  for (int q = 0; q <= j; ++q) hh[q] = g.h[q];
-  hh[j + 1] = tt;
-  g.sig[j + 1] = (tt != 0.0) ? 1.0 / tt : 1.0;
-  double hapbnd = fabs(tt / g.grs[j]);
-  if (hapbnd > 1.0e-30) hapbnd = 1.0e-30;
-  const bool hapend = tt < hapbnd;
-  for (int q = 1; q <= j; ++q) {  // KSPGMRESUpdateHessenberg
-    double t0 = hh[q - 1];
-    hh[q - 1] = g.cc[q - 1] * t0 + g.ss[q - 1] * hh[q];
-    hh[q] = g.cc[q - 1] * hh[q] - g.ss[q - 1] * t0;
+  double *hc = sh, *cs = sh + (j + 2), *sn = cs + (j + 1);
+  for (int q = threadIdx.x; q <= j; q += 256) {
+    hc[q] = g.h[q];
+    cs[q] = g.cc[q];
+    sn[q] = g.ss[q];
   }
-  double res;
-  if (!hapend) {
-    double t1 = sqrt(hh[j] * hh[j] + hh[j + 1] * hh[j + 1]);
-    if (t1 == 0.0) {
-      st->reason = KSP_DIVERGED_NULL;
-      return;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // KSPGMRESCycle after the orthogonalisation (CGS coefficients in h)
+    const double tt = sqrt(nrm2);
+    hc[j + 1] = tt;
+    g.sig[j + 1] = (tt != 0.0) ? 1.0 / tt : 1.0;
+    double hapbnd = fabs(tt / g.grs[j]);
+    if (hapbnd > 1.0e-30) hapbnd = 1.0e-30;
+    const bool hapend = tt < hapbnd;
+    double prev = hc[0];
+    for (int q = 1; q <= j; ++q) {  // KSPGMRESUpdateHessenberg
+      const double c = cs[q - 1], sq = sn[q - 1], cur = hc[q];
+      hc[q - 1] = c * prev + sq * cur;
+      prev = c * cur - sq * prev;
     }
-    g.cc[j] = hh[j] / t1;
-    g.ss[j] = hh[j + 1] / t1;
-    g.grs[j + 1] = -(g.ss[j] * g.grs[j]);
-    g.grs[j] = g.cc[j] * g.grs[j];
-    hh[j] = g.cc[j] * hh[j] + g.ss[j] * hh[j + 1];
-    res = fabs(g.grs[j + 1]);
-  } else {
-    res = 0.0;
+    hc[j] = prev;
+    double res = 0.0;
+    int reason = 0;
+    bool null_col = false;
+    if (!hapend) {
+      const double t1 = sqrt(hc[j] * hc[j] + hc[j + 1] * hc[j + 1]);
+      if (t1 == 0.0) {
+        null_col = true;
+      } else {
+        const double cj = hc[j] / t1, sj = hc[j + 1] / t1;
+        g.cc[j] = cj;
+        g.ss[j] = sj;
+        const double gj = g.grs[j];
+        g.grs[j + 1] = -(sj * gj);
+        g.grs[j] = cj * gj;
+        hc[j] = cj * hc[j] + sj * hc[j + 1];
+        res = fabs(g.grs[j + 1]);
+      }
+    }
+    if (null_col) {
+      st->reason = KSP_DIVERGED_NULL;
+    } else {
+      st->it = j + 1;
+      st->its += 1;
+      st->res = res;
+      if (st->monitor) g.hist[st->its] = res;
+      reason = converged(st, st->its, res);
+      if (hapend && !reason) reason = KSP_DIVERGED_BREAKDOWN;
+      if (st->fp32 && (reason == KSP_CONVERGED_RTOL || reason == KSP_CONVERGED_ATOL || hapend)) {
+        // an FP32 estimate only ends the cycle; the FP64 residual at the next
+        // restart decides convergence
+        reason = 0;
+        st->cycle_done = 1;
+      }
+      st->reason = reason;
+      if (!reason) {
+        // FP32 cycles stop at their precision floor and hand back to the FP64
+        // refinement; FP64 cycles run to the restart length like PETSc
+        if (st->fp32 && res <= 2e-6 * st->cyc_res0) st->cycle_done = 1;
+        if (st->it >= m || st->its >= st->max_it) st->cycle_done = 1;
+      }
+    }
   }
-  st->it = j + 1;
-  st->its += 1;
-  st->res = res;
-  if (st->monitor) g.hist[st->its] = res;
-  int reason = converged(st, st->its, res);
-  if (hapend && !reason) reason = KSP_DIVERGED_BREAKDOWN;
-  if (st->fp32 && (reason == KSP_CONVERGED_RTOL || reason == KSP_CONVERGED_ATOL || hapend)) {
-    // an FP32 estimate only ends the cycle; the FP64 residual at the next
-    // restart decides convergence
-    reason = 0;
-    st->cycle_done = 1;
-  }
-  st->reason = reason;
-  if (!reason) {
-    // FP32 cycles stop at their precision floor and hand back to the FP64
-    // refinement; FP64 cycles run to the restart length like PETSc
-    if (st->fp32 && res <= 2e-6 * st->cyc_res0) st->cycle_done = 1;
-    if (st->it >= m || st->its >= st->max_it) st->cycle_done = 1;
-  }
+  __syncthreads();
+  double *hh = g.HH + (size_t)j * (m + 1);
+  for (int q = threadIdx.x; q <= j + 1; q += 256) hh[q] = hc[q];
 }
 
 // ---- KSPGMRESBuildSoln: coef_q = y_q sig_q ----
@@ -379,15 +425,23 @@ __global__ void k_rhs2_bounded(int M, const double *__restrict__ rho, const doub
 constexpr int RED_BLOCKS = 1024;
 constexpr int MAX_RESTART = 1000;
 
-// basis vectors per multi-dot pass: QG + 1 independent row loads in flight
+// Launch shapes of the orthogonalisation kernels, from a sweep over grid
+// sizes and unroll depths at 0.9M and 7.2M rows with 16-200 basis vectors
+// (both ran at 5.2-5.8 TB/s on 7.2M rows).
 template <class T>
 struct Cfg;
 template <>
-struct Cfg<double> { static constexpr int QG = 8; };
+struct Cfg<double> { static constexpr int MAXPY_U = 4; };
 template <>
-struct Cfg<float> { static constexpr int QG = 16; };
-constexpr int MDOT_BLOCKS = 1024;   // partials per vector reduced by one block
-constexpr int MAXPY_BLOCKS = 4096;
+struct Cfg<float> { static constexpr int MAXPY_U = 8; };
+constexpr int MDOT_R = 4, MDOT_U = 2;
+constexpr int MDOT_BLOCKS = 1024;
+constexpr int MAXPY_BLOCKS = 1024;
+
+static int env_int(const char *name, int dflt) {
+  const char *v = getenv(name);
+  return (v && *v) ? std::max(1, atoi(v)) : dflt;
+}
 
 static int ensure_state(apc_ctx *c, int m, int max_it, int M, bool fp32) {
   if (!c->gm) c->gm = new SolveState();
@@ -446,11 +500,15 @@ static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, 
   *S->hst = h0;
   APC_HIP(c, hipMemcpyAsync(S->st.p, S->hst, sizeof(GState), hipMemcpyHostToDevice, c->s));
   APC_HIP(c, hipMemsetAsync(x, 0, sizeof(double) * M, c->s));
-  constexpr int QG = Cfg<T>::QG;
+  constexpr int MU = Cfg<T>::MAXPY_U;
   const size_t ld = (size_t)M;
   const unsigned gb = nblocks(M, 256);
-  const unsigned gdot = (unsigned)std::min<long>(MDOT_BLOCKS, gb);
-  const unsigned gaxpy = (unsigned)std::min<long>(MAXPY_BLOCKS, gb);
+  const long ntile = ((long)M + 256 * MDOT_R - 1) / (256 * MDOT_R);
+  // APC_MDOT_BLOCKS / APC_MAXPY_BLOCKS override the grid sizes (tuning runs)
+  static const int mdot_blocks = env_int("APC_MDOT_BLOCKS", MDOT_BLOCKS);
+  static const int maxpy_blocks = env_int("APC_MAXPY_BLOCKS", MAXPY_BLOCKS);
+  const unsigned gdot = (unsigned)std::min<long>(mdot_blocks, ntile);
+  const unsigned gaxpy = (unsigned)std::min<long>(maxpy_blocks, gb);
   const unsigned gres = (unsigned)std::min<long>(RED_BLOCKS, gb);
   const double nnz = (double)M * A64.lm;
   const double bspmv = fp32 ? 8.0 * nnz + 20.0 * M : 12.0 * nnz + 40.0 * M;
@@ -470,6 +528,7 @@ static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, 
       hipLaunchKernelGGL(k_resid<T>, dim3(gres), dim3(256), 0, c->s, A64, x, b, zero_guess ? 1 : 0,
                          V, g);
     }
+    hipLaunchKernelGGL(k_resid_fin, dim3(1), dim3(256), 0, c->s, (int)gres, g);
     ++ncyc;
     int it_launched = 0;
     const int budget = o.max_it - itcount;
@@ -487,12 +546,22 @@ static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, 
       const int nv = j + 1;
       {
         KScope ks(c, fp32 ? "gmres_mdot32" : "gmres_mdot", (double)sizeof(T) * M * (nv + 1));
-        hipLaunchKernelGGL((k_mdot<T, QG>), dim3(gdot), dim3(256), 0, c->s, M, nv, V, ld, w, g);
+        hipLaunchKernelGGL((k_mdot<T, MDOT_R, MDOT_U>), dim3(gdot), dim3(256),
+                           4 * nv * sizeof(double), c->s, M, nv, V, ld, w, g);
+      }
+      {
+        KScope ks(c, "gmres_mdot_fin", 8.0 * nv * gdot + 16.0 * nv);
+        hipLaunchKernelGGL(k_mdot_fin, dim3(nblocks(nv, 4)), dim3(256), 0, c->s, nv, (int)gdot, g);
       }
       {
         KScope ks(c, fp32 ? "gmres_maxpy32" : "gmres_maxpy", (double)sizeof(T) * M * (nv + 2));
-        hipLaunchKernelGGL((k_maxpy_hess<T>), dim3(gaxpy), dim3(256), nv * sizeof(double), c->s,
-                           M, nv, V, ld, w, g);
+        hipLaunchKernelGGL((k_maxpy<T, MU>), dim3(gaxpy), dim3(256), nv * sizeof(double), c->s, M,
+                           nv, V, ld, w, g);
+      }
+      {
+        KScope ks(c, "gmres_hess");
+        hipLaunchKernelGGL(k_hess, dim3(1), dim3(256), (3 * (size_t)j + 4) * sizeof(double), c->s,
+                           (int)gaxpy, g);
       }
       ++it_launched;
       if ((j + 1) % batch == 0) {
@@ -558,7 +627,8 @@ static int gmres(apc_ctx *c, const Op<double> &A, const double *b, double *x,
 int solve_density_potential(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_opts *o2,
                             apc_solve_info *info) {
   const int M = c->nc + c->nv;
-  if (o1->precision < 0 || o1->precision > 1 || o2->precision < 0 || o2->precision > 1)
+  if (M < 1) return fail(c, APC_ERR_ARG, "solve: no computational particles");
+  if (o1->precision < 0|| o1->precision > 1 || o2->precision < 0 || o2->precision > 1)
     return fail(c, APC_ERR_ARG, "GMRES precision must be 0 (FP64) or 1 (FP32 + FP64 refinement)");
   APC_ALLOC(c, c->rho, M + 1);
   APC_ALLOC(c, c->phi, M + 1);

@@ -93,6 +93,9 @@ struct apc_ctx {
   apc::DBuf<double> grad;  // 9 x M: gx gy gz hxx hyy hzz hxy hxz hyz
   apc::DBuf<double> nscratch;  // neighbour search scratch
   apc::DBuf<int> nscratch_i;
+  apc::DBuf<int> nscratch_f;   // candidate fingerprints
+  apc::DBuf<int> novf;         // nodes searched again with the full list bound
+  int nsearch_long = 0;
   apc::DBuf<int> status_dev;
   apc::SolveState *gm = nullptr;
   // reductions

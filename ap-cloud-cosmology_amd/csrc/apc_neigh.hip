@@ -4,6 +4,18 @@
 // (element e of the chunk's thread t at e*rows + t) so a wave's accesses to
 // the same element coalesce.  Built with -ffp-contract=off: the octant test
 // and the squared distances must match the reference bit for bit.
+//
+// Against the reference's loop structure (same output):
+//  * membership of a candidate is tested on 16-bit fingerprints of the list,
+//    eight per 16-byte load, and confirmed on the stored index;
+//  * octant and distance are computed only for candidates that join the list;
+//  * the octant-ordered snapshot and the penalised sort keys are built in one
+//    pass each from per-octant offsets held in registers;
+//  * a first pass runs with a short candidate list (FAST_CAP); the few nodes
+//    whose expansion outgrows it are queued and searched again with the full
+//    bound.
+#include <cstdlib>
+
 #include "apc_internal.h"
 #include "apc_prims.h"
 
@@ -29,12 +41,49 @@ struct Cand {
   double *d2;    // squared distance
   double *wd;    // sort keys
   int *wm;       // sort payload
+  uint4 *fp;     // word k: 16-bit fingerprints of entries 8k..8k+7
   int rows, t;
   __device__ uint32_t &MO(int e) const { return mo[(size_t)e * rows + t]; }
   __device__ double &D2(int e) const { return d2[(size_t)e * rows + t]; }
   __device__ double &WD(int e) const { return wd[(size_t)e * rows + t]; }
   __device__ int &WM(int e) const { return wm[(size_t)e * rows + t]; }
+  __device__ uint4 FPW(int k) const { return fp[(size_t)k * rows + t]; }
+  __device__ void set_fp(int e, uint32_t f) const {
+    reinterpret_cast<uint16_t *>(fp + (size_t)(e >> 3) * rows + t)[e & 7] = (uint16_t)f;
+  }
 };
+
+__device__ inline uint32_t fprint(uint32_t m) { return (m * 0x9E3779B1u) >> 16; }
+
+// Is m among the first l list entries?
+__device__ bool listed(const Cand &c, int l, uint32_t m) {
+  const uint32_t f = fprint(m);
+  const uint32_t ff = f | (f << 16);
+  const int nw = (l + 7) >> 3;
+  for (int k = 0; k < nw; ++k) {
+    const uint4 w = c.FPW(k);
+    const uint32_t h[4] = {w.x ^ ff, w.y ^ ff, w.z ^ ff, w.w ^ ff};
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int e = 8 * k + 2 * p;
+      if ((h[p] & 0xffffu) == 0 && e < l && (c.MO(e) & 0x1fffffffu) == m) return true;
+      if ((h[p] >> 16) == 0 && e + 1 < l && (c.MO(e + 1) & 0x1fffffffu) == m) return true;
+    }
+  }
+  return false;
+}
+
+// per-octant registers (constant indices only, so they stay in VGPRs)
+__device__ inline int sel8(const int (&a)[8], int o) {
+  int r = a[0];
+#pragma unroll
+  for (int q = 1; q < 8; ++q) r = (q == o) ? a[q] : r;
+  return r;
+}
+__device__ inline void inc8(int (&a)[8], int o) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) a[q] += (q == o) ? 1 : 0;
+}
 
 // apartition / aquickSort on the W arrays, ranges [l, r].  Sub-ranges are
 // disjoint, so processing them from an explicit stack (smaller side first)
@@ -74,35 +123,38 @@ __device__ void qsort_ref(const Cand &c, int l, int r) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_search(int M, int S, int row0, int rows, int lm, int cap,
+constexpr int FAST_CAP = 256;  // multiple of 8 (fingerprint words)
+
+// Rows are row0 + t, or list[t] when a list is given.  With ovf set, a node
+// whose candidate list would exceed cap is appended to ovf (count in
+// status[2]) instead of failing.
+__global__ void __launch_bounds__(256) k_search(int M, int S, int row0, int rows,
+                                                const int *__restrict__ list, int lm, int cap,
                                                 Frame F, const double *__restrict__ cx,
                                                 const double *__restrict__ cy,
                                                 const double *__restrict__ cz,
                                                 const int *__restrict__ n24,
                                                 const int *__restrict__ n24cnt, uint32_t *mo,
-                                                double *d2, double *wd, int *wm,
+                                                double *d2, double *wd, int *wm, uint4 *fpw,
                                                 int *__restrict__ nbr, double *__restrict__ dist,
-                                                int *__restrict__ status) {
+                                                int *__restrict__ status, int *__restrict__ ovf) {
   int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= rows) return;
-  const int i0 = row0 + t;
-  Cand c{mo, d2, wd, wm, rows, t};
+  const int i0 = list ? list[t] : row0 + t;
+  Cand c{mo, d2, wd, wm, fpw, rows, t};
   const int nq = 10 * lm;
   const double x0 = cx[i0], y0 = cy[i0], z0 = cz[i0];
-  auto octant = [&](int m) {
+  // wrapped offset to node m: octant in the rotated frame and squared
+  // distance (solver.cpp:476-479, 595-598)
+  auto place = [&](int m, double *dd) {
     double ex = pwrap(cx[m] - x0, F.len[0], F.bc);
     double ey = pwrap(cy[m] - y0, F.len[1], F.bc);
     double ez = pwrap(cz[m] - z0, F.len[2], F.bc);
     double xt = F.A[0] * ex + F.A[1] * ey + F.A[2] * ez;
     double yt = F.A[3] * ex + F.A[4] * ey + F.A[5] * ez;
     double zt = F.A[6] * ex + F.A[7] * ey + F.A[8] * ez;
+    *dd = ex * ex + ey * ey + ez * ez;
     return (xt > 0) + 2 * (yt > 0) + 4 * (zt > 0);
-  };
-  auto sqd = [&](int m) {  // solver.cpp:595-598
-    double dx = pwrap(cx[m] - x0, F.len[0], F.bc);
-    double dy = pwrap(cy[m] - y0, F.len[1], F.bc);
-    double dz = pwrap(cz[m] - z0, F.len[2], F.bc);
-    return dx * dx + dy * dy + dz * dz;
   };
   int nn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int l = 0;
@@ -111,21 +163,31 @@ __global__ void __launch_bounds__(256) k_search(int M, int S, int row0, int rows
     const int cnt = n24cnt[i0];
     for (int q = 0; q < cnt; ++q) {
       int m = n24[(size_t)i0 * 24 + q];
-      int o = octant(m);
+      double dd;
+      int o = place(m, &dd);
       c.MO(l) = (uint32_t)m | ((uint32_t)o << 29);
-      c.D2(l) = sqd(m);
-      nn[o]++;
+      c.D2(l) = dd;
+      c.set_fp(l, fprint((uint32_t)m));
+      inc8(nn, o);
       l++;
     }
   }
   // expand through the neighbours' Neigh24 until >= lm (solver.cpp:495-551)
   int ite = 0;
   while (l < lm) {
-    // snapshot of the current list in octant order
-    int k = 0;
-    for (int o = 0; o < 8; ++o)
-      for (int e = 0; e < l; ++e)
-        if ((int)(c.MO(e) >> 29) == o) c.WM(k++) = (int)(c.MO(e) & 0x1fffffff);
+    // snapshot of the current list in octant order (stable within an octant)
+    {
+      int off[8];
+      off[0] = 0;
+#pragma unroll
+      for (int q = 1; q < 8; ++q) off[q] = off[q - 1] + nn[q - 1];
+      for (int e = 0; e < l; ++e) {
+        const uint32_t v = c.MO(e);
+        const int o = (int)(v >> 29);
+        c.WM(sel8(off, o)) = (int)(v & 0x1fffffffu);
+        inc8(off, o);
+      }
+    }
     const int lsnap = l;
     for (int a = 0; a < lsnap; ++a) {
       int src = c.WM(a);
@@ -133,18 +195,24 @@ __global__ void __launch_bounds__(256) k_search(int M, int S, int row0, int rows
       for (int q = 0; q < cnt; ++q) {
         int m = n24[(size_t)src * 24 + q];
         if (m >= S) { status[0] = APC_ERR_NEIGH_INVALID; return; }
-        int o = octant(m);
-        bool seen = false;
-        for (int e = 0; e < l; ++e)
-          if ((int)(c.MO(e) & 0x1fffffff) == m) { seen = true; break; }
-        if (!seen && m != i0) {
-          if (nn[o] >= nq) { status[0] = APC_ERR_NEIGH_MANY; return; }
-          if (l >= cap) { status[0] = APC_ERR_UNSUPPORTED; status[1] = i0; return; }
-          c.MO(l) = (uint32_t)m | ((uint32_t)o << 29);
-          c.D2(l) = sqd(m);
-          nn[o]++;
-          l++;
+        if (m == i0 || listed(c, l, (uint32_t)m)) continue;
+        double dd;
+        int o = place(m, &dd);
+        if (sel8(nn, o) >= nq) { status[0] = APC_ERR_NEIGH_MANY; return; }
+        if (l >= cap) {
+          if (ovf) {
+            ovf[atomicAdd(&status[2], 1)] = i0;
+          } else {
+            status[0] = APC_ERR_UNSUPPORTED;
+            status[1] = i0;
+          }
+          return;
         }
+        c.MO(l) = (uint32_t)m | ((uint32_t)o << 29);
+        c.D2(l) = dd;
+        c.set_fp(l, fprint((uint32_t)m));
+        inc8(nn, o);
+        l++;
       }
     }
     ite++;
@@ -155,35 +223,52 @@ __global__ void __launch_bounds__(256) k_search(int M, int S, int row0, int rows
   // touches only octant 0's zero-initialised tail unless it reaches slot nq.
   {
     int ko = nn[0];
+#pragma unroll
     for (int o = 1; o < 8; ++o) {
       if (nn[o] >= 2 && ko + nn[o] - 1 >= nq) { status[0] = APC_ERR_UNSUPPORTED; status[1] = i0; return; }
       ko += nn[o];
     }
   }
-  // octant 0 in insertion order, quicksorted on the squared distance
-  int k = 0;
-  for (int e = 0; e < l; ++e)
-    if ((c.MO(e) >> 29) == 0) { c.WD(k) = c.D2(e); c.WM(k) = (int)(c.MO(e) & 0x1fffffff); ++k; }
+  // keys: octant 0 first (raw distances, insertion order), then octants 1..7
+  // in insertion order with the rank penalty 1e4^k applied
+  {
+    int off[8];
+    off[0] = 0;
+#pragma unroll
+    for (int q = 1; q < 8; ++q) off[q] = off[q - 1] + nn[q - 1];
+    double pen[8] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
+    for (int e = 0; e < l; ++e) {
+      const uint32_t v = c.MO(e);
+      const int o = (int)(v >> 29);
+      const int pos = sel8(off, o);
+      inc8(off, o);
+      double key = c.D2(e);
+      if (o != 0) {
+        double p = pen[1];
+#pragma unroll
+        for (int q = 2; q < 8; ++q) p = (q == o) ? pen[q] : p;
+        key = p * key;
+#pragma unroll
+        for (int q = 1; q < 8; ++q)
+          if (q == o) pen[q] = pen[q] * 10000.0;
+      }
+      c.WD(pos) = key;
+      c.WM(pos) = (int)(v & 0x1fffffffu);
+    }
+  }
+  // octant 0 quicksorted on the squared distance, then penalised
   if (nn[0] > 1) qsort_ref(c, 0, nn[0] - 1);
   {
     double pen = 1.0;
     for (int q = 0; q < nn[0]; ++q) { c.WD(q) = pen * c.WD(q); pen = pen * 10000.0; }
   }
-  for (int o = 1; o < 8; ++o) {
-    double pen = 1.0;
-    for (int e = 0; e < l; ++e)
-      if ((int)(c.MO(e) >> 29) == o) {
-        c.WD(k) = pen * c.D2(e);
-        c.WM(k) = (int)(c.MO(e) & 0x1fffffff);
-        pen = pen * 10000.0;
-        ++k;
-      }
-  }
   qsort_ref(c, 0, l - 1);
   for (int q = 0; q < lm; ++q) {
     int m = c.WM(q);
+    double dd;
+    (void)place(m, &dd);
     nbr[(size_t)q * M + i0] = m;
-    dist[(size_t)q * M + i0] = sqrt(sqd(m));
+    dist[(size_t)q * M + i0] = sqrt(dd);
   }
 }
 
@@ -213,26 +298,52 @@ int search_neighbors(apc_ctx *c) {
     F.len[2] = b[5] - b[4];
     F.bc = c->prm.bc;
   }
-  // candidate capacity: an expansion round starts with fewer than lm
+  // full candidate bound: an expansion round starts with fewer than lm
   // candidates, each adding at most 24 (the 2:1 bound), so 25*lm suffices
-  const int cap = 25 * lm + 32;
-  const int chunk = std::max(1, std::min(M, 131072));
-  APC_ALLOC(c, c->nscratch, (size_t)2 * cap * chunk);
-  APC_ALLOC(c, c->nscratch_i, (size_t)2 * cap * chunk);
+  const int cap_full = (25 * lm + 32 + 7) & ~7;
+  const int chunk = std::max(1, std::min(M, 1 << 19));
+  const int chunk_full = std::max(1, std::min(M, 1 << 16));
+  // APC_NEIGH_FAST_CAP shortens the first-pass list (tests drive the second
+  // pass with it); clamped to [32, FAST_CAP], multiple of 8
+  int fast_cap = FAST_CAP;
+  if (const char *v = getenv("APC_NEIGH_FAST_CAP"))
+    fast_cap = std::min(FAST_CAP, std::max(32, atoi(v))) & ~7;
+  const size_t per_row_fast = (size_t)fast_cap, per_row_full = (size_t)cap_full;
+  const size_t elems = std::max(per_row_fast * chunk, per_row_full * chunk_full);
+  APC_ALLOC(c, c->nscratch, 2 * elems);
+  APC_ALLOC(c, c->nscratch_i, 2 * elems);
+  APC_ALLOC(c, c->nscratch_f, elems / 2 + 4);  // 16-bit fingerprints as int
+  APC_ALLOC(c, c->novf, (size_t)M + 1);
   uint32_t *mo = (uint32_t *)c->nscratch_i.p;
-  int *wm = c->nscratch_i.p + (size_t)cap * chunk;
+  int *wm = c->nscratch_i.p + elems;
   double *d2 = c->nscratch.p;
-  double *wd = c->nscratch.p + (size_t)cap * chunk;
+  double *wd = c->nscratch.p + elems;
+  uint4 *fpw = (uint4 *)c->nscratch_f.p;
   for (int row0 = 0; row0 < M; row0 += chunk) {
     int rows = std::min(chunk, M - row0);
     KScope ks(c, "neighbor_search");
     hipLaunchKernelGGL(k_search, dim3(nblocks(rows, 256)), dim3(256), 0, c->s, M, S, row0, rows,
-                       lm, cap, F, c->cx.p, c->cy.p, c->cz.p, c->n24.p, c->n24cnt.p, mo, d2, wd,
-                       wm, c->nbr.p, c->dist.p, c->status_dev.p);
+                       (const int *)nullptr, lm, fast_cap, F, c->cx.p, c->cy.p, c->cz.p, c->n24.p,
+                       c->n24cnt.p, mo, d2, wd, wm, fpw, c->nbr.p, c->dist.p, c->status_dev.p,
+                       c->novf.p);
   }
-  int st[2] = {0, 0};
+  int st[3] = {0, 0, 0};
   APC_HIP(c, hipMemcpyAsync(st, c->status_dev.p, sizeof(st), hipMemcpyDeviceToHost, c->s));
   APC_HIP(c, hipStreamSynchronize(c->s));
+  // nodes whose expansion outgrew the short list, with the full bound
+  for (int r0 = 0; st[0] == 0 && r0 < st[2]; r0 += chunk_full) {
+    int rows = std::min(chunk_full, st[2] - r0);
+    KScope ks(c, "neighbor_search_long");
+    hipLaunchKernelGGL(k_search, dim3(nblocks(rows, 256)), dim3(256), 0, c->s, M, S, 0, rows,
+                       (const int *)c->novf.p + r0, lm, cap_full, F, c->cx.p, c->cy.p, c->cz.p,
+                       c->n24.p, c->n24cnt.p, mo, d2, wd, wm, fpw, c->nbr.p, c->dist.p,
+                       c->status_dev.p, (int *)nullptr);
+  }
+  if (st[2] > 0 && st[0] == 0) {
+    APC_HIP(c, hipMemcpyAsync(st, c->status_dev.p, 2 * sizeof(int), hipMemcpyDeviceToHost, c->s));
+    APC_HIP(c, hipStreamSynchronize(c->s));
+  }
+  c->nsearch_long = st[2];
   if (st[0]) {
     if (st[0] == APC_ERR_UNSUPPORTED)
       return fail(c, st[0], "neighbours: node %d needs more than the supported candidate list", st[1]);

@@ -138,6 +138,27 @@ def test_neighbor_lists(kind, n, bc, depth, c, lm):
 
 
 @pytest.mark.parametrize("kind,n,bc,c,lm", [
+    ("clustered", 6000, 1, 0.05, 17),
+    ("plummer", 20000, 0, 0.02, 27),
+])
+def test_neighbor_lists_second_pass(monkeypatch, kind, n, bc, c, lm):
+    """A 32-entry first-pass list sends most nodes through the second,
+    full-bound pass; the lists must not change."""
+    monkeypatch.setenv("APC_NEIGH_FAST_CAP", "32")
+    x, y, z = cloud(kind, n, 14)
+    g, o = pair(x, y, z, bc=bc, depth=20, c=c, lm=lm)
+    gc, oc = _selected(g, o, c)
+    p = relabel(gc, oc)
+    g.set_timing(True)
+    gn, gd = g.neighbors()
+    assert g.kernel_times().get("neighbor_search_long", (0, 0, 0))[1] > 0
+    on, od = o.neighbors()
+    M = gn.shape[0]
+    assert np.array_equal(p[gn], on[p[:M]])
+    assert np.array_equal(gd, od[p[:M]])
+
+
+@pytest.mark.parametrize("kind,n,bc,c,lm", [
     ("plummer", 20000, 1, 0.01, 27),
     ("ball", 20000, 0, 0.02, 17),
     ("jitter", 32768, 1, 0.01, 17),
