@@ -1,5 +1,6 @@
 """Benchmark: particle-steps/s of the AP-Cloud solve-and-move step on one
-MI355X.
+MI355X, FP32-inner / FP64-refinement GMRES (the precision form) against the
+plain FP64 GPU step, with the force error between the two.
 
 A step is one pass of the reference's time loop body (controller.cpp:119-208
 without output): Morton keys + radix sort + selection + Neigh24, neighbour
@@ -11,10 +12,18 @@ Workloads (synthetic, regenerated from their seeds):
   zeldovich256  256^3 Einstein-de Sitter perturbed lattice, P(k)~k^-2, RMS
                 displacement 0.04, 64 Plummer spheres (b=0.005, 20% of the
                 particles), seed 2; 10 steps a=0.02..0.1 (the precision-form
-                configuration).
+                configuration; default).
   zeldovich128  128^3 lattice, RMS 0.005, seed 1; 20 steps a=0.02..0.1.
-Both: leaf capacity 8, 27 GFD neighbours, GMRES restart 200, max 1000
+  zeldovich64   64^3 lattice, RMS 0.01, seed 4; 10 steps.
+All: leaf capacity 8, 27 GFD neighbours, GMRES restart 200, max 1000
 iterations (the reference's own run options, Makefile:30), rtol 1e-8.
+
+Legs, all from the same state (the initial conditions after the initial half
+drift): (1) accuracy: one FP64 step and one step of the measured precision,
+relative RMS difference of the accelerations and largest position
+difference; (2) the FP64 step timed over K steps (skipped with --no-fp64);
+(3) the measured precision timed over K steps -> "value".  The roofline
+entry is for the kernel with the largest device time in leg (3).
 """
 import argparse
 import json
@@ -34,11 +43,13 @@ WORKLOADS = {
 }
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s at 256^3: FP32-IR batched vs FP64 GPU at 1e-6 rel force err"
 
 
-def make_ic(w):
+def make_ic(w, n_side=None):
     from apcloud import ic
-    x, y, z, vx, vy, vz = ic.zeldovich(w["n_side"], seed=w["seed"], rms=w["rms"])
+    n = n_side or w["n_side"]
+    x, y, z, vx, vy, vz = ic.zeldovich(n, seed=w["seed"], rms=w["rms"])
     if w["plummer"]:
         x, y, z, vx, vy, vz = ic.plummer_clusters(x, y, z, vx, vy, vz, w["seed"], n_spheres=64,
                                                   scale=0.005, fraction=0.2)
@@ -46,9 +57,25 @@ def make_ic(w):
 
 
 def eds(a):
-    """units mode 2 with Omega=1: C_dens, C_pot, C_field, Cx(a), Cv(a)."""
+    """units mode 2 with Omega=1: C_pot, C_field, Cx(a), Cv(a)."""
     cx = 1.0 / np.sqrt(1.0 / a)
     return 1.5 / a, -1.0, cx, cx / a / a
+
+
+def traffic_from_profiles(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (profiles/pmc_traffic.json, written from rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs scaled by a copy kernel of known size), or None."""
+    path = os.path.join(REPO, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        e = d["kernels"][kernel]
+        if d.get("workload") != workload:
+            return None
+        return float(e["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError, TypeError):
+        return None
 
 
 def main():
@@ -58,11 +85,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="zeldovich256", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-side", type=int, default=64, help="lattice side of the CPU sample")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the timed FP64 leg")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the force-error leg")
+    ap.add_argument("--cpu-side", type=int, default=96, help="lattice side of the CPU sample")
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--restart", type=int, default=200)
     ap.add_argument("--max-it", type=int, default=1000)
-    ap.add_argument("--precision", type=int, default=0, help="0 FP64 GMRES, 1 FP32 + FP64 refinement")
+    ap.add_argument("--precision", type=int, default=1,
+                    help="measured form: 1 FP32 GMRES + FP64 refinement (default), 0 FP64")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -82,40 +112,69 @@ def main():
                                  leaf_capacity=8)
     ctx = apcloud.Context(params)
     ctx.set_particles(x, y, z, vx, vy, vz)
-    opts = apcloud.ksp(rtol=args.rtol, restart=args.restart, max_it=args.max_it,
-                       precision=args.precision)
     a_start, a_end = 0.02, 0.1
     da = (a_end - a_start) / nsteps_total
-    t = a_start
-    _, _, cx0, _ = eds(t)
+    _, _, cx0, _ = eds(a_start)
     ctx.drift(cx0, 0.5 * da)  # initial half drift (controller.cpp:106-111)
-    t += 0.5 * da
+    t_state0 = a_start + 0.5 * da
+    state0 = ctx.particles()[:6]
     cd = 1.0 / N
 
-    def one_step(t):
+    def opts(precision):
+        return apcloud.ksp(rtol=args.rtol, restart=args.restart, max_it=args.max_it,
+                           precision=precision)
+
+    def one_step(t, o):
         dt = min(da, a_end - t) if t + da > a_end else da
         cpot, cf, _, cv = eds(t)
         _, _, cxh, _ = eds(t + dt / 2)
-        mdv, mdx, info, timing = ctx.step(cd, cpot, cf, cv, cxh, dt, opts)
+        _, _, info, timing = ctx.step(cd, cpot, cf, cv, cxh, dt, o)
         return t + dt, info, timing
 
-    for _ in range(W):
-        t, info, _ = one_step(t)
-    ctx.sync()
-    ctx.set_timing(True)
-    ctx.reset_kernel_times()
-    infos = []
-    phases = np.zeros(3)
-    t0 = time.perf_counter()
-    for _ in range(K):
-        t, info, timing = one_step(t)
-        infos.append((info.its_rho, info.its_phi))
-        phases += timing
-    ctx.sync()
-    wall = time.perf_counter() - t0
-    kt = ctx.kernel_times()
-    ctx.set_timing(False)
-    M = info.nc + info.nv
+    # (1) accuracy: one step of each form from the same state
+    acc = None
+    if not args.no_accuracy:
+        ctx.set_particles(*state0)
+        one_step(t_state0, opts(0))
+        ref = ctx.particles()
+        ctx.set_particles(*state0)
+        one_step(t_state0, opts(args.precision))
+        got = ctx.particles()
+        da_ = np.stack([got[6] - ref[6], got[7] - ref[7], got[8] - ref[8]])
+        a64 = np.stack(ref[6:9])
+        dx = np.stack([got[k] - ref[k] for k in range(3)])
+        dx -= np.round(dx)  # periodic unit box
+        acc = {"rel_rms_force_err": float(np.sqrt((da_ ** 2).sum() / (a64 ** 2).sum())),
+               "max_abs_pos_err": float(np.abs(dx).max()),
+               "vs": "FP64 GPU step, same state"}
+
+    def timed(precision):
+        ctx.set_particles(*state0)
+        o = opts(precision)
+        t = t_state0
+        for _ in range(W):
+            t, _, _ = one_step(t, o)
+        ctx.sync()
+        ctx.set_timing(True)
+        ctx.reset_kernel_times()
+        infos, phases = [], np.zeros(3)
+        t0 = time.perf_counter()
+        for _ in range(K):
+            t, info, timing = one_step(t, o)
+            infos.append((info.its_rho, info.its_phi))
+            phases += timing
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        kt = ctx.kernel_times()
+        ctx.set_timing(False)
+        return wall, infos, phases, kt, info.nc + info.nv
+
+    fp64 = None
+    if not args.no_fp64 and args.precision != 0:
+        wall64, infos64, _, _, _ = timed(0)
+        fp64 = {"value": round(N * K / wall64, 1), "ms_per_step": round(1e3 * wall64 / K, 2),
+                "gmres_its": infos64}
+    wall, infos, phases, kt, M = timed(args.precision)
 
     # roofline of the dominant kernel: largest total device time among the
     # kernels whose launches carry algorithmic byte counts
@@ -132,13 +191,14 @@ def main():
         name, b, avg_ms, _, cnt = best
         ach = b / (avg_ms * 1e-3) / 1e9
         roof = {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profiles(name, args.workload),
                 "bytes_per_launch": int(b), "avg_launch_ms": round(avg_ms, 4), "launches": cnt,
                 "measured_copy_GBs": round(copy_gbs, 1)}
 
     value = N * K / wall
     out = {
-        "metric": "particle-steps/s (solve-and-move step, 1 GPU)",
+        "metric": METRIC,
         "value": round(value, 1),
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -148,12 +208,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if args.precision == 1 else "f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "particles": N, "rows": M, "lm": params.lm,
-                   "leaf_capacity": 8, "gmres": {"rtol": args.rtol, "restart": args.restart,
-                                                 "max_it": args.max_it},
+                   "leaf_capacity": 8,
+                   "gmres": {"rtol": args.rtol, "restart": args.restart, "max_it": args.max_it,
+                             "precision": "fp32-inner/fp64-refinement" if args.precision == 1
+                             else "fp64"},
                    "gmres_its": infos, "phase_s": [round(v, 3) for v in phases]},
+        "fp64_gpu": fp64,
+        "speedup_vs_fp64": round(value / fp64["value"], 3) if fp64 else None,
+        "accuracy": acc,
         "roofline": roof,
         "cpu_baseline": None,
     }
@@ -164,37 +229,39 @@ def main():
     ctx.close()
 
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args)
+        out["cpu_baseline"] = cpu_baseline(args, w)
     print(json.dumps(out))
 
 
-def cpu_baseline(args):
+def cpu_baseline(args, w):
     """The oracle (the FP64 CPU restatement, OpenMP where the reference has
-    it) timed on this host on a bounded sample: one step of an n^3
-    perturbed lattice with the same settings."""
+    it) timed on this host on a bounded sample: one FP64 step of the same
+    workload generator at --cpu-side^3 particles, same solver settings."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as po
-    from apcloud import ic
 
     n_side = args.cpu_side
-    x, y, z, vx, vy, vz = ic.zeldovich(n_side, seed=1, rms=0.005)
+    x, y, z, vx, vy, vz = make_ic(w, n_side)
     N = len(x)
-    cores = os.cpu_count() or 1
+    cores = min(16, os.cpu_count() or 1)  # this process's share of a one-GPU host
     p = po.make_params(bbox=(0, 1, 0, 1, 0, 1), depth=20, bc=1, c=0.0, lm=27, units=2, omega=1.0,
                        mass_per_particle=1.0 / N, rtol=args.rtol, restart=args.restart,
                        max_it=args.max_it, nthreads=cores, leaf_capacity=8)
     o = po.Oracle(p)
     o.set_particles(x, y, z, vx, vy, vz)
+    a = 0.02 + 0.004
     t0 = time.perf_counter()
     o.build_tree()
     S = o.select()
-    o.set_constants(1.0 / N, 1.5 / 0.022, -1.0)
+    o.set_constants(1.0 / N, 1.5 / a, -1.0)
     o.solve()
-    o.move_velocity(1.0, 0.004)
-    o.move_position(1.0, 0.004)
+    o.move_velocity(1.0, 0.008)
+    o.move_position(1.0, 0.008)
     wall = time.perf_counter() - t0
     return {"value": round(N / wall, 1), "unit": "particle-steps/s", "cores": cores,
-            "kind": "port", "sample": "1 step, %d^3 perturbed lattice (S=%d nodes)" % (n_side, S)}
+            "kind": "port",
+            "sample": "1 FP64 step, %d^3 particles from the %s generator (S=%d nodes), %.1f s"
+                      % (n_side, args.workload, S, wall)}
 
 
 if __name__ == "__main__":
