@@ -28,7 +28,7 @@ EXPORTS = [
     "apc_set_particles", "apc_get_particles", "apc_octree_build", "apc_get_sorted",
     "apc_octree_level", "apc_select", "apc_get_comp_sizes", "apc_get_comp", "apc_neighbors",
     "apc_solver_set_neighbors", "apc_solver_set_constants", "apc_build_system",
-    "apc_solver_solve", "apc_differentiate",
+    "apc_solver_solve", "apc_solver_amg_levels", "apc_differentiate",
     "apc_get_fields", "apc_set_gradients", "apc_interp_kick", "apc_drift", "apc_max_density",
     "apc_step", "apc_kernel_times", "apc_reset_kernel_times", "apc_set_timing", "apc_sync",
     "apc_stream_copy",
@@ -48,7 +48,7 @@ class Params(C.Structure):
 class KspOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("dtol", C.c_double),
                 ("max_it", C.c_int), ("restart", C.c_int), ("monitor", C.c_int),
-                ("precision", C.c_int)]
+                ("precision", C.c_int), ("pc_type", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -141,10 +141,16 @@ def make_params(bbox=(0, 1, 0, 1, 0, 1), depth=20, bc=1, c=1.0, lm=17, **kw):
     return p
 
 
-def ksp(rtol=1e-5, atol=1e-50, dtol=1e5, max_it=10000, restart=30, monitor=0, precision=0):
+PC_TYPES = {"jacobi": 0, "amg": 1, "gamg": 1}
+
+
+def ksp(rtol=1e-5, atol=1e-50, dtol=1e5, max_it=10000, restart=30, monitor=0, precision=0,
+        pc="jacobi"):
+    """KSP options (PETSc defaults); pc: "jacobi" (the reference's) or "amg"."""
     o = KspOpts()
     o.rtol, o.atol, o.dtol = rtol, atol, dtol
     o.max_it, o.restart, o.monitor, o.precision = max_it, restart, monitor, precision
+    o.pc_type = PC_TYPES[pc] if isinstance(pc, str) else int(pc)
     return o
 
 
@@ -327,6 +333,15 @@ class Context:
                                   C.byref(phi_opts), C.byref(mdv), C.byref(mdx), C.byref(info),
                                   _p(timing)), "apc_step")
         return mdv.value, mdx.value, info, timing
+
+    def amg_levels(self):
+        """[(rows, nnz)] of the multigrid built by the last pc="amg" solve."""
+        rows = np.zeros(32, np.int32)
+        nnz = np.zeros(32, np.int64)
+        n = C.c_int()
+        self._chk(self.L.apc_solver_amg_levels(self.h, _p(rows, _I), nnz.ctypes.data_as(C.c_void_p),
+                                               32, C.byref(n)), "apc_solver_amg_levels")
+        return [(int(rows[k]), int(nnz[k])) for k in range(min(n.value, 32))]
 
     def set_timing(self, on=True):
         self._chk(self.L.apc_set_timing(self.h, 1 if on else 0), "apc_set_timing")

@@ -1,0 +1,832 @@
+// Aggregation multigrid preconditioner for the GFD systems.
+//
+// The reference configures its solves with KSPSetFromOptions after choosing
+// GMRES + Jacobi (solver.cpp:197-202, 269-274), so its users can select
+// another preconditioner on the command line (`-pc_type gamg`).  This file is
+// that option for the GPU path: an algebraic multigrid built on the device
+// every solve, applied as a right preconditioner so GMRES converges on the
+// true residual.
+//
+//  * Aggregation: maximal independent set on the matrix graph (priority = a
+//    bijective hash of the row index, so the set is deterministic); every
+//    other row joins the root it is most strongly coupled to.
+//  * Coarse operators: Galerkin with the piecewise-constant prolongation,
+//    A_c[I,J] = sum of a_ij over i in I, j in J, formed by a stable radix
+//    sort of (I,J) keys and a segmented sum (deterministic).
+//  * Smoother: damped Jacobi; correction scaled by alpha; V- or W-cycle.
+//  * Coarsest level: dense inverse by Gauss-Jordan with partial pivoting.  On
+//    the periodic operator (one identity row pins the constant mode) the
+//    constant mode is nearly singular; the coarsest matrix gets the rank-one
+//    shift sigma*11^T/n before inversion so that the preconditioner does not
+//    amplify it (GMRES resolves that one direction itself).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "apc_solve.h"
+
+namespace apc {
+
+template <class T>
+struct Csr {
+  const int *rp, *col;
+  const T *val;
+  int n;
+};
+
+struct AmgLevel {
+  int n = 0, nnz = 0;
+  DBuf<int> rp, col;
+  DBuf<double> val, dinv;
+  DBuf<float> val32, dinv32;
+  // sliced ELL copy for the cycle: slices of 64 rows, slot-major within a
+  // slice, each padded to its longest row (padding: own column, value 0)
+  long long sell_len = 0;
+  DBuf<int> soff, swid, scol;  // slice offsets (elements), widths, columns
+  DBuf<double> sval;
+  DBuf<float> sval32;
+  DBuf<int> agg;      // coarse row of each row (all but the coarsest level)
+  DBuf<int> mp, ml;   // rows of each coarse row, grouped (CSR over next level)
+  DBuf<double> x, b, r, t;
+  DBuf<float> x32, b32, r32, t32;
+  void release() {
+    soff.release(); swid.release(); scol.release(); sval.release(); sval32.release();
+    rp.release(); col.release(); val.release(); dinv.release(); val32.release();
+    dinv32.release(); agg.release(); mp.release(); ml.release(); x.release(); b.release();
+    r.release(); t.release(); x32.release(); b32.release(); r32.release(); t32.release();
+  }
+};
+
+struct AmgHier {
+  std::vector<AmgLevel *> L;
+  int nlev = 0;
+  int nc = 0;
+  bool dense = false;
+  DBuf<double> gj, cinv, colk;
+  DBuf<float> cinv32;
+  AmgOptions o;
+  bool fp32 = false;
+  DBuf<uint64_t> key, key2;
+  DBuf<int> idx, idx2, sa, sb, flag, pos, cnt;
+  DBuf<double> ev, smax, diag;
+  ~AmgHier() {
+    for (auto *l : L) { l->release(); delete l; }
+    gj.release(); cinv.release(); colk.release(); cinv32.release(); key.release();
+    key2.release(); idx.release(); idx2.release(); sa.release(); sb.release(); flag.release();
+    pos.release(); cnt.release(); ev.release(); smax.release(); diag.release();
+  }
+};
+
+// ---------------------------------------------------------------- views
+// Off-diagonal entries of row i (duplicates included) and the diagonal.
+struct EllView {
+  Op<double> A;
+  __device__ int n() const { return A.M; }
+  __device__ double diag(int i) const { return A.d[i]; }
+  template <class F>
+  __device__ void offdiag(int i, F f) const {
+    const double fi = A.f[i];
+    for (int q = 0; q < A.lm; ++q) {
+      int m = A.nbr[(size_t)q * A.M + i];
+      if (m < A.M) f(m, fi * A.s[(size_t)q * A.M + i]);
+    }
+  }
+};
+
+struct CsrView {
+  Csr<double> A;
+  __device__ int n() const { return A.n; }
+  __device__ double diag(int i) const {
+    double d = 0.0;
+    for (int e = A.rp[i]; e < A.rp[i + 1]; ++e)
+      if (A.col[e] == i) d += A.val[e];
+    return d;
+  }
+  template <class F>
+  __device__ void offdiag(int i, F f) const {
+    for (int e = A.rp[i]; e < A.rp[i + 1]; ++e)
+      if (A.col[e] != i) f(A.col[e], A.val[e]);
+  }
+};
+
+__device__ inline uint32_t prio_of(uint32_t i) {  // bijective on 32 bits
+  uint32_t x = i * 0x9E3779B1u;
+  x ^= x >> 16;
+  x *= 0x85EBCA6Bu;
+  x ^= x >> 13;
+  return x;
+}
+
+// ---------------------------------------------------------------- setup kernels
+template <class V>
+__global__ void __launch_bounds__(256) k_amg_smax(V v, double *__restrict__ smax) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= v.n()) return;
+  double m = 0.0;
+  v.offdiag(i, [&](int, double a) { m = fmax(m, fabs(a)); });
+  smax[i] = m;
+}
+
+// MIS round, part 1: an undecided row whose priority beats every undecided
+// strong neighbour becomes a root.  a: states in (0 undecided, 1 root,
+// 2 covered), b: states out.
+template <class V>
+__global__ void __launch_bounds__(256) k_amg_pick(V v, double theta, const double *__restrict__ smax,
+                                                  const int *__restrict__ a, int *__restrict__ b) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= v.n()) return;
+  int s = a[i];
+  if (s != 0) { b[i] = s; return; }
+  const uint32_t pi = prio_of((uint32_t)i);
+  const double lim = theta * smax[i];
+  bool best = true;
+  v.offdiag(i, [&](int j, double x) {
+    if (fabs(x) >= lim && a[j] == 0 && prio_of((uint32_t)j) > pi) best = false;
+  });
+  b[i] = best ? 1 : 0;
+}
+
+// MIS round, part 2: an undecided row with a strong root neighbour is
+// covered; the rest are counted.
+template <class V>
+__global__ void __launch_bounds__(256) k_amg_cover(V v, double theta, const double *__restrict__ smax,
+                                                   int *__restrict__ b, int *__restrict__ undecided) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= v.n()) return;
+  if (b[i] != 0) return;
+  const double lim = theta * smax[i];
+  bool cov = false;
+  v.offdiag(i, [&](int j, double x) {
+    if (fabs(x) >= lim && b[j] == 1) cov = true;
+  });
+  if (cov) b[i] = 2;
+  else atomicAdd(undecided, 1);
+}
+
+__global__ void k_amg_rootflag(int n, const int *__restrict__ st, int *__restrict__ flag) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) flag[i] = (st[i] == 1) ? 1 : 0;
+}
+
+// Aggregate of every row: a root's rank, or the strongest root neighbour's
+// (largest |a_ij|, then smallest j).
+template <class V>
+__global__ void __launch_bounds__(256) k_amg_agg(V v, double theta, const double *__restrict__ smax,
+                                                 const int *__restrict__ st,
+                                                 const int *__restrict__ rank, int *__restrict__ agg,
+                                                 int *__restrict__ bad) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= v.n()) return;
+  if (st[i] == 1) { agg[i] = rank[i]; return; }
+  const double lim = theta * smax[i];
+  double bv = -1.0;
+  int bj = -1;
+  v.offdiag(i, [&](int j, double x) {
+    double ax = fabs(x);
+    if (ax >= lim && st[j] == 1 && (ax > bv || (ax == bv && j < bj))) { bv = ax; bj = j; }
+  });
+  if (bj < 0) { atomicAdd(bad, 1); agg[i] = 0; return; }
+  agg[i] = rank[bj];
+}
+
+__global__ void k_amg_keys_rows(int n, const int *__restrict__ agg, uint64_t *__restrict__ key,
+                                int *__restrict__ idx) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  key[i] = (uint64_t)agg[i];
+  idx[i] = i;
+}
+
+// first sorted position of every key (keys 0..nc-1 all present)
+__global__ void k_amg_bounds(int n, int nc, const uint64_t *__restrict__ key, int *__restrict__ mp) {
+  int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  if (k == 0 || key[k] != key[k - 1]) mp[key[k]] = k;
+  if (k == n - 1) mp[nc] = n;
+}
+
+constexpr uint64_t SENT = ~0ull;
+
+// Galerkin entries from the ELL level: entry q*M + i, q = 0 the diagonal.
+__global__ void __launch_bounds__(256) k_amg_entries_ell(Op<double> A, const int *__restrict__ agg,
+                                                         int nc, uint64_t *__restrict__ key,
+                                                         double *__restrict__ ev,
+                                                         int *__restrict__ idx) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= A.M) return;
+  const uint64_t I = (uint64_t)agg[i];
+  const double fi = A.f[i];
+  key[i] = I * nc + I;
+  ev[i] = A.d[i];
+  idx[i] = i;
+  for (int q = 0; q < A.lm; ++q) {
+    const size_t e = (size_t)(q + 1) * A.M + i;
+    int m = A.nbr[(size_t)q * A.M + i];
+    if (m < A.M) {
+      key[e] = I * nc + (uint64_t)agg[m];
+      ev[e] = fi * A.s[(size_t)q * A.M + i];
+    } else {
+      key[e] = SENT;
+      ev[e] = 0.0;
+    }
+    idx[e] = (int)e;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_amg_entries_csr(Csr<double> A, const int *__restrict__ agg,
+                                                         int nc, uint64_t *__restrict__ key,
+                                                         double *__restrict__ ev,
+                                                         int *__restrict__ idx) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= A.n) return;
+  const uint64_t I = (uint64_t)agg[i];
+  for (int e = A.rp[i]; e < A.rp[i + 1]; ++e) {
+    key[e] = I * nc + (uint64_t)agg[A.col[e]];
+    ev[e] = A.val[e];
+    idx[e] = e;
+  }
+}
+
+__global__ void k_amg_heads(int n, uint64_t lim, const uint64_t *__restrict__ key,
+                            int *__restrict__ flag) {
+  int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  flag[k] = (key[k] < lim && (k == 0 || key[k] != key[k - 1])) ? 1 : 0;
+}
+
+// Segmented sum of the sorted entries into the coarse CSR.
+__global__ void __launch_bounds__(256) k_amg_reduce(int n, int nc, const uint64_t *__restrict__ key,
+                                                    const int *__restrict__ idx,
+                                                    const double *__restrict__ ev,
+                                                    const int *__restrict__ flag,
+                                                    const int *__restrict__ pos,
+                                                    int *__restrict__ rp, int *__restrict__ col,
+                                                    double *__restrict__ val,
+                                                    double *__restrict__ diag) {
+  int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n || !flag[k]) return;
+  const uint64_t kk = key[k];
+  double s = 0.0;
+  for (int q = k; q < n && key[q] == kk; ++q) s += ev[idx[q]];
+  const int p = pos[k];
+  const int I = (int)(kk / (uint64_t)nc), J = (int)(kk % (uint64_t)nc);
+  col[p] = J;
+  val[p] = s;
+  if (J == I) diag[I] = s;
+  if (k == 0 || (int)(key[k - 1] / (uint64_t)nc) != I) rp[I] = p;
+}
+
+__global__ void k_amg_dinv(int n, const double *__restrict__ diag, double *__restrict__ dinv) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dinv[i] = (diag[i] != 0.0) ? 1.0 / diag[i] : 0.0;
+}
+
+// width of each 64-row slice
+__global__ void k_sell_width(int n, const int *__restrict__ rp, int *__restrict__ wid) {
+  const int s = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int r = s * 64 + lane;
+  int w = (r < n) ? rp[r + 1] - rp[r] : 0;
+  for (int o = 32; o > 0; o >>= 1) w = max(w, __shfl_xor(w, o, 64));
+  if (lane == 0 && s * 64 < n) wid[s] = w;
+}
+
+__global__ void k_sell_fill(int n, const int *__restrict__ rp, const int *__restrict__ col,
+                            const double *__restrict__ val, const int *__restrict__ soff,
+                            const int *__restrict__ wid, int *__restrict__ scol,
+                            double *__restrict__ sval) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  const int s = r >> 6, lane = r & 63;
+  const long long base = (long long)soff[s] * 64 + lane;
+  const int b = rp[r], e = rp[r + 1], w = wid[s];
+  for (int k = 0; k < w; ++k) {
+    const bool real = b + k < e;
+    scol[base + (long long)k * 64] = real ? col[b + k] : r;
+    sval[base + (long long)k * 64] = real ? val[b + k] : 0.0;
+  }
+}
+
+template <class S, class D>
+__global__ void k_amg_convert(size_t n, const S *__restrict__ a, D *__restrict__ b) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+    b[i] = (D)a[i];
+}
+
+// [A | I] for Gauss-Jordan, A from CSR plus shift/n everywhere.
+__global__ void k_gj_fill(Csr<double> A, double shift, double *__restrict__ g) {
+  const int n = A.n;
+  int i = blockIdx.x;
+  for (int j = threadIdx.x; j < 2 * n; j += blockDim.x)
+    g[(size_t)i * 2 * n + j] = (j < n) ? shift / n : (j - n == i ? 1.0 : 0.0);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int e = A.rp[i]; e < A.rp[i + 1]; ++e) g[(size_t)i * 2 * n + A.col[e]] += A.val[e];
+}
+
+// pivot step k: pick the largest |g[i][k]|, i >= k, swap it into row k,
+// scale row k, save column k for the elimination
+__global__ void __launch_bounds__(256) k_gj_pivot(int n, int k, double *__restrict__ g,
+                                                  double *__restrict__ colk, int *__restrict__ bad) {
+  __shared__ double bv[256];
+  __shared__ int bi[256];
+  const int t = threadIdx.x, w = 2 * n;
+  double v = -1.0;
+  int vi = k;
+  for (int i = k + t; i < n; i += 256) {
+    double a = fabs(g[(size_t)i * w + k]);
+    if (a > v) { v = a; vi = i; }
+  }
+  bv[t] = v;
+  bi[t] = vi;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s && (bv[t + s] > bv[t] || (bv[t + s] == bv[t] && bi[t + s] < bi[t]))) {
+      bv[t] = bv[t + s];
+      bi[t] = bi[t + s];
+    }
+    __syncthreads();
+  }
+  const int p = bi[0];
+  if (bv[0] <= 0.0) {
+    if (t == 0) *bad = 1;
+    return;
+  }
+  if (p != k)
+    for (int j = t; j < w; j += 256) {
+      double a = g[(size_t)k * w + j];
+      g[(size_t)k * w + j] = g[(size_t)p * w + j];
+      g[(size_t)p * w + j] = a;
+    }
+  __syncthreads();
+  const double inv = 1.0 / g[(size_t)k * w + k];
+  for (int j = t; j < w; j += 256) g[(size_t)k * w + j] *= inv;
+  __syncthreads();
+  for (int i = t; i < n; i += 256) colk[i] = g[(size_t)i * w + k];
+}
+
+__global__ void __launch_bounds__(256) k_gj_elim(int n, int k, double *__restrict__ g,
+                                                 const double *__restrict__ colk) {
+  const int w = 2 * n;
+  const size_t tot = (size_t)n * w;
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < tot; e += (size_t)gridDim.x * 256) {
+    const int i = (int)(e / w), j = (int)(e % w);
+    if (i == k) continue;
+    g[e] -= colk[i] * g[(size_t)k * w + j];
+  }
+}
+
+__global__ void k_gj_extract(int n, const double *__restrict__ g, double *__restrict__ inv) {
+  int i = blockIdx.x;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) inv[(size_t)i * n + j] = g[(size_t)i * 2 * n + n + j];
+}
+
+// ---------------------------------------------------------------- cycle kernels
+template <class T>
+__device__ inline double ell_apply(const Op<T> &A, const T *__restrict__ x, int i) {
+  return row_apply(A, x, i);
+}
+template <class T>
+struct Sell {
+  const int *soff, *wid, *col;
+  const T *val;
+  int n;
+};
+
+template <class T>
+__device__ inline double csr_apply(const Sell<T> &A, const T *__restrict__ x, int i) {
+  const int s = i >> 6;
+  const long long base = (long long)A.soff[s] * 64 + (i & 63);
+  const int w = A.wid[s];
+  double acc = 0.0;
+  for (int k = 0; k < w; ++k) {
+    const long long e = base + (long long)k * 64;
+    acc += (double)A.val[e] * (double)x[A.col[e]];
+  }
+  return acc;
+}
+
+template <class T>
+__global__ void k_cy_first(int n, const T *__restrict__ dinv, const T *__restrict__ b, T *__restrict__ x,
+                           double omega, const GState *st) {
+  if (st && halted(st)) return;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) x[i] = (T)(omega * (double)dinv[i] * (double)b[i]);
+}
+
+template <class T, bool ELL>
+__global__ void __launch_bounds__(256) k_cy_sweep(Op<T> E, Sell<T> C, const T *__restrict__ dinv,
+                                                  const T *__restrict__ b, const T *__restrict__ xin,
+                                                  T *__restrict__ xout, double omega,
+                                                  const GState *st) {
+  if (st && halted(st)) return;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  const int n = ELL ? E.M : C.n;
+  if (i >= n) return;
+  const double ax = ELL ? ell_apply(E, xin, i) : csr_apply(C, xin, i);
+  xout[i] = (T)((double)xin[i] + omega * (double)dinv[i] * ((double)b[i] - ax));
+}
+
+template <class T, bool ELL>
+__global__ void __launch_bounds__(256) k_cy_resid(Op<T> E, Sell<T> C, const T *__restrict__ b,
+                                                  const T *__restrict__ x, T *__restrict__ r,
+                                                  const GState *st) {
+  if (st && halted(st)) return;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  const int n = ELL ? E.M : C.n;
+  if (i >= n) return;
+  const double ax = ELL ? ell_apply(E, x, i) : csr_apply(C, x, i);
+  r[i] = (T)((double)b[i] - ax);
+}
+
+template <class T>
+__global__ void k_cy_restrict(int nc, const int *__restrict__ mp, const int *__restrict__ ml,
+                              const T *__restrict__ r, T *__restrict__ bc, const GState *st) {
+  if (st && halted(st)) return;
+  int I = blockIdx.x * 256 + threadIdx.x;
+  if (I >= nc) return;
+  double s = 0.0;
+  for (int e = mp[I]; e < mp[I + 1]; ++e) s += (double)r[ml[e]];
+  bc[I] = (T)s;
+}
+
+template <class T>
+__global__ void k_cy_prolong(int n, const int *__restrict__ agg, const T *__restrict__ xc,
+                             T *__restrict__ x, double alpha, const GState *st) {
+  if (st && halted(st)) return;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) x[i] = (T)((double)x[i] + alpha * (double)xc[agg[i]]);
+}
+
+template <class T>
+__global__ void __launch_bounds__(64) k_cy_dense(int n, const T *__restrict__ inv, const T *__restrict__ b,
+                                                 T *__restrict__ x, const GState *st) {
+  if (st && halted(st)) return;
+  const int i = blockIdx.x, lane = threadIdx.x;
+  double s = 0.0;
+  for (int j = lane; j < n; j += 64) s += (double)inv[(size_t)i * n + j] * (double)b[j];
+  s = wave_sum(s);
+  if (lane == 0) x[i] = (T)s;
+}
+
+// ---------------------------------------------------------------- host side
+static int ilog2_ceil(uint64_t v) {
+  int b = 0;
+  while (b < 64 && (v >> b) != 0) ++b;
+  return std::max(b, 1);
+}
+
+template <class V>
+static int aggregate_level(apc_ctx *c, AmgHier *h, const V &v, int n, AmgLevel *Lf, int *nc_out) {
+  const AmgOptions &o = h->o;
+  const unsigned g = nblocks(n, 256);
+  APC_ALLOC(c, h->smax, n + 1);
+  APC_ALLOC(c, h->sa, n + 1);
+  APC_ALLOC(c, h->sb, n + 1);
+  APC_ALLOC(c, h->flag, n + 1);
+  APC_ALLOC(c, h->pos, n + 1);
+  APC_ALLOC(c, h->cnt, 4);
+  if (o.theta > 0) hipLaunchKernelGGL((k_amg_smax<V>), dim3(g), dim3(256), 0, c->s, v, h->smax.p);
+  else APC_HIP(c, hipMemsetAsync(h->smax.p, 0, sizeof(double) * n, c->s));
+  APC_HIP(c, hipMemsetAsync(h->sa.p, 0, sizeof(int) * n, c->s));
+  int *a = h->sa.p, *b = h->sb.p;
+  for (int round = 0;; ++round) {
+    if (round > 200) return fail(c, APC_ERR_UNSUPPORTED, "amg: independent set did not settle");
+    APC_HIP(c, hipMemsetAsync(h->cnt.p, 0, sizeof(int), c->s));
+    hipLaunchKernelGGL((k_amg_pick<V>), dim3(g), dim3(256), 0, c->s, v, o.theta, h->smax.p, a, b);
+    hipLaunchKernelGGL((k_amg_cover<V>), dim3(g), dim3(256), 0, c->s, v, o.theta, h->smax.p, b, h->cnt.p);
+    int und = 0;
+    APC_HIP(c, hipMemcpyAsync(&und, h->cnt.p, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    APC_HIP(c, hipStreamSynchronize(c->s));
+    std::swap(a, b);
+    if (und == 0) break;
+  }
+  hipLaunchKernelGGL(k_amg_rootflag, dim3(g), dim3(256), 0, c->s, n, a, h->flag.p);
+  int r = scan_exclusive(c, h->flag.p, h->pos.p, n, h->cnt.p + 1);
+  if (r) return r;
+  APC_ALLOC(c, Lf->agg, n + 1);
+  APC_HIP(c, hipMemsetAsync(h->cnt.p + 2, 0, sizeof(int), c->s));
+  hipLaunchKernelGGL((k_amg_agg<V>), dim3(g), dim3(256), 0, c->s, v, o.theta, h->smax.p, a, h->pos.p,
+                     Lf->agg.p, h->cnt.p + 2);
+  int hc[3] = {0, 0, 0};
+  APC_HIP(c, hipMemcpyAsync(hc, h->cnt.p, sizeof(hc), hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  if (hc[2]) return fail(c, APC_ERR_UNSUPPORTED, "amg: %d rows without a root neighbour", hc[2]);
+  *nc_out = hc[1];
+  return 0;
+}
+
+// rows of each aggregate, grouped in row order (restriction order)
+static int group_members(apc_ctx *c, AmgHier *h, AmgLevel *Lf, int n, int nc) {
+  APC_ALLOC(c, h->key, n + 1);
+  APC_ALLOC(c, h->key2, n + 1);
+  APC_ALLOC(c, h->idx, n + 1);
+  APC_ALLOC(c, h->idx2, n + 1);
+  APC_ALLOC(c, Lf->mp, nc + 1);
+  APC_ALLOC(c, Lf->ml, n + 1);
+  const unsigned g = nblocks(n, 256);
+  hipLaunchKernelGGL(k_amg_keys_rows, dim3(g), dim3(256), 0, c->s, n, Lf->agg.p, h->key.p, h->idx.p);
+  int r = radix_sort_pairs(c, h->key.p, h->idx.p, h->key2.p, h->idx2.p, n, ilog2_ceil((uint64_t)nc));
+  if (r) return r;
+  hipLaunchKernelGGL(k_amg_bounds, dim3(g), dim3(256), 0, c->s, n, nc, h->key.p, Lf->mp.p);
+  APC_HIP(c, hipMemcpyAsync(Lf->ml.p, h->idx.p, sizeof(int) * n, hipMemcpyDeviceToDevice, c->s));
+  return 0;
+}
+
+// Galerkin product into level Lc (CSR) from `ne` generated entries
+static int galerkin_reduce(apc_ctx *c, AmgHier *h, size_t ne, int nc, AmgLevel *Lc) {
+  if (ne >= (size_t)INT32_MAX) return fail(c, APC_ERR_UNSUPPORTED, "amg: too many entries");
+  const int n = (int)ne;
+  const uint64_t lim = (uint64_t)nc * (uint64_t)nc;
+  int r = radix_sort_pairs(c, h->key.p, h->idx.p, h->key2.p, h->idx2.p, n, ilog2_ceil(lim + 1));
+  if (r) return r;
+  APC_ALLOC(c, h->flag, ne + 1);
+  APC_ALLOC(c, h->pos, ne + 1);
+  APC_ALLOC(c, h->cnt, 4);
+  const unsigned g = nblocks(n, 256);
+  hipLaunchKernelGGL(k_amg_heads, dim3(g), dim3(256), 0, c->s, n, lim, h->key.p, h->flag.p);
+  r = scan_exclusive(c, h->flag.p, h->pos.p, n, h->cnt.p);
+  if (r) return r;
+  int nnz = 0;
+  APC_HIP(c, hipMemcpyAsync(&nnz, h->cnt.p, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  Lc->n = nc;
+  Lc->nnz = nnz;
+  APC_ALLOC(c, Lc->rp, nc + 1);
+  APC_ALLOC(c, Lc->col, nnz + 1);
+  APC_ALLOC(c, Lc->val, nnz + 1);
+  APC_ALLOC(c, Lc->dinv, nc + 1);
+  APC_ALLOC(c, h->diag, nc + 1);
+  APC_HIP(c, hipMemsetAsync(h->diag.p, 0, sizeof(double) * nc, c->s));
+  hipLaunchKernelGGL(k_amg_reduce, dim3(g), dim3(256), 0, c->s, n, nc, h->key.p, h->idx.p, h->ev.p,
+                     h->flag.p, h->pos.p, Lc->rp.p, Lc->col.p, Lc->val.p, h->diag.p);
+  APC_HIP(c, hipMemcpyAsync(Lc->rp.p + nc, &Lc->nnz, sizeof(int), hipMemcpyHostToDevice, c->s));
+  hipLaunchKernelGGL(k_amg_dinv, dim3(nblocks(nc, 256)), dim3(256), 0, c->s, nc, h->diag.p,
+                     Lc->dinv.p);
+  APC_HIP(c, hipStreamSynchronize(c->s));  // Lc->nnz copied from the host stack above
+  return 0;
+}
+
+static int ensure_work(apc_ctx *c, AmgLevel *L, int n, bool fp32) {
+  if (fp32) {
+    APC_ALLOC(c, L->x32, n + 1);
+    APC_ALLOC(c, L->b32, n + 1);
+    APC_ALLOC(c, L->r32, n + 1);
+    APC_ALLOC(c, L->t32, n + 1);
+  }
+  APC_ALLOC(c, L->x, n + 1);
+  APC_ALLOC(c, L->b, n + 1);
+  APC_ALLOC(c, L->r, n + 1);
+  APC_ALLOC(c, L->t, n + 1);
+  return 0;
+}
+
+static int build_sell(apc_ctx *c, AmgHier *h, AmgLevel *L) {
+  const int n = L->n, ns = (n + 63) / 64;
+  APC_ALLOC(c, L->swid, ns + 1);
+  APC_ALLOC(c, L->soff, ns + 1);
+  hipLaunchKernelGGL(k_sell_width, dim3(nblocks(ns, 4)), dim3(256), 0, c->s, n, L->rp.p, L->swid.p);
+  APC_ALLOC(c, h->cnt, 4);
+  int r = scan_exclusive(c, L->swid.p, L->soff.p, ns, h->cnt.p);
+  if (r) return r;
+  int tot = 0;
+  APC_HIP(c, hipMemcpyAsync(&tot, h->cnt.p, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  L->sell_len = (long long)tot * 64;
+  APC_ALLOC(c, L->scol, L->sell_len + 1);
+  APC_ALLOC(c, L->sval, L->sell_len + 1);
+  hipLaunchKernelGGL(k_sell_fill, dim3(nblocks(n, 256)), dim3(256), 0, c->s, n, L->rp.p, L->col.p,
+                     L->val.p, L->soff.p, L->swid.p, L->scol.p, L->sval.p);
+  return 0;
+}
+
+static int dense_inverse(apc_ctx *c, AmgHier *h, AmgLevel *L, bool periodic) {
+  const int n = L->n;
+  Csr<double> A{L->rp.p, L->col.p, L->val.p, n};
+  double shift = 0.0;
+  if (periodic) {  // mean |diagonal|, from the host copy of the level's dinv
+    std::vector<double> dv(n);
+    APC_HIP(c, hipMemcpyAsync(dv.data(), L->dinv.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+    APC_HIP(c, hipStreamSynchronize(c->s));
+    double s = 0.0;
+    int k = 0;
+    for (double d : dv)
+      if (d != 0.0) { s += 1.0 / fabs(d); ++k; }
+    shift = k ? s / k : 1.0;
+  }
+  APC_ALLOC(c, h->gj, (size_t)2 * n * n + 1);
+  APC_ALLOC(c, h->colk, n + 1);
+  APC_ALLOC(c, h->cinv, (size_t)n * n + 1);
+  APC_ALLOC(c, h->cnt, 4);
+  APC_HIP(c, hipMemsetAsync(h->cnt.p, 0, sizeof(int), c->s));
+  hipLaunchKernelGGL(k_gj_fill, dim3(n), dim3(256), 0, c->s, A, shift, h->gj.p);
+  const unsigned ge = std::max(1u, std::min(1024u, nblocks((long)2 * n * n, 256)));
+  for (int k = 0; k < n; ++k) {
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, c->s, n, k, h->gj.p, h->colk.p, h->cnt.p);
+    hipLaunchKernelGGL(k_gj_elim, dim3(ge), dim3(256), 0, c->s, n, k, h->gj.p, h->colk.p);
+  }
+  hipLaunchKernelGGL(k_gj_extract, dim3(n), dim3(256), 0, c->s, n, h->gj.p, h->cinv.p);
+  int bad = 0;
+  APC_HIP(c, hipMemcpyAsync(&bad, h->cnt.p, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  if (bad) return fail(c, APC_ERR_UNSUPPORTED, "amg: singular coarsest matrix (%d rows)", n);
+  if (h->fp32) {
+    APC_ALLOC(c, h->cinv32, (size_t)n * n + 1);
+    hipLaunchKernelGGL((k_amg_convert<double, float>), dim3(256), dim3(256), 0, c->s, (size_t)n * n,
+                       h->cinv.p, h->cinv32.p);
+  }
+  return 0;
+}
+
+int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool fp32,
+              const AmgOptions &o) {
+  if (!*hp) *hp = new AmgHier();
+  AmgHier *h = *hp;
+  h->o = o;
+  h->fp32 = fp32;
+  h->nlev = 0;
+  h->dense = false;
+  auto level = [&](int l) -> AmgLevel * {
+    while ((int)h->L.size() <= l) h->L.push_back(new AmgLevel());
+    return h->L[l];
+  };
+  KScope ks(c, "amg_setup");
+  AmgLevel *L0 = level(0);
+  L0->n = A.M;
+  L0->nnz = A.M * (A.lm + 1);
+  int r = ensure_work(c, L0, A.M, fp32);
+  if (r) return r;
+  int n = A.M;
+  int k = 0;  // deepest level built by this setup
+  for (int l = 0;; ++l) {
+    AmgLevel *Lf = level(l);
+    // stop at the dense size (but coarsen level 0 at least once) or the cap
+    if ((l > 0 && n <= o.coarse_max) || l + 1 >= o.max_levels) break;
+    int nc = 0;
+    if (l == 0) r = aggregate_level(c, h, EllView{A}, n, Lf, &nc);
+    else r = aggregate_level(c, h, CsrView{Csr<double>{Lf->rp.p, Lf->col.p, Lf->val.p, n}}, n, Lf, &nc);
+    if (r) return r;
+    if (nc < 1 || nc >= n) break;  // no coarsening: this level is the coarsest
+    r = group_members(c, h, Lf, n, nc);
+    if (r) return r;
+    // Galerkin entries
+    size_t ne = (l == 0) ? (size_t)n * (A.lm + 1) : (size_t)Lf->nnz;
+    APC_ALLOC(c, h->key, ne + 1);
+    APC_ALLOC(c, h->key2, ne + 1);
+    APC_ALLOC(c, h->idx, ne + 1);
+    APC_ALLOC(c, h->idx2, ne + 1);
+    APC_ALLOC(c, h->ev, ne + 1);
+    if (l == 0)
+      hipLaunchKernelGGL(k_amg_entries_ell, dim3(nblocks(n, 256)), dim3(256), 0, c->s, A, Lf->agg.p,
+                         nc, h->key.p, h->ev.p, h->idx.p);
+    else
+      hipLaunchKernelGGL(k_amg_entries_csr, dim3(nblocks(n, 256)), dim3(256), 0, c->s,
+                         Csr<double>{Lf->rp.p, Lf->col.p, Lf->val.p, n}, Lf->agg.p, nc, h->key.p,
+                         h->ev.p, h->idx.p);
+    AmgLevel *Lc = level(l + 1);
+    r = galerkin_reduce(c, h, ne, nc, Lc);
+    if (r) return r;
+    r = ensure_work(c, Lc, nc, fp32);
+    if (r) return r;
+    r = build_sell(c, h, Lc);
+    if (r) return r;
+    if (fp32) {
+      APC_ALLOC(c, Lc->sval32, Lc->sell_len + 1);
+      APC_ALLOC(c, Lc->dinv32, nc + 1);
+      hipLaunchKernelGGL((k_amg_convert<double, float>), dim3(1024), dim3(256), 0, c->s,
+                         (size_t)Lc->sell_len, Lc->sval.p, Lc->sval32.p);
+      hipLaunchKernelGGL((k_amg_convert<double, float>), dim3(256), dim3(256), 0, c->s, (size_t)nc,
+                         Lc->dinv.p, Lc->dinv32.p);
+    }
+    n = nc;
+    k = l + 1;
+  }
+  h->nlev = k + 1;
+  AmgLevel *Lc = h->L[k];
+  h->nc = Lc->n;
+  if (k > 0 && Lc->n <= 1024) {  // else (coarsening stalled) smoothing only
+    r = dense_inverse(c, h, Lc, periodic);
+    if (r) return r;
+    h->dense = true;
+  }
+  APC_HIP(c, hipGetLastError());
+  return 0;
+}
+
+template <class T>
+struct LevelPtrs {
+  T *x, *b, *r, *t;
+  const T *dinv, *val;
+};
+
+template <class T>
+static LevelPtrs<T> ptrs(AmgLevel *L);
+template <>
+LevelPtrs<double> ptrs<double>(AmgLevel *L) {
+  return {L->x.p, L->b.p, L->r.p, L->t.p, L->dinv.p, L->sval.p};
+}
+template <>
+LevelPtrs<float> ptrs<float>(AmgLevel *L) {
+  return {L->x32.p, L->b32.p, L->r32.p, L->t32.p, L->dinv32.p, L->sval32.p};
+}
+
+template <class T>
+static const T *cinv_of(AmgHier *h);
+template <>
+const double *cinv_of<double>(AmgHier *h) { return h->cinv.p; }
+template <>
+const float *cinv_of<float>(AmgHier *h) { return h->cinv32.p; }
+
+// x = cycle(b) on level l; x and b are device vectors of the level's size,
+// tmp a scratch vector of that size (distinct from x and b).
+template <class T>
+static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *x, T *tmp,
+                 const GState *st) {
+  AmgLevel *L = h->L[l];
+  const AmgOptions &o = h->o;
+  const int n = (l == 0) ? A0.M : L->n;
+  const unsigned g = nblocks(n, 256);
+  LevelPtrs<T> P = ptrs<T>(L);
+  const T *dinv = (l == 0) ? A0.dinv : P.dinv;
+  Op<T> E = A0;
+  Sell<T> C{L->soff.p, L->swid.p, L->scol.p, P.val, n};
+  const bool ell = (l == 0);
+  auto sweep = [&](const T *xin, T *xout) {
+    if (ell)
+      hipLaunchKernelGGL((k_cy_sweep<T, true>), dim3(g), dim3(256), 0, c->s, E, C, dinv, b, xin,
+                         xout, o.omega, st);
+    else
+      hipLaunchKernelGGL((k_cy_sweep<T, false>), dim3(g), dim3(256), 0, c->s, E, C, dinv, b, xin,
+                         xout, o.omega, st);
+  };
+  if (l == h->nlev - 1) {
+    if (h->dense) {
+      hipLaunchKernelGGL((k_cy_dense<T>), dim3(n), dim3(64), 0, c->s, n, cinv_of<T>(h), b, x, st);
+    } else {  // no dense solve: smoothing only (odd sweep count ends in x)
+      hipLaunchKernelGGL((k_cy_first<T>), dim3(g), dim3(256), 0, c->s, n, dinv, b, x, o.omega, st);
+      for (int s = 0; s < 4 * o.nu; ++s) {
+        sweep(x, tmp);
+        sweep(tmp, x);
+      }
+    }
+    return 0;
+  }
+  // smoothing writes alternate between tmp and x; with 2*nu - 1 sweeps after
+  // the first (odd), starting in tmp ends in x
+  T *cur = tmp, *alt = x;
+  hipLaunchKernelGGL((k_cy_first<T>), dim3(g), dim3(256), 0, c->s, n, dinv, b, cur, o.omega, st);
+  for (int s = 1; s < o.nu; ++s) {
+    sweep(cur, alt);
+    std::swap(cur, alt);
+  }
+  AmgLevel *Lc = h->L[l + 1];
+  LevelPtrs<T> Pc = ptrs<T>(Lc);
+  T *r = P.r;
+  for (int q = 0; q < o.gamma; ++q) {
+    if (ell)
+      hipLaunchKernelGGL((k_cy_resid<T, true>), dim3(g), dim3(256), 0, c->s, E, C, b, cur, r, st);
+    else
+      hipLaunchKernelGGL((k_cy_resid<T, false>), dim3(g), dim3(256), 0, c->s, E, C, b, cur, r, st);
+    hipLaunchKernelGGL((k_cy_restrict<T>), dim3(nblocks(Lc->n, 256)), dim3(256), 0, c->s, Lc->n,
+                       L->mp.p, L->ml.p, r, Pc.b, st);
+    int rr = cycle<T>(c, h, A0, l + 1, Pc.b, Pc.x, Pc.t, st);
+    if (rr) return rr;
+    hipLaunchKernelGGL((k_cy_prolong<T>), dim3(g), dim3(256), 0, c->s, n, L->agg.p, Pc.x, cur,
+                       o.alpha, st);
+  }
+  for (int s = 0; s < o.nu; ++s) {
+    sweep(cur, alt);
+    std::swap(cur, alt);
+  }
+  if (cur != x) APC_HIP(c, hipMemcpyAsync(x, cur, sizeof(T) * n, hipMemcpyDeviceToDevice, c->s));
+  return 0;
+}
+
+template <class T>
+int amg_apply(apc_ctx *c, AmgHier *h, const Op<T> &A, const T *in, T *out, const GState *st) {
+  if (!h || h->nlev < 1) return fail(c, APC_ERR_ARG, "amg: no hierarchy");
+  if (sizeof(T) == 4 && !h->fp32) return fail(c, APC_ERR_ARG, "amg: hierarchy has no FP32 copy");
+  KScope ks(c, sizeof(T) == 4 ? "amg_cycle32" : "amg_cycle");
+  T *tmp = ptrs<T>(h->L[0]).t;
+  int r = cycle<T>(c, h, A, 0, in, out, tmp, st);
+  if (r) return r;
+  APC_HIP(c, hipGetLastError());
+  return 0;
+}
+
+template int amg_apply<double>(apc_ctx *, AmgHier *, const Op<double> &, const double *, double *,
+                               const GState *);
+template int amg_apply<float>(apc_ctx *, AmgHier *, const Op<float> &, const float *, float *,
+                              const GState *);
+
+void amg_free(AmgHier *h) { delete h; }
+
+int amg_levels(const AmgHier *h, int *rows, long long *nnz, int cap) {
+  if (!h) return 0;
+  for (int l = 0; l < h->nlev && l < cap; ++l) {
+    if (rows) rows[l] = h->L[l]->n;
+    if (nnz) nnz[l] = h->L[l]->nnz;
+  }
+  return h->nlev;
+}
+
+}  // namespace apc

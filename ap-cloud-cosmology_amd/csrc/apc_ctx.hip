@@ -237,6 +237,12 @@ int apc_solver_solve(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_opts *o2,
   return differentiate(c);
 }
 
+int apc_solver_amg_levels(apc_ctx *c, int *rows, long long *nnz, int cap, int *nlev) {
+  if (!c || !nlev || cap < 0) return APC_ERR_ARG;
+  *nlev = amg_info(c, rows, nnz, cap);
+  return 0;
+}
+
 int apc_differentiate(apc_ctx *c, const double *phi) {
   if (!c || !phi) return APC_ERR_ARG;
   if (!c->have_comp) return fail(c, APC_ERR_ARG, "differentiate: no computational particles");

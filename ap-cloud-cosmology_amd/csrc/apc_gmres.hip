@@ -25,24 +25,9 @@
 // and the correction is accumulated into the FP64 solution.
 #include <cstdlib>
 
-#include "apc_internal.h"
-#include "apc_prims.h"
+#include "apc_solve.h"
 
 namespace apc {
-
-template <class T>
-struct Op {
-  const int *nbr;
-  const T *s, *d, *f, *dinv;
-  int M, lm;
-};
-
-struct GState {
-  int it, its, reason, cycle_done;
-  int m, max_it, fp32, monitor;
-  double res, rnorm0, ttol, cyc_res0;
-  double rtol, atol, dtol;
-};
 
 enum { KSP_CONVERGED_RTOL = 2, KSP_CONVERGED_ATOL = 3, KSP_DIVERGED_NULL = -2,
        KSP_DIVERGED_ITS = -3, KSP_DIVERGED_DTOL = -4, KSP_DIVERGED_BREAKDOWN = -5,
@@ -54,17 +39,19 @@ struct GDev {
 };
 
 struct SolveState {
-  DBuf<double> V64, dscratch, r64, part;
-  DBuf<float> V32, s32, d32, f32, dinv32;
+  DBuf<double> V64, dscratch, r64, part, z64, u64;
+  DBuf<float> V32, s32, d32, f32, dinv32, z32, u32, Z32;
   DBuf<double> hess;  // HH, cc, ss, grs, sig, h, coef, hist
   DBuf<GState> st;
   GState *hst = nullptr;  // pinned mirror
+  AmgHier *amg = nullptr;
   int mcap = 0, histcap = 0;
   ~SolveState() {
-    V64.release(); dscratch.release(); r64.release(); part.release();
-    V32.release(); s32.release(); d32.release(); f32.release(); dinv32.release();
-    hess.release(); st.release();
+    V64.release(); dscratch.release(); r64.release(); part.release(); z64.release();
+    u64.release(); V32.release(); s32.release(); d32.release(); f32.release();
+    dinv32.release(); z32.release(); u32.release(); Z32.release(); hess.release(); st.release();
     if (hst) hipHostFree(hst);
+    amg_free(amg);
   }
 };
 
@@ -79,8 +66,6 @@ __device__ inline int converged(GState *st, int nit, double rn) {
   return 0;
 }
 
-__device__ inline bool halted(const GState *st) { return st->reason != 0 || st->cycle_done != 0; }
-
 // Fixed-order sum of nb partials by the calling block; result to all threads.
 __device__ inline double sum_partials(const double *part, int nb, double *ws) {
   double a = 0.0;
@@ -89,26 +74,18 @@ __device__ inline double sum_partials(const double *part, int nb, double *ws) {
 }
 
 
-template <class T>
-__device__ inline double row_apply(const Op<T> &A, const T *__restrict__ x, int i) {
-  T acc = 0;
-  for (int j = 0; j < A.lm; ++j) {
-    int m = A.nbr[(size_t)j * A.M + i];
-    if (m < A.M) acc += A.s[(size_t)j * A.M + i] * x[m];
-  }
-  return (double)(A.d[i] * x[i] + A.f[i] * acc);
-}
-
-// ---- cycle start: v0 = D^-1 (b - A x) in FP64, ||v0||, PETSc's check ----
+// ---- cycle start: v0 = D^-1 (b - A x) in FP64 (left Jacobi) or b - A x
+// (right preconditioning), ||v0||, PETSc's check ----
 template <class T>
 __global__ void __launch_bounds__(256) k_resid(Op<double> A, const double *__restrict__ x,
                                                const double *__restrict__ b, int zero_guess,
-                                               T *__restrict__ v0, GDev g) {
+                                               int left_pc, T *__restrict__ v0, GDev g) {
   __shared__ double ws[4];
   if (g.st->reason) return;
   double acc = 0.0;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < A.M; i += gridDim.x * 256) {
-    double r = zero_guess ? A.dinv[i] * b[i] : A.dinv[i] * (b[i] - row_apply(A, x, i));
+    double r = zero_guess ? b[i] : b[i] - row_apply(A, x, i);
+    if (left_pc) r = A.dinv[i] * r;
     v0[i] = (T)r;
     acc += r * r;
   }
@@ -137,15 +114,38 @@ __global__ void __launch_bounds__(256) k_resid_fin(int nb, GDev g) {
   }
 }
 
-// ---- w = sig_j D^-1 A w_j ----
+// ---- w = sig_j D^-1 A w_j (left Jacobi) or w = sig_j A z_j ----
 template <class T>
 __global__ void __launch_bounds__(256) k_spmv(Op<T> A, const T *__restrict__ vj,
-                                              T *__restrict__ w, int j, GDev g) {
+                                              T *__restrict__ w, int j, int left_pc, GDev g) {
   if (halted(g.st)) return;
   const T s = (T)g.sig[j];
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= A.M) return;
-  w[i] = (T)(s * (T)(A.dinv[i] * (T)row_apply(A, vj, i)));
+  const T av = (T)row_apply(A, vj, i);
+  w[i] = left_pc ? (T)(s * (T)(A.dinv[i] * av)) : (T)(s * av);
+}
+
+// ---- flexible form: w = sig_j A z_j, z_j = M^-1 w_j computed in FP32 ----
+__global__ void __launch_bounds__(256) k_spmv_z(Op<double> A, const float *__restrict__ z,
+                                                double *__restrict__ w, int j, GDev g) {
+  if (halted(g.st)) return;
+  const double s = g.sig[j];
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= A.M) return;
+  double acc = 0.0;
+  for (int q = 0; q < A.lm; ++q) {
+    int m = A.nbr[(size_t)q * A.M + i];
+    if (m < A.M) acc += A.s[(size_t)q * A.M + i] * (double)z[m];
+  }
+  w[i] = s * (A.d[i] * (double)z[i] + A.f[i] * acc);
+}
+
+__global__ void k_to_float_gated(int n, const double *__restrict__ a, float *__restrict__ b,
+                                 const GState *st) {
+  if (halted(st)) return;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
 }
 
 // ---- h_q = sig_q (w_q . w), q = 0..nv-1 ----
@@ -378,6 +378,26 @@ __global__ void __launch_bounds__(256) k_axpy_sol(int M, int nv, const T *__rest
   }
 }
 
+// ---- u = sum_q coef_q w_q (right preconditioning: x += M^-1 u afterwards) ----
+template <class T>
+__global__ void __launch_bounds__(256) k_comb(int M, int nv, const T *__restrict__ V, size_t ld,
+                                              const double *__restrict__ coef, T *__restrict__ u) {
+  extern __shared__ double cs[];
+  for (int q = threadIdx.x; q < nv; q += 256) cs[q] = coef[q];
+  __syncthreads();
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < M; i += (long)gridDim.x * 256) {
+    double s = 0.0;
+    for (int q = 0; q < nv; ++q) s += cs[q] * (double)V[(size_t)q * ld + i];
+    u[i] = (T)s;
+  }
+}
+
+template <class T>
+__global__ void k_add_to(int M, const T *__restrict__ z, double *__restrict__ x) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < M) x[i] += (double)z[i];
+}
+
 __global__ void k_to_float(size_t n, const double *__restrict__ a, float *__restrict__ b) {
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
     b[i] = (float)a[i];
@@ -480,10 +500,18 @@ static int read_state(apc_ctx *c, SolveState *S) {
   return 0;
 }
 
+// One GMRES solve.  amg: right preconditioning with the multigrid in
+// S->amg; zb, ub are M-vectors of T for the preconditioned direction and the
+// cycle's combination.  flex (T = double): the multigrid runs in FP32 on the
+// FP32 operator A32 and its outputs are kept, one per Arnoldi step, in Z32
+// ((m+1) x M floats) — flexible GMRES, so the inexact FP32 preconditioner
+// costs no accuracy and the cycle's update is x += Z32 y; v32 is an M-vector.
 template <class T>
 static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, const double *b,
-                      double *x, const apc_ksp_opts &o, bool fp32, int *its_out, int *reason_out,
-                      double *rnorm_out, int *sweeps) {
+                      double *x, const apc_ksp_opts &o, bool fp32, bool amg, T *zb, T *ub,
+                      int *its_out, int *reason_out, double *rnorm_out, int *sweeps,
+                      const Op<float> *A32 = nullptr, float *Z32 = nullptr, float *v32 = nullptr) {
+  const bool flex = Z32 != nullptr;
   const int M = A64.M;
   const int m = std::min(std::max(1, o.restart), MAX_RESTART);
   SolveState *S = c->gm;
@@ -526,7 +554,7 @@ static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, 
     {
       KScope ks(c, fp32 ? "gmres_resid64" : "gmres_resid", 12.0 * nnz + 48.0 * M + sizeof(T) * M);
       hipLaunchKernelGGL(k_resid<T>, dim3(gres), dim3(256), 0, c->s, A64, x, b, zero_guess ? 1 : 0,
-                         V, g);
+                         amg ? 0 : 1, V, g);
     }
     hipLaunchKernelGGL(k_resid_fin, dim3(1), dim3(256), 0, c->s, (int)gres, g);
     ++ncyc;
@@ -539,9 +567,23 @@ static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, 
     for (int j = 0; j < m && j < budget; ++j) {
       const T *vj = V + (size_t)j * ld;
       T *w = V + (size_t)(j + 1) * ld;
-      {
+      if (flex) {
+        float *zj = Z32 + (size_t)j * ld;
+        hipLaunchKernelGGL(k_to_float_gated, dim3(gb), dim3(256), 0, c->s, M, (const double *)vj,
+                           v32, (const GState *)S->st.p);
+        r = amg_apply<float>(c, S->amg, *A32, v32, zj, S->st.p);
+        if (r) return r;
+        KScope ks(c, "gmres_spmv_z", 12.0 * nnz + 28.0 * M);
+        hipLaunchKernelGGL(k_spmv_z, dim3(gb), dim3(256), 0, c->s, A64, (const float *)zj,
+                           (double *)w, j, g);
+      } else if (amg) {
+        r = amg_apply<T>(c, S->amg, AT, vj, zb, S->st.p);
+        if (r) return r;
         KScope ks(c, fp32 ? "gmres_spmv32" : "gmres_spmv", bspmv);
-        hipLaunchKernelGGL(k_spmv<T>, dim3(gb), dim3(256), 0, c->s, AT, vj, w, j, g);
+        hipLaunchKernelGGL(k_spmv<T>, dim3(gb), dim3(256), 0, c->s, AT, (const T *)zb, w, j, 0, g);
+      } else {
+        KScope ks(c, fp32 ? "gmres_spmv32" : "gmres_spmv", bspmv);
+        hipLaunchKernelGGL(k_spmv<T>, dim3(gb), dim3(256), 0, c->s, AT, vj, w, j, 1, g);
       }
       const int nv = j + 1;
       {
@@ -577,7 +619,20 @@ static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, 
     monitor_flush(S->hst->its);
     if (it > 0) {
       hipLaunchKernelGGL(k_buildsoln, dim3(1), dim3(64), 0, c->s, g, it - 1);
-      {
+      if (flex) {  // x += sum_q y_q z_q
+        KScope ks(c, "gmres_update_z", 4.0 * M * it + 16.0 * M);
+        hipLaunchKernelGGL(k_axpy_sol<float>, dim3(gres), dim3(256), it * sizeof(double), c->s, M,
+                           it, (const float *)Z32, ld, g.coef, x);
+      } else if (amg) {  // x += M^-1 (sum_q y_q v_q); runs whatever the solve's state
+        {
+          KScope ks(c, fp32 ? "gmres_update32" : "gmres_update", (double)sizeof(T) * M * (it + 1));
+          hipLaunchKernelGGL(k_comb<T>, dim3(gres), dim3(256), it * sizeof(double), c->s, M, it, V,
+                             ld, g.coef, ub);
+        }
+        r = amg_apply<T>(c, S->amg, AT, ub, zb, nullptr);
+        if (r) return r;
+        hipLaunchKernelGGL(k_add_to<T>, dim3(gb), dim3(256), 0, c->s, M, (const T *)zb, x);
+      } else {
         KScope ks(c, fp32 ? "gmres_update32" : "gmres_update", (double)sizeof(T) * M * it + 16.0 * M);
         hipLaunchKernelGGL(k_axpy_sol<T>, dim3(gres), dim3(256), it * sizeof(double), c->s, M, it,
                            V, ld, g.coef, x);
@@ -600,16 +655,46 @@ static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, 
   return 0;
 }
 
+static AmgOptions amg_options() {
+  // APC_AMG_* override the multigrid settings (tuning runs)
+  AmgOptions a;
+  a.nu = env_int("APC_AMG_NU", a.nu);
+  a.gamma = env_int("APC_AMG_GAMMA", a.gamma);
+  a.coarse_max = env_int("APC_AMG_COARSE", a.coarse_max);
+  if (const char *v = getenv("APC_AMG_OMEGA")) a.omega = atof(v);
+  if (const char *v = getenv("APC_AMG_ALPHA")) a.alpha = atof(v);
+  if (const char *v = getenv("APC_AMG_THETA")) a.theta = atof(v);
+  return a;
+}
+
 // Solve A x = b (x overwritten) with the matrix (nbr, s, d, f, dinv).
+// pinned: A is the periodic Poisson operator with its identity row.
 static int gmres(apc_ctx *c, const Op<double> &A, const double *b, double *x,
-                 const apc_ksp_opts &o, int *its, int *reason, double *rnorm, int *sweeps) {
+                 const apc_ksp_opts &o, bool pinned, int *its, int *reason, double *rnorm,
+                 int *sweeps) {
   const int M = A.M;
   const int m = std::min(std::max(1, o.restart), MAX_RESTART);
   const bool fp32 = o.precision == 1;
+  const bool amg = o.pc_type == 1;
   int r = ensure_state(c, m, std::max(1, o.max_it), M, fp32);
   if (r) return r;
   SolveState *S = c->gm;
-  if (!fp32) return run_cycles<double>(c, A, A, S->V64.p, b, x, o, false, its, reason, rnorm, sweeps);
+  if (amg) {
+    APC_ALLOC(c, S->z64, M + 1);
+    APC_ALLOC(c, S->u64, M + 1);
+    if (fp32) {
+      APC_ALLOC(c, S->z32, M + 1);
+      APC_ALLOC(c, S->u32, M + 1);
+    }
+  }
+  if (!fp32) {
+    if (amg) {
+      r = amg_setup(c, &S->amg, A, pinned, false, amg_options());
+      if (r) return r;
+    }
+    return run_cycles<double>(c, A, A, S->V64.p, b, x, o, false, amg, S->z64.p, S->u64.p, its,
+                              reason, rnorm, sweeps);
+  }
   // FP32 copies of the operator for the inner cycles
   const size_t nnz = (size_t)M * A.lm;
   APC_ALLOC(c, S->s32, nnz + 1);
@@ -621,15 +706,28 @@ static int gmres(apc_ctx *c, const Op<double> &A, const double *b, double *x,
   hipLaunchKernelGGL(k_to_float, dim3(256), dim3(256), 0, c->s, (size_t)M, A.f, S->f32.p);
   hipLaunchKernelGGL(k_to_float, dim3(256), dim3(256), 0, c->s, (size_t)M, A.dinv, S->dinv32.p);
   Op<float> A32{A.nbr, S->s32.p, S->d32.p, S->f32.p, S->dinv32.p, M, A.lm};
-  return run_cycles<float>(c, A, A32, S->V32.p, b, x, o, true, its, reason, rnorm, sweeps);
+  if (amg) {
+    // FP32 multigrid inside FP64 flexible GMRES (the FP64 Krylov cycle is the
+    // refinement loop)
+    r = amg_setup(c, &S->amg, A, pinned, true, amg_options());
+    if (r) return r;
+    APC_ALLOC(c, S->V64, (size_t)(m + 1) * M + 1);
+    APC_ALLOC(c, S->Z32, (size_t)(m + 1) * M + 1);
+    return run_cycles<double>(c, A, A, S->V64.p, b, x, o, false, true, S->z64.p, S->u64.p, its,
+                              reason, rnorm, sweeps, &A32, S->Z32.p, S->z32.p);
+  }
+  return run_cycles<float>(c, A, A32, S->V32.p, b, x, o, true, false, S->z32.p, S->u32.p, its,
+                           reason, rnorm, sweeps);
 }
 
 int solve_density_potential(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_opts *o2,
                             apc_solve_info *info) {
   const int M = c->nc + c->nv;
   if (M < 1) return fail(c, APC_ERR_ARG, "solve: no computational particles");
-  if (o1->precision < 0|| o1->precision > 1 || o2->precision < 0 || o2->precision > 1)
+  if (o1->precision < 0 || o1->precision > 1 || o2->precision < 0 || o2->precision > 1)
     return fail(c, APC_ERR_ARG, "GMRES precision must be 0 (FP64) or 1 (FP32 + FP64 refinement)");
+  if (o1->pc_type < 0 || o1->pc_type > 1 || o2->pc_type < 0 || o2->pc_type > 1)
+    return fail(c, APC_ERR_ARG, "pc_type must be 0 (Jacobi) or 1 (multigrid)");
   APC_ALLOC(c, c->rho, M + 1);
   APC_ALLOC(c, c->phi, M + 1);
   APC_ALLOC(c, c->rhs2, M + 1);
@@ -637,8 +735,8 @@ int solve_density_potential(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_op
   Op<double> A2{c->nbr.p, c->sco.p, c->d2.p, c->f2.p, c->dinv2.p, M, c->prm.lm};
   apc_solve_info inf;
   memset(&inf, 0, sizeof(inf));
-  int r = gmres(c, A1, c->rhs1.p, c->rho.p, *o1, &inf.its_rho, &inf.reason_rho, &inf.rnorm_rho,
-                &inf.refine_rho);
+  int r = gmres(c, A1, c->rhs1.p, c->rho.p, *o1, false, &inf.its_rho, &inf.reason_rho,
+                &inf.rnorm_rho, &inf.refine_rho);
   if (r) return r;
   // right-hand side of the Poisson solve (solver.cpp:233-261)
   if (c->prm.bc == 1) {
@@ -652,14 +750,19 @@ int solve_density_potential(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_op
     hipLaunchKernelGGL(k_rhs2_bounded, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, c->rho.p,
                        c->pbnd.p, c->c_pot, c->rhs2.p);
   }
-  r = gmres(c, A2, c->rhs2.p, c->phi.p, *o2, &inf.its_phi, &inf.reason_phi, &inf.rnorm_phi,
-            &inf.refine_phi);
+  r = gmres(c, A2, c->rhs2.p, c->phi.p, *o2, c->prm.bc == 1, &inf.its_phi, &inf.reason_phi,
+            &inf.rnorm_phi, &inf.refine_phi);
   if (r) return r;
   inf.nc = c->nc;
   inf.nv = c->nv;
   inf.nb = c->nb;
   if (info) *info = inf;
   return 0;
+}
+
+int amg_info(apc_ctx *c, int *rows, long long *nnz, int cap) {
+  if (!c->gm || !c->gm->amg) return 0;
+  return amg_levels(c->gm->amg, rows, nnz, cap);
 }
 
 void free_solve_state(apc_ctx *c) {

@@ -152,6 +152,9 @@ struct KScope {
 int scan_exclusive(apc_ctx *c, const int *in, int *out, int n, int *total_dev);
 // Max of n non-negative doubles (device pointer in), result to host.
 int reduce_max_nonneg(apc_ctx *c, const double *in, int n, double *out_host);
+// Stable LSD radix sort of (k, v) on the low `bits` bits of k; k2/v2 are
+// same-sized scratch (apc_sort.hip).
+int radix_sort_pairs(apc_ctx *c, uint64_t *k, int *v, uint64_t *k2, int *v2, int n, int bits);
 
 // Stage entry points implemented across files.
 int sort_particles(apc_ctx *c);                  // apc_sort.hip
@@ -162,6 +165,7 @@ int differentiate(apc_ctx *c);                   // apc_stencil.hip
 int solve_density_potential(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_opts *o2,
                             apc_solve_info *info);  // apc_gmres.hip
 void free_solve_state(apc_ctx *c);              // apc_gmres.hip
+int amg_info(apc_ctx *c, int *rows, long long *nnz, int cap);  // apc_gmres.hip
 int interp_kick(apc_ctx *c, double Cv, double dt, double *maxdv);  // apc_mover.hip
 int drift(apc_ctx *c, double Cx, double dt, double *maxdx);        // apc_mover.hip
 

@@ -21,8 +21,6 @@
 
 namespace apc {
 
-int radix_sort_pairs(apc_ctx *c, uint64_t *k, int *v, uint64_t *k2, int *v2, int n, int bits);
-
 struct SelGeom {
   int L, bc, order, mindepth, maxdepth, leaf_cap, faces;
   double c;

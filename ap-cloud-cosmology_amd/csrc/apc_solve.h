@@ -1,0 +1,73 @@
+// Types shared by the Krylov solver (apc_gmres.hip) and the multigrid
+// preconditioner (apc_amg.hip).
+#pragma once
+#include "apc_internal.h"
+#include "apc_prims.h"
+
+namespace apc {
+
+// One GFD system in slot-major ELL form: row i is
+//   (A x)_i = d_i x_i + f_i * sum_j s_ij x_{nbr_ij}   over nbr_ij < M
+// (neighbours at or beyond M are boundary nodes, already in the RHS).
+template <class T>
+struct Op {
+  const int *nbr;
+  const T *s, *d, *f, *dinv;
+  int M, lm;
+};
+
+// Device-resident state of one GMRES solve; kernels launched ahead of the
+// host's checks return at once when the solve has stopped or the cycle ended.
+struct GState {
+  int it, its, reason, cycle_done;
+  int m, max_it, fp32, monitor;
+  double res, rnorm0, ttol, cyc_res0;
+  double rtol, atol, dtol;
+};
+
+__device__ inline bool halted(const GState *st) { return st->reason != 0 || st->cycle_done != 0; }
+
+template <class T>
+__device__ inline T row_offdiag_sum(const Op<T> &A, const T *__restrict__ x, int i) {
+  T acc = 0;
+  for (int j = 0; j < A.lm; ++j) {
+    int m = A.nbr[(size_t)j * A.M + i];
+    if (m < A.M) acc += A.s[(size_t)j * A.M + i] * x[m];
+  }
+  return acc;
+}
+
+template <class T>
+__device__ inline double row_apply(const Op<T> &A, const T *__restrict__ x, int i) {
+  return (double)(A.d[i] * x[i] + A.f[i] * row_offdiag_sum(A, x, i));
+}
+
+struct AmgHier;  // apc_amg.hip
+
+// Defaults from sweeps on the 256^3 Zel'dovich + Plummer system (7.5M rows,
+// 6 levels): a W-cycle with one sweep each side and a 1.5x coarse correction
+// converged in the fewest seconds (64 GMRES steps to rtol 1e-8).
+struct AmgOptions {
+  int nu = 1;             // Jacobi sweeps before and after the coarse correction
+  double omega = 0.7;     // Jacobi damping
+  double alpha = 1.5;     // scaling of the coarse correction
+  int gamma = 2;          // 1: V-cycle, 2: W-cycle
+  double theta = 0.0;     // strength threshold (fraction of the row's largest |a_ij|)
+  int coarse_max = 256;   // dense solve at or below this many rows
+  int max_levels = 12;
+};
+
+// Build the hierarchy for A (FP64); `periodic` shifts the near-null constant
+// mode of the pinned periodic operator in the coarsest solve.  FP32 copies
+// of every level are made when `fp32` is set.
+int amg_setup(apc_ctx *c, AmgHier **h, const Op<double> &A, bool periodic, bool fp32,
+              const AmgOptions &o);
+// out = M^-1 in (one cycle), level 0 given by A (same matrix as the setup, in
+// the precision T).  Every kernel returns at once if st (device) says the
+// solve has halted.
+template <class T>
+int amg_apply(apc_ctx *c, AmgHier *h, const Op<T> &A, const T *in, T *out, const GState *st);
+void amg_free(AmgHier *h);
+int amg_levels(const AmgHier *h, int *rows, long long *nnz, int cap);
+
+}  // namespace apc

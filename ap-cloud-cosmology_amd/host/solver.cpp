@@ -15,6 +15,7 @@ void apc_parse_ksp_options(int argc, char **argv, apc_ksp_opts *o) {
   o->restart = 30;
   o->monitor = 0;
   o->precision = 0;
+  o->pc_type = 0;  // PCJACOBI (solver.cpp:201,273)
   for (int i = 1; i < argc; ++i) {
     const char *a = argv[i];
     const char *v = (i + 1 < argc) ? argv[i + 1] : nullptr;
@@ -25,6 +26,14 @@ void apc_parse_ksp_options(int argc, char **argv, apc_ksp_opts *o) {
     else if (!strcmp(a, "-ksp_gmres_restart") && v) o->restart = atoi(v), ++i;
     else if (!strcmp(a, "-ksp_gmres_precision") && v) o->precision = atoi(v), ++i;
     else if (!strcmp(a, "-ksp_monitor")) o->monitor = 1;
+    else if (!strcmp(a, "-pc_type") && v) {
+      // jacobi: the reference's PCJACOBI; gamg/amg: the device multigrid
+      // (right preconditioning, true-residual convergence test)
+      if (!strcmp(v, "jacobi")) o->pc_type = 0;
+      else if (!strcmp(v, "gamg") || !strcmp(v, "amg")) o->pc_type = 1;
+      else fprintf(stderr, "apcloud: -pc_type %s not supported (jacobi, gamg)\n", v);
+      ++i;
+    }
   }
 }
 

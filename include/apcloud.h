@@ -59,6 +59,11 @@ typedef struct {
   int max_it, restart;
   int monitor;             /* print "%3d KSP Residual norm %14.12e" like -ksp_monitor */
   int precision;           /* 0: FP64 GMRES; 1: FP32 inner GMRES in FP64 iterative refinement */
+  int pc_type;             /* 0: Jacobi, left (the reference's PCJACOBI, solver.cpp:201,273);
+                              1: aggregation multigrid built on the device, right
+                              preconditioning (what -pc_type gamg -ksp_pc_side right selects
+                              through the reference's KSPSetFromOptions, solver.cpp:202,274);
+                              convergence is then tested on the true residual */
 } apc_ksp_opts;
 
 typedef struct {
@@ -136,6 +141,11 @@ int apc_build_system(apc_ctx *ctx, int *cols, double *a1, double *a2, double *d1
  * Solve_Potential, Differentiate. */
 int apc_solver_solve(apc_ctx *ctx, const apc_ksp_opts *rho, const apc_ksp_opts *phi,
                      apc_solve_info *info);
+/* Multigrid hierarchy of the last solve run with pc_type 1: *nlev levels, the
+ * first min(cap, *nlev) row counts and stored nonzeros (level 0 = the GFD
+ * matrix).  *nlev = 0 if no multigrid was built.  No reference counterpart
+ * (PETSc's -ksp_view prints the same for gamg). */
+int apc_solver_amg_levels(apc_ctx *ctx, int *rows, long long *nnz, int cap, int *nlev);
 /* Solver::Differentiate alone on a caller-supplied potential (M values). */
 int apc_differentiate(apc_ctx *ctx, const double *phi);
 /* Node fields after a solve: rho, phi (M), g (M x 3), H (M x 6: xx yy zz xy xz yz). */
