@@ -750,13 +750,20 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
   Op<T> E = A0;
   Sell<T> C{L->soff.p, L->swid.p, L->scol.p, P.val, n};
   const bool ell = (l == 0);
+  // level-0 algorithmic bytes per row: each of the lm slots reads its index,
+  // weight and the gathered x; plus d, f, dinv, b, own x read and x written
+  const double sT = sizeof(T);
+  const double lm0 = A0.lm;
+  const bool f32 = sizeof(T) == 4;
   auto sweep = [&](const T *xin, T *xout) {
-    if (ell)
+    if (ell) {
+      KScope ks(c, f32 ? "amg_sweep0_32" : "amg_sweep0", (double)n * (lm0 * (4 + 2 * sT) + 6 * sT));
       hipLaunchKernelGGL((k_cy_sweep<T, true>), dim3(g), dim3(256), 0, c->s, E, C, dinv, b, xin,
                          xout, o.omega, st);
-    else
+    } else {
       hipLaunchKernelGGL((k_cy_sweep<T, false>), dim3(g), dim3(256), 0, c->s, E, C, dinv, b, xin,
                          xout, o.omega, st);
+    }
   };
   if (l == h->nlev - 1) {
     if (h->dense) {
@@ -782,16 +789,26 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
   LevelPtrs<T> Pc = ptrs<T>(Lc);
   T *r = P.r;
   for (int q = 0; q < o.gamma; ++q) {
-    if (ell)
+    if (ell) {
+      KScope ks(c, f32 ? "amg_resid0_32" : "amg_resid0", (double)n * (lm0 * (4 + 2 * sT) + 5 * sT));
       hipLaunchKernelGGL((k_cy_resid<T, true>), dim3(g), dim3(256), 0, c->s, E, C, b, cur, r, st);
-    else
+    } else {
       hipLaunchKernelGGL((k_cy_resid<T, false>), dim3(g), dim3(256), 0, c->s, E, C, b, cur, r, st);
-    hipLaunchKernelGGL((k_cy_restrict<T>), dim3(nblocks(Lc->n, 256)), dim3(256), 0, c->s, Lc->n,
-                       L->mp.p, L->ml.p, r, Pc.b, st);
+    }
+    {
+      KScope ks(c, ell ? (f32 ? "amg_restrict0_32" : "amg_restrict0") : "amg_restrict",
+                ell ? (double)n * (sT + 4) + (double)Lc->n * (4 + sT) : 0.0);
+      hipLaunchKernelGGL((k_cy_restrict<T>), dim3(nblocks(Lc->n, 256)), dim3(256), 0, c->s, Lc->n,
+                         L->mp.p, L->ml.p, r, Pc.b, st);
+    }
     int rr = cycle<T>(c, h, A0, l + 1, Pc.b, Pc.x, Pc.t, st);
     if (rr) return rr;
-    hipLaunchKernelGGL((k_cy_prolong<T>), dim3(g), dim3(256), 0, c->s, n, L->agg.p, Pc.x, cur,
-                       o.alpha, st);
+    {
+      KScope ks(c, ell ? (f32 ? "amg_prolong0_32" : "amg_prolong0") : "amg_prolong",
+                ell ? (double)n * (4 + 3 * sT) : 0.0);
+      hipLaunchKernelGGL((k_cy_prolong<T>), dim3(g), dim3(256), 0, c->s, n, L->agg.p, Pc.x, cur,
+                         o.alpha, st);
+    }
   }
   for (int s = 0; s < o.nu; ++s) {
     sweep(cur, alt);

@@ -16,14 +16,21 @@ Workloads (synthetic, regenerated from their seeds):
   zeldovich128  128^3 lattice, RMS 0.005, seed 1; 20 steps a=0.02..0.1.
   zeldovich64   64^3 lattice, RMS 0.01, seed 4; 10 steps.
 All: leaf capacity 8, 27 GFD neighbours, GMRES restart 200, max 1000
-iterations (the reference's own run options, Makefile:30), rtol 1e-8.
+iterations (the reference's own run options, Makefile:30), rtol 1e-8.  The
+density solve uses the reference's Jacobi preconditioner (it converges in
+~7 steps); the Poisson solve uses --pc (default amg: the device multigrid,
+right preconditioning; the reference's users select a multigrid with
+-pc_type through KSPSetFromOptions, solver.cpp:202,274).
 
 Legs, all from the same state (the initial conditions after the initial half
 drift): (1) accuracy: one FP64 step and one step of the measured precision,
 relative RMS difference of the accelerations and largest position
-difference; (2) the FP64 step timed over K steps (skipped with --no-fp64);
-(3) the measured precision timed over K steps -> "value".  The roofline
-entry is for the kernel with the largest device time in leg (3).
+difference; (2) the reference configuration (FP64, Jacobi, the same GMRES
+settings) for one step: its time, iterations and its force difference from
+the converged FP64 step (skipped with --no-ref-config); (3) the FP64 step
+with the same solver settings timed over K steps (skipped with --no-fp64);
+(4) the measured precision timed over K steps -> "value".  The roofline entry
+is for the kernel with the largest device time in leg (4).
 """
 import argparse
 import json
@@ -87,6 +94,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the timed FP64 leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the force-error leg")
+    ap.add_argument("--no-ref-config", action="store_true",
+                    help="skip the reference-configuration (Jacobi) leg")
+    ap.add_argument("--pc", default="amg", choices=["amg", "jacobi"],
+                    help="Poisson preconditioner (the density solve always uses Jacobi)")
     ap.add_argument("--cpu-side", type=int, default=96, help="lattice side of the CPU sample")
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--restart", type=int, default=200)
@@ -120,37 +131,62 @@ def main():
     state0 = ctx.particles()[:6]
     cd = 1.0 / N
 
-    def opts(precision):
-        return apcloud.ksp(rtol=args.rtol, restart=args.restart, max_it=args.max_it,
-                           precision=precision)
+    def opts(precision, pc):
+        rho = apcloud.ksp(rtol=args.rtol, restart=args.restart, max_it=args.max_it,
+                          precision=precision, pc="jacobi")
+        phi = apcloud.ksp(rtol=args.rtol, restart=args.restart, max_it=args.max_it,
+                          precision=precision, pc=pc)
+        return rho, phi
 
     def one_step(t, o):
         dt = min(da, a_end - t) if t + da > a_end else da
         cpot, cf, _, cv = eds(t)
         _, _, cxh, _ = eds(t + dt / 2)
-        _, _, info, timing = ctx.step(cd, cpot, cf, cv, cxh, dt, o)
+        _, _, info, timing = ctx.step(cd, cpot, cf, cv, cxh, dt, o[0], o[1])
         return t + dt, info, timing
 
-    # (1) accuracy: one step of each form from the same state
-    acc = None
-    if not args.no_accuracy:
-        ctx.set_particles(*state0)
-        one_step(t_state0, opts(0))
-        ref = ctx.particles()
-        ctx.set_particles(*state0)
-        one_step(t_state0, opts(args.precision))
-        got = ctx.particles()
-        da_ = np.stack([got[6] - ref[6], got[7] - ref[7], got[8] - ref[8]])
-        a64 = np.stack(ref[6:9])
+    def force_err(got, ref):
+        d = np.stack([got[6] - ref[6], got[7] - ref[7], got[8] - ref[8]])
+        a = np.stack(ref[6:9])
         dx = np.stack([got[k] - ref[k] for k in range(3)])
         dx -= np.round(dx)  # periodic unit box
-        acc = {"rel_rms_force_err": float(np.sqrt((da_ ** 2).sum() / (a64 ** 2).sum())),
-               "max_abs_pos_err": float(np.abs(dx).max()),
-               "vs": "FP64 GPU step, same state"}
+        return float(np.sqrt((d ** 2).sum() / (a ** 2).sum())), float(np.abs(dx).max())
+
+    # (1) accuracy: one step of each form from the same state
+    acc, ref = None, None
+    if not args.no_accuracy:
+        ctx.set_particles(*state0)
+        _, info64, _ = one_step(t_state0, opts(0, args.pc))
+        ref = ctx.particles()
+        ctx.set_particles(*state0)
+        _, info1, _ = one_step(t_state0, opts(args.precision, args.pc))
+        e, p = force_err(ctx.particles(), ref)
+        acc = {"rel_rms_force_err": e, "max_abs_pos_err": p,
+               "vs": "FP64 GPU step, same state and solver settings",
+               "poisson_converged": [info64.reason_phi > 0, info1.reason_phi > 0],
+               "amg_levels": ctx.amg_levels() if args.pc == "amg" else None}
+
+    # (2) the reference configuration: FP64, Jacobi, same GMRES limits
+    refcfg = None
+    if not args.no_ref_config and args.pc != "jacobi":
+        ctx.set_particles(*state0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        _, infor, _ = one_step(t_state0, opts(0, "jacobi"))
+        ctx.sync()
+        wall_r = time.perf_counter() - t0
+        refcfg = {"value": round(N / wall_r, 1), "ms_per_step": round(1e3 * wall_r, 2),
+                  "gmres_its": (infor.its_rho, infor.its_phi),
+                  "poisson_converged": infor.reason_phi > 0,
+                  "note": "FP64, Jacobi GMRES(%d), max %d iterations; one step (includes its "
+                          "first-use allocations)" % (args.restart, args.max_it)}
+        if ref is not None:
+            e, p = force_err(ctx.particles(), ref)
+            refcfg["rel_rms_force_err_vs_converged_fp64"] = e
 
     def timed(precision):
         ctx.set_particles(*state0)
-        o = opts(precision)
+        o = opts(precision, args.pc)
         t = t_state0
         for _ in range(W):
             t, _, _ = one_step(t, o)
@@ -173,7 +209,7 @@ def main():
     if not args.no_fp64 and args.precision != 0:
         wall64, infos64, _, _, _ = timed(0)
         fp64 = {"value": round(N * K / wall64, 1), "ms_per_step": round(1e3 * wall64 / K, 2),
-                "gmres_its": infos64}
+                "gmres_its": infos64, "poisson_pc": args.pc}
     wall, infos, phases, kt, M = timed(args.precision)
 
     # roofline of the dominant kernel: largest total device time among the
@@ -213,11 +249,16 @@ def main():
         "config": {"workload": args.workload, "particles": N, "rows": M, "lm": params.lm,
                    "leaf_capacity": 8,
                    "gmres": {"rtol": args.rtol, "restart": args.restart, "max_it": args.max_it,
-                             "precision": "fp32-inner/fp64-refinement" if args.precision == 1
-                             else "fp64"},
+                             "poisson_pc": args.pc, "density_pc": "jacobi",
+                             "precision": ("fp32 multigrid in fp64 flexible gmres"
+                                           if args.pc == "amg" else
+                                           "fp32 inner gmres, fp64 refinement")
+                             if args.precision == 1 else "fp64"},
                    "gmres_its": infos, "phase_s": [round(v, 3) for v in phases]},
         "fp64_gpu": fp64,
         "speedup_vs_fp64": round(value / fp64["value"], 3) if fp64 else None,
+        "reference_config_fp64_gpu": refcfg,
+        "speedup_vs_reference_config": round(value / refcfg["value"], 3) if refcfg else None,
         "accuracy": acc,
         "roofline": roof,
         "cpu_baseline": None,
@@ -260,8 +301,8 @@ def cpu_baseline(args, w):
     wall = time.perf_counter() - t0
     return {"value": round(N / wall, 1), "unit": "particle-steps/s", "cores": cores,
             "kind": "port",
-            "sample": "1 FP64 step, %d^3 particles from the %s generator (S=%d nodes), %.1f s"
-                      % (n_side, args.workload, S, wall)}
+            "sample": "1 FP64 step (Jacobi GMRES, the reference's solver), %d^3 particles from "
+                      "the %s generator (S=%d nodes), %.1f s" % (n_side, args.workload, S, wall)}
 
 
 if __name__ == "__main__":
