@@ -57,8 +57,26 @@ struct AmgLevel {
   }
 };
 
+// A captured cycle: one per (input, output, status, precision) while the
+// hierarchy is unchanged.
+struct CycleGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void *in = nullptr, *st = nullptr, *a0 = nullptr;
+  void *out = nullptr;
+  int f32 = 0, gen = -1;
+  void release() {
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+};
+
 struct AmgHier {
   std::vector<AmgLevel *> L;
+  std::vector<CycleGraph> graphs;
+  int gen = 0;  // bumped by every setup; graphs of older generations are stale
   int nlev = 0;
   int nc = 0;
   bool dense = false;
@@ -70,6 +88,7 @@ struct AmgHier {
   DBuf<int> idx, idx2, sa, sb, flag, pos, cnt;
   DBuf<double> ev, smax, diag;
   ~AmgHier() {
+    for (auto &g : graphs) g.release();
     for (auto *l : L) { l->release(); delete l; }
     gj.release(); cinv.release(); colk.release(); cinv32.release(); key.release();
     key2.release(); idx.release(); idx2.release(); sa.release(); sb.release(); flag.release();
@@ -641,6 +660,9 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
               const AmgOptions &o) {
   if (!*hp) *hp = new AmgHier();
   AmgHier *h = *hp;
+  for (auto &gr : h->graphs) gr.release();
+  h->graphs.clear();
+  ++h->gen;
   h->o = o;
   h->fp32 = fp32;
   h->nlev = 0;
@@ -818,15 +840,59 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
   return 0;
 }
 
+// The cycle is a fixed sequence of a few hundred small launches (a W-cycle
+// visits the deep levels 2^l times), so it is captured into a HIP graph on
+// first use for a given (input, output, status word) and replayed; every
+// kernel in it still checks the status word.  With kernel timing on, the
+// launches are issued directly so that each one is timed.
 template <class T>
 int amg_apply(apc_ctx *c, AmgHier *h, const Op<T> &A, const T *in, T *out, const GState *st) {
   if (!h || h->nlev < 1) return fail(c, APC_ERR_ARG, "amg: no hierarchy");
   if (sizeof(T) == 4 && !h->fp32) return fail(c, APC_ERR_ARG, "amg: hierarchy has no FP32 copy");
   KScope ks(c, sizeof(T) == 4 ? "amg_cycle32" : "amg_cycle");
   T *tmp = ptrs<T>(h->L[0]).t;
+  static const bool nograph = getenv("APC_AMG_NOGRAPH") != nullptr;
+  if (c->timing || nograph) {
+    int r = cycle<T>(c, h, A, 0, in, out, tmp, st);
+    if (r) return r;
+    APC_HIP(c, hipGetLastError());
+    return 0;
+  }
+  const int f32 = sizeof(T) == 4;
+  for (auto &g : h->graphs)
+    if (g.exec && g.gen == h->gen && g.in == in && g.out == out && g.st == st && g.f32 == f32 &&
+        g.a0 == (const void *)A.s) {
+      APC_HIP(c, hipGraphLaunch(g.exec, c->s));
+      return 0;
+    }
+  CycleGraph g;
+  g.in = in;
+  g.out = out;
+  g.st = st;
+  g.f32 = f32;
+  g.gen = h->gen;
+  g.a0 = A.s;
+  APC_HIP(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
   int r = cycle<T>(c, h, A, 0, in, out, tmp, st);
-  if (r) return r;
-  APC_HIP(c, hipGetLastError());
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(c->s, &graph);
+  if (r) {
+    if (graph) hipGraphDestroy(graph);
+    return r;
+  }
+  APC_HIP(c, e);
+  g.graph = graph;
+  e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    g.release();
+    return hip_fail(c, e, "hipGraphInstantiate (multigrid cycle)");
+  }
+  if (h->graphs.size() >= 4) {
+    h->graphs.front().release();
+    h->graphs.erase(h->graphs.begin());
+  }
+  h->graphs.push_back(g);
+  APC_HIP(c, hipGraphLaunch(g.exec, c->s));
   return 0;
 }
 

@@ -127,8 +127,10 @@ __global__ void __launch_bounds__(256) k_spmv(Op<T> A, const T *__restrict__ vj,
 }
 
 // ---- flexible form: w = sig_j A z_j, z_j = M^-1 w_j computed in FP32 ----
+// (z is the cycle's fixed output buffer; it is also stored as z_j in zj)
 __global__ void __launch_bounds__(256) k_spmv_z(Op<double> A, const float *__restrict__ z,
-                                                double *__restrict__ w, int j, GDev g) {
+                                                double *__restrict__ w, float *__restrict__ zj,
+                                                int j, GDev g) {
   if (halted(g.st)) return;
   const double s = g.sig[j];
   int i = blockIdx.x * 256 + threadIdx.x;
@@ -138,7 +140,16 @@ __global__ void __launch_bounds__(256) k_spmv_z(Op<double> A, const float *__res
     int m = A.nbr[(size_t)q * A.M + i];
     if (m < A.M) acc += A.s[(size_t)q * A.M + i] * (double)z[m];
   }
-  w[i] = s * (A.d[i] * (double)z[i] + A.f[i] * acc);
+  const float zi = z[i];
+  zj[i] = zi;
+  w[i] = s * (A.d[i] * (double)zi + A.f[i] * acc);
+}
+
+template <class T>
+__global__ void k_copy_gated(int n, const T *__restrict__ a, T *__restrict__ b, const GState *st) {
+  if (halted(st)) return;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) b[i] = a[i];
 }
 
 __global__ void k_to_float_gated(int n, const double *__restrict__ a, float *__restrict__ b,
@@ -505,12 +516,14 @@ static int read_state(apc_ctx *c, SolveState *S) {
 // cycle's combination.  flex (T = double): the multigrid runs in FP32 on the
 // FP32 operator A32 and its outputs are kept, one per Arnoldi step, in Z32
 // ((m+1) x M floats) — flexible GMRES, so the inexact FP32 preconditioner
-// costs no accuracy and the cycle's update is x += Z32 y; v32 is an M-vector.
+// costs no accuracy and the cycle's update is x += Z32 y; v32 and z32 are
+// M-vectors (the cycle's fixed input and output, so it replays as a graph).
 template <class T>
 static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, const double *b,
                       double *x, const apc_ksp_opts &o, bool fp32, bool amg, T *zb, T *ub,
                       int *its_out, int *reason_out, double *rnorm_out, int *sweeps,
-                      const Op<float> *A32 = nullptr, float *Z32 = nullptr, float *v32 = nullptr) {
+                      const Op<float> *A32 = nullptr, float *Z32 = nullptr, float *v32 = nullptr,
+                      float *z32 = nullptr) {
   const bool flex = Z32 != nullptr;
   const int M = A64.M;
   const int m = std::min(std::max(1, o.restart), MAX_RESTART);
@@ -571,13 +584,15 @@ static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, 
         float *zj = Z32 + (size_t)j * ld;
         hipLaunchKernelGGL(k_to_float_gated, dim3(gb), dim3(256), 0, c->s, M, (const double *)vj,
                            v32, (const GState *)S->st.p);
-        r = amg_apply<float>(c, S->amg, *A32, v32, zj, S->st.p);
+        r = amg_apply<float>(c, S->amg, *A32, v32, z32, S->st.p);
         if (r) return r;
-        KScope ks(c, "gmres_spmv_z", 12.0 * nnz + 28.0 * M);
-        hipLaunchKernelGGL(k_spmv_z, dim3(gb), dim3(256), 0, c->s, A64, (const float *)zj,
-                           (double *)w, j, g);
+        KScope ks(c, "gmres_spmv_z", 12.0 * nnz + 32.0 * M);
+        hipLaunchKernelGGL(k_spmv_z, dim3(gb), dim3(256), 0, c->s, A64, (const float *)z32,
+                           (double *)w, zj, j, g);
       } else if (amg) {
-        r = amg_apply<T>(c, S->amg, AT, vj, zb, S->st.p);
+        hipLaunchKernelGGL(k_copy_gated<T>, dim3(gb), dim3(256), 0, c->s, M, vj, ub,
+                           (const GState *)S->st.p);
+        r = amg_apply<T>(c, S->amg, AT, ub, zb, S->st.p);
         if (r) return r;
         KScope ks(c, fp32 ? "gmres_spmv32" : "gmres_spmv", bspmv);
         hipLaunchKernelGGL(k_spmv<T>, dim3(gb), dim3(256), 0, c->s, AT, (const T *)zb, w, j, 0, g);
@@ -714,7 +729,7 @@ static int gmres(apc_ctx *c, const Op<double> &A, const double *b, double *x,
     APC_ALLOC(c, S->V64, (size_t)(m + 1) * M + 1);
     APC_ALLOC(c, S->Z32, (size_t)(m + 1) * M + 1);
     return run_cycles<double>(c, A, A, S->V64.p, b, x, o, false, true, S->z64.p, S->u64.p, its,
-                              reason, rnorm, sweeps, &A32, S->Z32.p, S->z32.p);
+                              reason, rnorm, sweeps, &A32, S->Z32.p, S->z32.p, S->u32.p);
   }
   return run_cycles<float>(c, A, A32, S->V32.p, b, x, o, true, false, S->z32.p, S->u32.p, its,
                            reason, rnorm, sweeps);

@@ -13,7 +13,12 @@
 //    pass each from per-octant offsets held in registers;
 //  * a first pass runs with a short candidate list (FAST_CAP); the few nodes
 //    whose expansion outgrows it are queued and searched again with the full
-//    bound.
+//    bound;
+//  * one launch of as many threads as the GPU keeps resident, each walking
+//    the rows with a stride and reusing its own scratch slot (no launch
+//    tails, scratch sized to residency).  The expansion lists are
+//    element-major (the scans run in lockstep and coalesce); the sort keys
+//    are thread-major, so a thread's partition scans stay in its own lines.
 #include <cstdlib>
 
 #include "apc_internal.h"
@@ -42,11 +47,11 @@ struct Cand {
   double *wd;    // sort keys
   int *wm;       // sort payload
   uint4 *fp;     // word k: 16-bit fingerprints of entries 8k..8k+7
-  int rows, t;
+  int rows, t, cap;  // rows: scratch slots (stride of the element-major arrays)
   __device__ uint32_t &MO(int e) const { return mo[(size_t)e * rows + t]; }
   __device__ double &D2(int e) const { return d2[(size_t)e * rows + t]; }
-  __device__ double &WD(int e) const { return wd[(size_t)e * rows + t]; }
-  __device__ int &WM(int e) const { return wm[(size_t)e * rows + t]; }
+  __device__ double &WD(int e) const { return wd[(size_t)t * cap + e]; }
+  __device__ int &WM(int e) const { return wm[(size_t)t * cap + e]; }
   __device__ uint4 FPW(int k) const { return fp[(size_t)k * rows + t]; }
   __device__ void set_fp(int e, uint32_t f) const {
     reinterpret_cast<uint16_t *>(fp + (size_t)(e >> 3) * rows + t)[e & 7] = (uint16_t)f;
@@ -125,23 +130,15 @@ __device__ void qsort_ref(const Cand &c, int l, int r) {
 
 constexpr int FAST_CAP = 256;  // multiple of 8 (fingerprint words)
 
-// Rows are row0 + t, or list[t] when a list is given.  With ovf set, a node
-// whose candidate list would exceed cap is appended to ovf (count in
-// status[2]) instead of failing.
-__global__ void __launch_bounds__(256) k_search(int M, int S, int row0, int rows,
-                                                const int *__restrict__ list, int lm, int cap,
-                                                Frame F, const double *__restrict__ cx,
-                                                const double *__restrict__ cy,
-                                                const double *__restrict__ cz,
-                                                const int *__restrict__ n24,
-                                                const int *__restrict__ n24cnt, uint32_t *mo,
-                                                double *d2, double *wd, int *wm, uint4 *fpw,
-                                                int *__restrict__ nbr, double *__restrict__ dist,
-                                                int *__restrict__ status, int *__restrict__ ovf) {
-  int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= rows) return;
-  const int i0 = list ? list[t] : row0 + t;
-  Cand c{mo, d2, wd, wm, fpw, rows, t};
+// Neighbours of node i0 with the candidate lists in scratch slot c.t.
+// Returns 0, 1 if the list would exceed cap and ovf is given (the node is
+// appended to ovf, count in status[2]), or -1 after setting status.
+__device__ int search_node(int i0, const Cand &c, int M, int S, int lm, int cap, const Frame &F,
+                           const double *__restrict__ cx, const double *__restrict__ cy,
+                           const double *__restrict__ cz, const int *__restrict__ n24,
+                           const int *__restrict__ n24cnt, int *__restrict__ nbr,
+                           double *__restrict__ dist, int *__restrict__ status,
+                           int *__restrict__ ovf) {
   const int nq = 10 * lm;
   const double x0 = cx[i0], y0 = cy[i0], z0 = cz[i0];
   // wrapped offset to node m: octant in the rotated frame and squared
@@ -194,19 +191,19 @@ __global__ void __launch_bounds__(256) k_search(int M, int S, int row0, int rows
       int cnt = (src < M) ? n24cnt[src] : 0;
       for (int q = 0; q < cnt; ++q) {
         int m = n24[(size_t)src * 24 + q];
-        if (m >= S) { status[0] = APC_ERR_NEIGH_INVALID; return; }
+        if (m >= S) { status[0] = APC_ERR_NEIGH_INVALID; return -1; }
         if (m == i0 || listed(c, l, (uint32_t)m)) continue;
         double dd;
         int o = place(m, &dd);
-        if (sel8(nn, o) >= nq) { status[0] = APC_ERR_NEIGH_MANY; return; }
+        if (sel8(nn, o) >= nq) { status[0] = APC_ERR_NEIGH_MANY; return -1; }
         if (l >= cap) {
           if (ovf) {
             ovf[atomicAdd(&status[2], 1)] = i0;
-          } else {
-            status[0] = APC_ERR_UNSUPPORTED;
-            status[1] = i0;
+            return 1;
           }
-          return;
+          status[0] = APC_ERR_UNSUPPORTED;
+          status[1] = i0;
+          return -1;
         }
         c.MO(l) = (uint32_t)m | ((uint32_t)o << 29);
         c.D2(l) = dd;
@@ -216,7 +213,7 @@ __global__ void __launch_bounds__(256) k_search(int M, int S, int row0, int rows
       }
     }
     ite++;
-    if (ite > 5) { status[0] = APC_ERR_NEIGH_FEW; return; }
+    if (ite > 5) { status[0] = APC_ERR_NEIGH_FEW; return -1; }
   }
   // The reference sorts td[k..k+nn[o]-1] but reads octant o's own slots
   // (solver.cpp:557-561), so only octant 0 is sorted; for o >= 1 the range
@@ -225,7 +222,11 @@ __global__ void __launch_bounds__(256) k_search(int M, int S, int row0, int rows
     int ko = nn[0];
 #pragma unroll
     for (int o = 1; o < 8; ++o) {
-      if (nn[o] >= 2 && ko + nn[o] - 1 >= nq) { status[0] = APC_ERR_UNSUPPORTED; status[1] = i0; return; }
+      if (nn[o] >= 2 && ko + nn[o] - 1 >= nq) {
+        status[0] = APC_ERR_UNSUPPORTED;
+        status[1] = i0;
+        return -1;
+      }
       ko += nn[o];
     }
   }
@@ -270,6 +271,29 @@ __global__ void __launch_bounds__(256) k_search(int M, int S, int row0, int rows
     nbr[(size_t)q * M + i0] = m;
     dist[(size_t)q * M + i0] = sqrt(dd);
   }
+  return 0;
+}
+
+// Rows q = 0..nrows-1 (node q, or list[q] when a list is given), strided over
+// nslots threads; thread t owns scratch slot t.
+__global__ void __launch_bounds__(256) k_search(int M, int S, int nrows, const int *__restrict__ list,
+                                                int lm, int cap, int nslots, Frame F,
+                                                const double *__restrict__ cx,
+                                                const double *__restrict__ cy,
+                                                const double *__restrict__ cz,
+                                                const int *__restrict__ n24,
+                                                const int *__restrict__ n24cnt, uint32_t *mo,
+                                                double *d2, double *wd, int *wm, uint4 *fpw,
+                                                int *__restrict__ nbr, double *__restrict__ dist,
+                                                int *__restrict__ status, int *__restrict__ ovf) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= nslots) return;
+  const Cand c{mo, d2, wd, wm, fpw, nslots, t, cap};
+  for (int q = t; q < nrows; q += nslots) {
+    const int i0 = list ? list[q] : q;
+    if (search_node(i0, c, M, S, lm, cap, F, cx, cy, cz, n24, n24cnt, nbr, dist, status, ovf) < 0)
+      return;
+  }
 }
 
 int search_neighbors(apc_ctx *c) {
@@ -301,15 +325,20 @@ int search_neighbors(apc_ctx *c) {
   // full candidate bound: an expansion round starts with fewer than lm
   // candidates, each adding at most 24 (the 2:1 bound), so 25*lm suffices
   const int cap_full = (25 * lm + 32 + 7) & ~7;
-  const int chunk = std::max(1, std::min(M, 1 << 19));
-  const int chunk_full = std::max(1, std::min(M, 1 << 16));
   // APC_NEIGH_FAST_CAP shortens the first-pass list (tests drive the second
   // pass with it); clamped to [32, FAST_CAP], multiple of 8
   int fast_cap = FAST_CAP;
   if (const char *v = getenv("APC_NEIGH_FAST_CAP"))
     fast_cap = std::min(FAST_CAP, std::max(32, atoi(v))) & ~7;
-  const size_t per_row_fast = (size_t)fast_cap, per_row_full = (size_t)cap_full;
-  const size_t elems = std::max(per_row_fast * chunk, per_row_full * chunk_full);
+  if (c->search_slots == 0) {  // resident threads of k_search on this device
+    int nb = 0, ncu = 0;
+    APC_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_search, 256, 0));
+    APC_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
+    c->search_slots = std::max(1, nb) * std::max(1, ncu) * 256;
+  }
+  const int slots = std::max(1, std::min(M, c->search_slots));
+  const int slots_full = std::max(1, std::min(M, 1 << 16));
+  const size_t elems = std::max((size_t)fast_cap * slots, (size_t)cap_full * slots_full);
   APC_ALLOC(c, c->nscratch, 2 * elems);
   APC_ALLOC(c, c->nscratch_i, 2 * elems);
   APC_ALLOC(c, c->nscratch_f, elems / 2 + 4);  // 16-bit fingerprints as int
@@ -319,23 +348,23 @@ int search_neighbors(apc_ctx *c) {
   double *d2 = c->nscratch.p;
   double *wd = c->nscratch.p + elems;
   uint4 *fpw = (uint4 *)c->nscratch_f.p;
-  for (int row0 = 0; row0 < M; row0 += chunk) {
-    int rows = std::min(chunk, M - row0);
+  {
     KScope ks(c, "neighbor_search");
-    hipLaunchKernelGGL(k_search, dim3(nblocks(rows, 256)), dim3(256), 0, c->s, M, S, row0, rows,
-                       (const int *)nullptr, lm, fast_cap, F, c->cx.p, c->cy.p, c->cz.p, c->n24.p,
-                       c->n24cnt.p, mo, d2, wd, wm, fpw, c->nbr.p, c->dist.p, c->status_dev.p,
-                       c->novf.p);
+    hipLaunchKernelGGL(k_search, dim3(nblocks(slots, 256)), dim3(256), 0, c->s, M, S, M,
+                       (const int *)nullptr, lm, fast_cap, slots, F, c->cx.p, c->cy.p, c->cz.p,
+                       c->n24.p, c->n24cnt.p, mo, d2, wd, wm, fpw, c->nbr.p, c->dist.p,
+                       c->status_dev.p, c->novf.p);
   }
   int st[3] = {0, 0, 0};
   APC_HIP(c, hipMemcpyAsync(st, c->status_dev.p, sizeof(st), hipMemcpyDeviceToHost, c->s));
   APC_HIP(c, hipStreamSynchronize(c->s));
   // nodes whose expansion outgrew the short list, with the full bound
-  for (int r0 = 0; st[0] == 0 && r0 < st[2]; r0 += chunk_full) {
-    int rows = std::min(chunk_full, st[2] - r0);
+  if (st[0] == 0 && st[2] > 0) {
+    const int rows = st[2];
+    const int ns = std::min(rows, slots_full);
     KScope ks(c, "neighbor_search_long");
-    hipLaunchKernelGGL(k_search, dim3(nblocks(rows, 256)), dim3(256), 0, c->s, M, S, 0, rows,
-                       (const int *)c->novf.p + r0, lm, cap_full, F, c->cx.p, c->cy.p, c->cz.p,
+    hipLaunchKernelGGL(k_search, dim3(nblocks(ns, 256)), dim3(256), 0, c->s, M, S, rows,
+                       (const int *)c->novf.p, lm, cap_full, ns, F, c->cx.p, c->cy.p, c->cz.p,
                        c->n24.p, c->n24cnt.p, mo, d2, wd, wm, fpw, c->nbr.p, c->dist.p,
                        c->status_dev.p, (int *)nullptr);
   }

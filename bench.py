@@ -30,7 +30,8 @@ settings) for one step: its time, iterations and its force difference from
 the converged FP64 step (skipped with --no-ref-config); (3) the FP64 step
 with the same solver settings timed over K steps (skipped with --no-fp64);
 (4) the measured precision timed over K steps -> "value".  The roofline entry
-is for the kernel with the largest device time in leg (4).
+is for the kernel with the largest device time in one extra step of leg (4)
+run with the library's kernel timers on (the timed steps run without them).
 """
 import argparse
 import json
@@ -184,15 +185,16 @@ def main():
             e, p = force_err(ctx.particles(), ref)
             refcfg["rel_rms_force_err_vs_converged_fp64"] = e
 
-    def timed(precision):
+    def timed(precision, profile=False):
+        """K steps from state0 after W warmup steps, kernel timers off; with
+        profile, one more step with the library's kernel timers on (HIP
+        events on its stream, launches issued directly) for the roofline."""
         ctx.set_particles(*state0)
         o = opts(precision, args.pc)
         t = t_state0
         for _ in range(W):
             t, _, _ = one_step(t, o)
         ctx.sync()
-        ctx.set_timing(True)
-        ctx.reset_kernel_times()
         infos, phases = [], np.zeros(3)
         t0 = time.perf_counter()
         for _ in range(K):
@@ -201,8 +203,14 @@ def main():
             phases += timing
         ctx.sync()
         wall = time.perf_counter() - t0
-        kt = ctx.kernel_times()
-        ctx.set_timing(False)
+        kt = None
+        if profile:
+            ctx.set_timing(True)
+            ctx.reset_kernel_times()
+            one_step(t, o)
+            ctx.sync()
+            kt = ctx.kernel_times()
+            ctx.set_timing(False)
         return wall, infos, phases, kt, info.nc + info.nv
 
     fp64 = None
@@ -210,7 +218,7 @@ def main():
         wall64, infos64, _, _, _ = timed(0)
         fp64 = {"value": round(N * K / wall64, 1), "ms_per_step": round(1e3 * wall64 / K, 2),
                 "gmres_its": infos64, "poisson_pc": args.pc}
-    wall, infos, phases, kt, M = timed(args.precision)
+    wall, infos, phases, kt, M = timed(args.precision, profile=True)
 
     # roofline of the dominant kernel: largest total device time among the
     # kernels whose launches carry algorithmic byte counts
