@@ -146,7 +146,8 @@ int apc_destroy(apc_ctx *c) {
                              &c->t0, &c->t1, &c->t2, &c->t3, &c->t4, &c->t5, &c->ncx, &c->ncy,
                              &c->ncz, &c->cx, &c->cy, &c->cz, &c->ch, &c->dist, &c->sco, &c->d1,
                              &c->d2, &c->f1, &c->f2, &c->dinv1, &c->dinv2, &c->rhs1, &c->pbnd,
-                             &c->bv, &c->rho, &c->phi, &c->rhs2, &c->grad, &c->nscratch, &c->red_d};
+                             &c->bv, &c->rho, &c->phi, &c->rhs2, &c->grad, &c->nscratch, &c->red_d,
+                             &c->nxyz};
   for (auto b : ds) b->release();
   apc::DBuf<int> *is[] = {&c->id, &c->tid, &c->perm, &c->perm2, &c->hist, &c->histx,
                           &c->scratch_i, &c->nlevel, &c->nfirst, &c->nnp, &c->nstatus, &c->nchild,

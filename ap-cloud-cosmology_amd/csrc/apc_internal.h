@@ -95,6 +95,7 @@ struct apc_ctx {
   apc::DBuf<int> nscratch_i;
   apc::DBuf<int> nscratch_f;   // candidate fingerprints
   apc::DBuf<int> novf;         // nodes searched again with the full list bound
+  apc::DBuf<double> nxyz;      // node centres packed 4 doubles per node
   int nsearch_long = 0;
   int search_slots = 0;        // resident threads of the neighbour search
   apc::DBuf<int> status_dev;

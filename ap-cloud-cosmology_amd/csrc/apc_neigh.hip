@@ -108,6 +108,10 @@ __device__ int qpart(const Cand &c, int l, int r) {
   return j;
 }
 
+// Each pushed range is at least as large as the side the loop continues
+// with, so d stacked ranges need n >= 2^d elements: at most log2(n) + 1
+// entries (the arrays are larger than that needs; measured, a 16-entry stack
+// made the whole search 20% slower).
 __device__ void qsort_ref(const Cand &c, int l, int r) {
   int sl[40], sr[40];
   int top = 0;
@@ -133,20 +137,21 @@ constexpr int FAST_CAP = 256;  // multiple of 8 (fingerprint words)
 // Neighbours of node i0 with the candidate lists in scratch slot c.t.
 // Returns 0, 1 if the list would exceed cap and ovf is given (the node is
 // appended to ovf, count in status[2]), or -1 after setting status.
+// xyz: node centres packed (x, y, z, unused), one 32-byte load per node.
 __device__ int search_node(int i0, const Cand &c, int M, int S, int lm, int cap, const Frame &F,
-                           const double *__restrict__ cx, const double *__restrict__ cy,
-                           const double *__restrict__ cz, const int *__restrict__ n24,
-                           const int *__restrict__ n24cnt, int *__restrict__ nbr,
-                           double *__restrict__ dist, int *__restrict__ status,
-                           int *__restrict__ ovf) {
+                           const double4 *__restrict__ xyz, const int *__restrict__ n24, const int *__restrict__ n24cnt,
+                           int *__restrict__ nbr, double *__restrict__ dist,
+                           int *__restrict__ status, int *__restrict__ ovf) {
   const int nq = 10 * lm;
-  const double x0 = cx[i0], y0 = cy[i0], z0 = cz[i0];
+  const double4 p0 = xyz[i0];
+  const double x0 = p0.x, y0 = p0.y, z0 = p0.z;
   // wrapped offset to node m: octant in the rotated frame and squared
   // distance (solver.cpp:476-479, 595-598)
   auto place = [&](int m, double *dd) {
-    double ex = pwrap(cx[m] - x0, F.len[0], F.bc);
-    double ey = pwrap(cy[m] - y0, F.len[1], F.bc);
-    double ez = pwrap(cz[m] - z0, F.len[2], F.bc);
+    const double4 pm = xyz[m];
+    double ex = pwrap(pm.x - x0, F.len[0], F.bc);
+    double ey = pwrap(pm.y - y0, F.len[1], F.bc);
+    double ez = pwrap(pm.z - z0, F.len[2], F.bc);
     double xt = F.A[0] * ex + F.A[1] * ey + F.A[2] * ez;
     double yt = F.A[3] * ex + F.A[4] * ey + F.A[5] * ez;
     double zt = F.A[6] * ex + F.A[7] * ey + F.A[8] * ez;
@@ -265,7 +270,7 @@ __device__ int search_node(int i0, const Cand &c, int M, int S, int lm, int cap,
   }
   qsort_ref(c, 0, l - 1);
   for (int q = 0; q < lm; ++q) {
-    int m = c.WM(q);
+    const int m = c.WM(q);
     double dd;
     (void)place(m, &dd);
     nbr[(size_t)q * M + i0] = m;
@@ -274,13 +279,17 @@ __device__ int search_node(int i0, const Cand &c, int M, int S, int lm, int cap,
   return 0;
 }
 
+__global__ void k_pack_xyz(int n, const double *__restrict__ x, const double *__restrict__ y,
+                           const double *__restrict__ z, double4 *__restrict__ xyz) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) xyz[i] = make_double4(x[i], y[i], z[i], 0.0);
+}
+
 // Rows q = 0..nrows-1 (node q, or list[q] when a list is given), strided over
 // nslots threads; thread t owns scratch slot t.
 __global__ void __launch_bounds__(256) k_search(int M, int S, int nrows, const int *__restrict__ list,
                                                 int lm, int cap, int nslots, Frame F,
-                                                const double *__restrict__ cx,
-                                                const double *__restrict__ cy,
-                                                const double *__restrict__ cz,
+                                                const double4 *__restrict__ xyz,
                                                 const int *__restrict__ n24,
                                                 const int *__restrict__ n24cnt, uint32_t *mo,
                                                 double *d2, double *wd, int *wm, uint4 *fpw,
@@ -291,7 +300,7 @@ __global__ void __launch_bounds__(256) k_search(int M, int S, int nrows, const i
   const Cand c{mo, d2, wd, wm, fpw, nslots, t, cap};
   for (int q = t; q < nrows; q += nslots) {
     const int i0 = list ? list[q] : q;
-    if (search_node(i0, c, M, S, lm, cap, F, cx, cy, cz, n24, n24cnt, nbr, dist, status, ovf) < 0)
+    if (search_node(i0, c, M, S, lm, cap, F, xyz, n24, n24cnt, nbr, dist, status, ovf) < 0)
       return;
   }
 }
@@ -300,6 +309,7 @@ int search_neighbors(apc_ctx *c) {
   if (!c->have_comp) return fail(c, APC_ERR_ARG, "neighbours: no computational particles");
   const int M = c->nc + c->nv, S = M + c->nb, lm = c->prm.lm;
   if (lm < 9) return fail(c, APC_ERR_ARG, "NumberOfNeighbors must be >= 9");
+  if (lm > 1000) return fail(c, APC_ERR_UNSUPPORTED, "NumberOfNeighbors above 1000 not supported");
   APC_ALLOC(c, c->nbr, (size_t)lm * M + 1);
   APC_ALLOC(c, c->dist, (size_t)lm * M + 1);
   APC_ALLOC(c, c->status_dev, 4);
@@ -348,11 +358,14 @@ int search_neighbors(apc_ctx *c) {
   double *d2 = c->nscratch.p;
   double *wd = c->nscratch.p + elems;
   uint4 *fpw = (uint4 *)c->nscratch_f.p;
+  APC_ALLOC(c, c->nxyz, 4 * (size_t)S + 4);
+  double4 *xyz = (double4 *)c->nxyz.p;
+  hipLaunchKernelGGL(k_pack_xyz, dim3(nblocks(S, 256)), dim3(256), 0, c->s, S, c->cx.p, c->cy.p,
+                     c->cz.p, xyz);
   {
     KScope ks(c, "neighbor_search");
     hipLaunchKernelGGL(k_search, dim3(nblocks(slots, 256)), dim3(256), 0, c->s, M, S, M,
-                       (const int *)nullptr, lm, fast_cap, slots, F, c->cx.p, c->cy.p, c->cz.p,
-                       c->n24.p, c->n24cnt.p, mo, d2, wd, wm, fpw, c->nbr.p, c->dist.p,
+                       (const int *)nullptr, lm, fast_cap, slots, F, (const double4 *)xyz, c->n24.p, c->n24cnt.p, mo, d2, wd, wm, fpw, c->nbr.p, c->dist.p,
                        c->status_dev.p, c->novf.p);
   }
   int st[3] = {0, 0, 0};
@@ -364,8 +377,7 @@ int search_neighbors(apc_ctx *c) {
     const int ns = std::min(rows, slots_full);
     KScope ks(c, "neighbor_search_long");
     hipLaunchKernelGGL(k_search, dim3(nblocks(ns, 256)), dim3(256), 0, c->s, M, S, rows,
-                       (const int *)c->novf.p, lm, cap_full, ns, F, c->cx.p, c->cy.p, c->cz.p,
-                       c->n24.p, c->n24cnt.p, mo, d2, wd, wm, fpw, c->nbr.p, c->dist.p,
+                       (const int *)c->novf.p, lm, cap_full, ns, F, (const double4 *)xyz, c->n24.p, c->n24cnt.p, mo, d2, wd, wm, fpw, c->nbr.p, c->dist.p,
                        c->status_dev.p, (int *)nullptr);
   }
   if (st[2] > 0 && st[0] == 0) {
