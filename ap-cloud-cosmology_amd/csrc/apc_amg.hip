@@ -783,12 +783,14 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
       hipLaunchKernelGGL((k_cy_sweep<T, true>), dim3(g), dim3(256), 0, c->s, E, C, dinv, b, xin,
                          xout, o.omega, st);
     } else {
+      KScope ks(c, "amg_sweep_coarse");
       hipLaunchKernelGGL((k_cy_sweep<T, false>), dim3(g), dim3(256), 0, c->s, E, C, dinv, b, xin,
                          xout, o.omega, st);
     }
   };
   if (l == h->nlev - 1) {
     if (h->dense) {
+      KScope ks(c, "amg_dense");
       hipLaunchKernelGGL((k_cy_dense<T>), dim3(n), dim3(64), 0, c->s, n, cinv_of<T>(h), b, x, st);
     } else {  // no dense solve: smoothing only (odd sweep count ends in x)
       hipLaunchKernelGGL((k_cy_first<T>), dim3(g), dim3(256), 0, c->s, n, dinv, b, x, o.omega, st);
@@ -802,7 +804,10 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
   // smoothing writes alternate between tmp and x; with 2*nu - 1 sweeps after
   // the first (odd), starting in tmp ends in x
   T *cur = tmp, *alt = x;
-  hipLaunchKernelGGL((k_cy_first<T>), dim3(g), dim3(256), 0, c->s, n, dinv, b, cur, o.omega, st);
+  {
+    KScope ks(c, ell ? "amg_first0" : "amg_first_coarse");
+    hipLaunchKernelGGL((k_cy_first<T>), dim3(g), dim3(256), 0, c->s, n, dinv, b, cur, o.omega, st);
+  }
   for (int s = 1; s < o.nu; ++s) {
     sweep(cur, alt);
     std::swap(cur, alt);
@@ -810,11 +815,13 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
   AmgLevel *Lc = h->L[l + 1];
   LevelPtrs<T> Pc = ptrs<T>(Lc);
   T *r = P.r;
-  for (int q = 0; q < o.gamma; ++q) {
+  const int gam = (l < o.wlevels) ? o.gamma : 1;
+  for (int q = 0; q < gam; ++q) {
     if (ell) {
       KScope ks(c, f32 ? "amg_resid0_32" : "amg_resid0", (double)n * (lm0 * (4 + 2 * sT) + 5 * sT));
       hipLaunchKernelGGL((k_cy_resid<T, true>), dim3(g), dim3(256), 0, c->s, E, C, b, cur, r, st);
     } else {
+      KScope ks(c, "amg_resid_coarse");
       hipLaunchKernelGGL((k_cy_resid<T, false>), dim3(g), dim3(256), 0, c->s, E, C, b, cur, r, st);
     }
     {

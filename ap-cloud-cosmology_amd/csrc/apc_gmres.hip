@@ -675,6 +675,7 @@ static AmgOptions amg_options() {
   AmgOptions a;
   a.nu = env_int("APC_AMG_NU", a.nu);
   a.gamma = env_int("APC_AMG_GAMMA", a.gamma);
+  a.wlevels = env_int("APC_AMG_WLEVELS", a.wlevels);
   a.coarse_max = env_int("APC_AMG_COARSE", a.coarse_max);
   if (const char *v = getenv("APC_AMG_OMEGA")) a.omega = atof(v);
   if (const char *v = getenv("APC_AMG_ALPHA")) a.alpha = atof(v);

@@ -44,16 +44,18 @@ __device__ inline double row_apply(const Op<T> &A, const T *__restrict__ x, int 
 
 struct AmgHier;  // apc_amg.hip
 
-// Defaults from sweeps on the 256^3 Zel'dovich + Plummer system (7.5M rows,
-// 6 levels): a W-cycle with one sweep each side and a 1.5x coarse correction
-// converged in the fewest seconds (64 GMRES steps to rtol 1e-8).
+// Defaults from sweeps on the 256^3 Zel'dovich + Plummer system (7.5M rows):
+// a W-cycle on every level with one sweep each side, damping 0.85, a 1.5x
+// coarse correction and a dense solve once a level has <= 1024 rows (5
+// levels) converged in the fewest seconds (55 GMRES steps to rtol 1e-8).
 struct AmgOptions {
   int nu = 1;             // Jacobi sweeps before and after the coarse correction
-  double omega = 0.7;     // Jacobi damping
+  double omega = 0.85;    // Jacobi damping
   double alpha = 1.5;     // scaling of the coarse correction
   int gamma = 2;          // 1: V-cycle, 2: W-cycle
+  int wlevels = 64;       // gamma applies on the first wlevels levels, V below
   double theta = 0.0;     // strength threshold (fraction of the row's largest |a_ij|)
-  int coarse_max = 256;   // dense solve at or below this many rows
+  int coarse_max = 1024;  // dense solve at or below this many rows
   int max_levels = 12;
 };
 
