@@ -31,7 +31,9 @@ EXPORTS = [
     "apc_solver_solve", "apc_solver_amg_levels", "apc_differentiate",
     "apc_get_fields", "apc_set_gradients", "apc_interp_kick", "apc_drift", "apc_max_density",
     "apc_step", "apc_kernel_times", "apc_reset_kernel_times", "apc_set_timing", "apc_sync",
-    "apc_stream_copy",
+    "apc_stream_copy", "apc_freeze_topology", "apc_set_state", "apc_checkpoint",
+    "apc_adjoint_seed_cic", "apc_adjoint_cic_weights", "apc_adjoint_get", "apc_adjoint_set", "apc_adjoint_drift",
+    "apc_adjoint_kick", "apc_adjoint_fields", "apc_gradient_adjoint", "apc_solve_adjoint",
 ]
 
 
@@ -342,6 +344,78 @@ class Context:
         self._chk(self.L.apc_solver_amg_levels(self.h, _p(rows, _I), nnz.ctypes.data_as(C.c_void_p),
                                                32, C.byref(n)), "apc_solver_amg_levels")
         return [(int(rows[k]), int(nnz[k])) for k in range(min(n.value, 32))]
+
+    # ---- differentiation form (topology held fixed) ----
+    def freeze_topology(self, on=True):
+        self._chk(self.L.apc_freeze_topology(self.h, 1 if on else 0), "apc_freeze_topology")
+
+    def set_state(self, x, y, z, vx, vy, vz):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (x, y, z, vx, vy, vz)]
+        self._chk(self.L.apc_set_state(self.h, *[_p(a) for a in arrs]), "apc_set_state")
+
+    def checkpoint(self, slot, restore=False):
+        self._chk(self.L.apc_checkpoint(self.h, slot, 1 if restore else 0), "apc_checkpoint")
+
+    def adjoint_seed_cic(self, n):
+        out = C.c_double()
+        self._chk(self.L.apc_adjoint_seed_cic(self.h, n, C.byref(out)), "apc_adjoint_seed_cic")
+        return out.value
+
+    def adjoint_cic_weights(self, w=None):
+        """Per-particle weights of the CIC loss (caller order); None: equal."""
+        if w is None:
+            self._chk(self.L.apc_adjoint_cic_weights(self.h, None), "apc_adjoint_cic_weights")
+            return
+        w = np.ascontiguousarray(w, np.float64)
+        assert w.shape == (self.n,)
+        self._chk(self.L.apc_adjoint_cic_weights(self.h, _p(w)), "apc_adjoint_cic_weights")
+
+    def adjoint_get(self):
+        out = [np.zeros(self.n) for _ in range(6)]
+        self._chk(self.L.apc_adjoint_get(self.h, *[_p(a) for a in out]), "apc_adjoint_get")
+        return out
+
+    def adjoint_set(self, lx, ly, lz, lvx, lvy, lvz):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (lx, ly, lz, lvx, lvy, lvz)]
+        self._chk(self.L.apc_adjoint_set(self.h, *[_p(a) for a in arrs]), "apc_adjoint_set")
+
+    def adjoint_drift(self, cx, dt):
+        self._chk(self.L.apc_adjoint_drift(self.h, C.c_double(cx), C.c_double(dt)),
+                  "apc_adjoint_drift")
+
+    def adjoint_kick(self, cv, dt, accumulate=False, clear=False):
+        self._chk(self.L.apc_adjoint_kick(self.h, C.c_double(cv), C.c_double(dt),
+                                          1 if accumulate else 0, 1 if clear else 0),
+                  "apc_adjoint_kick")
+
+    def adjoint_fields(self):
+        M = self.comp_rows()
+        dg, dH = np.zeros(3 * M), np.zeros(6 * M)
+        self._chk(self.L.apc_adjoint_fields(self.h, _p(dg), _p(dH)), "apc_adjoint_fields")
+        return dg.reshape(M, 3), dH.reshape(M, 6)
+
+    def gradient_adjoint(self, dg, dH):
+        M = self.comp_rows()
+        dg = np.ascontiguousarray(dg, np.float64).reshape(M * 3)
+        dH = np.ascontiguousarray(dH, np.float64).reshape(M * 6)
+        out = np.zeros(M)
+        self._chk(self.L.apc_gradient_adjoint(self.h, _p(dg), _p(dH), _p(out)),
+                  "apc_gradient_adjoint")
+        return out
+
+    def solve_adjoint(self, which, opts, rhs):
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        out = np.zeros_like(rhs)
+        its, reason = C.c_int(), C.c_int()
+        self._chk(self.L.apc_solve_adjoint(self.h, which, C.byref(opts), _p(rhs), _p(out),
+                                           C.byref(its), C.byref(reason)), "apc_solve_adjoint")
+        return out, its.value, reason.value
+
+    def comp_rows(self):
+        nc, nv = C.c_int(), C.c_int()
+        self._chk(self.L.apc_get_comp_sizes(self.h, C.byref(nc), C.byref(nv), None, None),
+                  "apc_get_comp_sizes")
+        return nc.value + nv.value
 
     def set_timing(self, on=True):
         self._chk(self.L.apc_set_timing(self.h, 1 if on else 0), "apc_set_timing")

@@ -23,12 +23,12 @@ def lattice(n_side):
     return X.ravel(), Y.ravel(), Z.ravel()
 
 
-def zeldovich(n_side, seed, rms, spectral_index=-2.0, a0=0.02):
-    """Perturbed unit-cube lattice: Zel'dovich displacements of a Gaussian
-    field with P(k) ~ k^n (psi_k = i k delta_k / k^2), scaled to the given
-    RMS displacement; velocities are the growing mode in the drift convention
-    dx/da = Cx v with Cx = sqrt(a) (Einstein-de Sitter), v = psi / (a0^1.5).
-    Returns x, y, z, vx, vy, vz (periodically wrapped into [0,1))."""
+def zeldovich_field(n_side, seed, rms, spectral_index=-2.0, a0=0.02):
+    """Lattice positions q (3, N), Zel'dovich displacements psi (3, N) of a
+    Gaussian field with P(k) ~ k^n (psi_k = i k delta_k / k^2) scaled to the
+    given RMS, and the growing-mode velocity factor v = vfac * psi in the
+    drift convention dx/da = Cx v with Cx = sqrt(a) (Einstein-de Sitter),
+    vfac = a0^-1.5."""
     rng = np.random.default_rng(seed)
     n = n_side
     w = rng.standard_normal((n, n, n))
@@ -45,13 +45,16 @@ def zeldovich(n_side, seed, rms, spectral_index=-2.0, a0=0.02):
     for K in (KX, KY, KZ):
         psi.append(np.fft.irfftn(1j * K * D / k2, s=(n, n, n), axes=(0, 1, 2)).ravel())
     psi = np.stack(psi)
-    scale = rms / np.sqrt(np.mean(np.sum(psi * psi, axis=0)))
-    psi *= scale
-    qx, qy, qz = lattice(n)
-    x = np.mod(qx + psi[0], 1.0)
-    y = np.mod(qy + psi[1], 1.0)
-    z = np.mod(qz + psi[2], 1.0)
-    vfac = 1.0 / (a0 * np.sqrt(a0))
+    psi *= rms / np.sqrt(np.mean(np.sum(psi * psi, axis=0)))
+    q = np.stack(lattice(n))
+    return q, psi, 1.0 / (a0 * np.sqrt(a0))
+
+
+def zeldovich(n_side, seed, rms, spectral_index=-2.0, a0=0.02):
+    """Perturbed unit-cube lattice (zeldovich_field): returns x, y, z, vx, vy,
+    vz, positions periodically wrapped into [0,1)."""
+    q, psi, vfac = zeldovich_field(n_side, seed, rms, spectral_index, a0)
+    x, y, z = (np.mod(q[d] + psi[d], 1.0) for d in range(3))
     return x, y, z, psi[0] * vfac, psi[1] * vfac, psi[2] * vfac
 
 

@@ -147,16 +147,20 @@ int apc_destroy(apc_ctx *c) {
                              &c->ncz, &c->cx, &c->cy, &c->cz, &c->ch, &c->dist, &c->sco, &c->d1,
                              &c->d2, &c->f1, &c->f2, &c->dinv1, &c->dinv2, &c->rhs1, &c->pbnd,
                              &c->bv, &c->rho, &c->phi, &c->rhs2, &c->grad, &c->nscratch, &c->red_d,
-                             &c->nxyz};
+                             &c->nxyz, &c->lam, &c->dfield, &c->cic, &c->adj_tmp, &c->adj_tmp2,
+                             &c->adj_vec, &c->adj_vec2, &c->cic_w};
   for (auto b : ds) b->release();
   apc::DBuf<int> *is[] = {&c->id, &c->tid, &c->perm, &c->perm2, &c->hist, &c->histx,
                           &c->scratch_i, &c->nlevel, &c->nfirst, &c->nnp, &c->nstatus, &c->nchild,
                           &c->nparent, &c->nflag, &c->nrank, &c->nout, &c->dcount, &c->leaf_node,
                           &c->cnp, &c->cfirst, &c->ctype, &c->n24, &c->n24cnt, &c->n24first,
                           &c->owner, &c->nbr, &c->nscratch_i, &c->nscratch_f, &c->novf,
-                          &c->status_dev};
+                          &c->status_dev, &c->tr_pos, &c->tr_pos2, &c->tr_rp, &c->cic_v,
+                          &c->cic_v2, &c->cic_rng};
   for (auto b : is) b->release();
+  for (auto &b : c->ckpt) b.release();
   c->key.release(); c->key2.release(); c->nprefix.release(); c->red_u64.release();
+  c->tr_key.release(); c->tr_key2.release(); c->cic_key.release(); c->cic_key2.release();
   hipStreamDestroy(c->s);
   delete c;
   return APC_OK;
@@ -165,6 +169,9 @@ int apc_destroy(apc_ctx *c) {
 int apc_set_particles(apc_ctx *c, int n, const double *x, const double *y, const double *z,
                       const double *vx, const double *vy, const double *vz) {
   if (!c || n < 1 || !x || !y || !z) return fail(c, APC_ERR_ARG, "set_particles: bad arguments");
+  c->frozen = 0;  // new particles: a new topology
+  c->have_tr = 0;
+  c->have_cic_w = 0;
   c->N = n;
   APC_ALLOC(c, c->x, n); APC_ALLOC(c, c->y, n); APC_ALLOC(c, c->z, n);
   APC_ALLOC(c, c->vx, n); APC_ALLOC(c, c->vy, n); APC_ALLOC(c, c->vz, n);
@@ -302,11 +309,14 @@ int apc_step(apc_ctx *c, double c_dens, double c_pot, double c_field, double Cv,
   if (!c) return APC_ERR_ARG;
   using clk = std::chrono::steady_clock;
   auto t0 = clk::now();
-  int r = apc_octree_build(c);
-  if (r) return r;
-  int S = 0;
-  r = apc_select(c, c->prm.c, c->prm.order, c->prm.mindepth, c->prm.maxdepth, &S);
-  if (r) return r;
+  int r = 0;
+  if (!c->frozen) {  // a fixed topology (differentiation form) keeps tree and leaves
+    r = apc_octree_build(c);
+    if (r) return r;
+    int S = 0;
+    r = apc_select(c, c->prm.c, c->prm.order, c->prm.mindepth, c->prm.maxdepth, &S);
+    if (r) return r;
+  }
   APC_HIP(c, hipStreamSynchronize(c->s));
   auto t1 = clk::now();
   r = apc_solver_set_constants(c, c_dens, c_pot, c_field);

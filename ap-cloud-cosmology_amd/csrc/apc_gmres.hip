@@ -776,6 +776,19 @@ int solve_density_potential(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_op
   return 0;
 }
 
+int solve_transposed(apc_ctx *c, int which, const apc_ksp_opts *o, const double *b, double *out,
+                     int *its, int *reason) {
+  const int M = c->nc + c->nv;
+  Op<double> A = (which == 1)
+                     ? Op<double>{c->nbr.p, c->sco.p, c->d1.p, c->f1.p, c->dinv1.p, M, c->prm.lm}
+                     : Op<double>{c->nbr.p, c->sco.p, c->d2.p, c->f2.p, c->dinv2.p, M, c->prm.lm};
+  A.trp = c->tr_rp.p;
+  A.tpos = c->tr_pos.p;
+  double rn = 0.0;
+  int sw = 0;
+  return gmres(c, A, b, out, *o, false, its, reason, &rn, &sw);
+}
+
 int amg_info(apc_ctx *c, int *rows, long long *nnz, int cap) {
   if (!c->gm || !c->gm->amg) return 0;
   return amg_levels(c->gm->amg, rows, nnz, cap);

@@ -96,6 +96,17 @@ struct apc_ctx {
   apc::DBuf<int> nscratch_f;   // candidate fingerprints
   apc::DBuf<int> novf;         // nodes searched again with the full list bound
   apc::DBuf<double> nxyz;      // node centres packed 4 doubles per node
+  // differentiation form (apc_adjoint.hip): topology held fixed
+  int frozen = 0, have_tr = 0;
+  apc::DBuf<uint64_t> tr_key, tr_key2;
+  apc::DBuf<int> tr_pos, tr_pos2, tr_rp;   // column lists of the ELL matrix
+  apc::DBuf<double> lam, dfield, cic, adj_tmp, adj_tmp2, adj_vec, adj_vec2;
+  apc::DBuf<uint64_t> cic_key, cic_key2;   // CIC contributions sorted by cell
+  apc::DBuf<int> cic_v, cic_v2, cic_rng;
+  apc::DBuf<double> cic_w;                 // per-particle loss weights (device order)
+  int have_cic_w = 0;
+  double cic_wsum = 0.0;
+  std::vector<apc::DBuf<double>> ckpt;
   int nsearch_long = 0;
   int search_slots = 0;        // resident threads of the neighbour search
   apc::DBuf<int> status_dev;
@@ -168,6 +179,10 @@ int solve_density_potential(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_op
                             apc_solve_info *info);  // apc_gmres.hip
 void free_solve_state(apc_ctx *c);              // apc_gmres.hip
 int amg_info(apc_ctx *c, int *rows, long long *nnz, int cap);  // apc_gmres.hip
+// A^T out = b (which 1: density, 2: Poisson), b and out device M-vectors
+int solve_transposed(apc_ctx *c, int which, const apc_ksp_opts *o, const double *b, double *out,
+                     int *its, int *reason);  // apc_gmres.hip
+int build_transpose(apc_ctx *c);                // apc_adjoint.hip
 int interp_kick(apc_ctx *c, double Cv, double dt, double *maxdv);  // apc_mover.hip
 int drift(apc_ctx *c, double Cx, double dt, double *maxdx);        // apc_mover.hip
 

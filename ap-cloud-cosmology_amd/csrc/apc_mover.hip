@@ -15,13 +15,19 @@ __global__ void __launch_bounds__(256) k_interp_kick(
     const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
     double *__restrict__ vx, double *__restrict__ vy, double *__restrict__ vz,
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az, double Cv,
-    double dt, unsigned long long *__restrict__ maxbits) {
+    double dt, unsigned long long *__restrict__ maxbits, int wrapdx, double lx, double ly,
+    double lz) {
   __shared__ double ws[4];
   int s = blockIdx.x * 256 + threadIdx.x;
   double change = 0.0;
   if (s < N) {
     int i = owner[s];
     double dx = x[s] - cx[i], dy = y[s] - cy[i], dz = z[s] - cz[i];
+    if (wrapdx) {  // fixed topology on a periodic box: minimum image to the leaf
+      dx -= lx * rint(dx / lx);
+      dy -= ly * rint(dy / ly);
+      dz -= lz * rint(dz / lz);
+    }
     double gx = grad[i], gy = grad[(size_t)M + i], gz = grad[2 * (size_t)M + i];
     double hxx = grad[3 * (size_t)M + i], hyy = grad[4 * (size_t)M + i],
            hzz = grad[5 * (size_t)M + i];
@@ -95,11 +101,17 @@ int interp_kick(apc_ctx *c, double Cv, double dt, double *maxdv) {
   APC_HIP(c, hipMemsetAsync(c->red_u64.p, 0, sizeof(unsigned long long), c->s));
   {
     // per particle: owner 4, x 24, v 24 read + 24 written, a 24 written;
-    // per leaf: centre 24 + g,H 72 read
+    // per leaf: centre 24 + g,H 72 read.  The reference's offset is not
+    // wrapped (octree.cpp:3003-3031): its particles always sit in their
+    // leaf.  With the topology held fixed a particle can cross the periodic
+    // boundary away from its leaf, so there the minimum image is used.
+    const double *b = c->prm.bbox;
+    const int wrapdx = (c->frozen && c->prm.bc == 1) ? 1 : 0;
     KScope ks(c, "interp_kick", 100.0 * N + 96.0 * M);
     hipLaunchKernelGGL(k_interp_kick, dim3(nblocks(N, 256)), dim3(256), 0, c->s, N, M, c->owner.p,
                        c->cx.p, c->cy.p, c->cz.p, c->grad.p, c->x.p, c->y.p, c->z.p, c->vx.p,
-                       c->vy.p, c->vz.p, c->ax.p, c->ay.p, c->az.p, Cv, dt, c->red_u64.p);
+                       c->vy.p, c->vz.p, c->ax.p, c->ay.p, c->az.p, Cv, dt, c->red_u64.p, wrapdx,
+                       b[1] - b[0], b[3] - b[2], b[5] - b[4]);
   }
   APC_HIP(c, hipGetLastError());
   if (maxdv) return read_max(c, maxdv);
@@ -118,10 +130,14 @@ int drift(apc_ctx *c, double Cx, double dt, double *maxdx) {
                        b[5], c->prm.bc, c->red_u64.p);
   }
   APC_HIP(c, hipGetLastError());
-  // positions moved: the tree and everything built on it are stale
-  c->sorted = false;
-  c->have_comp = false;
-  c->have_nbr = c->have_sys = c->have_fields = 0;
+  // positions moved: the tree and everything built on it are stale, unless
+  // the topology is held fixed (then only the node fields are recomputed)
+  c->have_fields = 0;
+  if (!c->frozen) {
+    c->sorted = false;
+    c->have_comp = false;
+    c->have_nbr = c->have_sys = 0;
+  }
   if (maxdx) return read_max(c, maxdx);
   return 0;
 }

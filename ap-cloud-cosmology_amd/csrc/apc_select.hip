@@ -611,6 +611,7 @@ int select_tree(apc_ctx *c, double cc, int order, int mindepth, int maxdepth) {
   c->nv = nv;
   c->nb = nb;
   c->have_comp = true;
+  c->have_tr = 0;
   c->have_nbr = c->have_sys = c->have_fields = 0;
   return 0;
 }
@@ -621,6 +622,7 @@ using namespace apc;
 
 extern "C" int apc_select(apc_ctx *c, double cc, int order, int mindepth, int maxdepth, int *S_out) {
   if (!c) return APC_ERR_ARG;
+  if (c->frozen) return fail(c, APC_ERR_ARG, "select: the topology is frozen");
   int r = select_tree(c, cc, order, mindepth, maxdepth);
   if (r) return r;
   if (S_out) *S_out = c->nc + c->nv + c->nb;

@@ -9,11 +9,15 @@ namespace apc {
 // One GFD system in slot-major ELL form: row i is
 //   (A x)_i = d_i x_i + f_i * sum_j s_ij x_{nbr_ij}   over nbr_ij < M
 // (neighbours at or beyond M are boundary nodes, already in the RHS).
+// With trp set the operator is the transpose, applied by gathers through
+// the column lists of A: column j's entries are e in [trp[j], trp[j+1]),
+// each the ELL position tpos[e] = q*M + i of a_ij.
 template <class T>
 struct Op {
   const int *nbr;
   const T *s, *d, *f, *dinv;
   int M, lm;
+  const int *trp = nullptr, *tpos = nullptr;
 };
 
 // Device-resident state of one GMRES solve; kernels launched ahead of the
@@ -39,6 +43,15 @@ __device__ inline T row_offdiag_sum(const Op<T> &A, const T *__restrict__ x, int
 
 template <class T>
 __device__ inline double row_apply(const Op<T> &A, const T *__restrict__ x, int i) {
+  if (A.trp) {  // (A^T x)_i = d_i x_i + sum over rows r with a_ri of f_r s_ri x_r
+    T acc = 0;
+    for (int e = A.trp[i]; e < A.trp[i + 1]; ++e) {
+      const int p = A.tpos[e];
+      const int r = p % A.M;
+      acc += A.f[r] * A.s[p] * x[r];
+    }
+    return (double)(A.d[i] * x[i] + acc);
+  }
   return (double)(A.d[i] * x[i] + A.f[i] * row_offdiag_sum(A, x, i));
 }
 

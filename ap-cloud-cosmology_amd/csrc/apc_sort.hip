@@ -261,6 +261,7 @@ extern "C" int apc_octree_level(apc_ctx *c, int level, int *count, uint64_t *key
 
 extern "C" int apc_octree_build(apc_ctx *c) {
   if (!c) return APC_ERR_ARG;
+  if (c->frozen) return fail(c, APC_ERR_ARG, "octree_build: the topology is frozen");
   int r = sort_particles(c);
   if (r) return r;
   APC_HIP(c, hipGetLastError());

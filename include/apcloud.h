@@ -170,6 +170,48 @@ int apc_step(apc_ctx *ctx, double c_dens, double c_pot, double c_field, double C
              double dt, const apc_ksp_opts *rho, const apc_ksp_opts *phi, double *max_dv,
              double *max_dx, apc_solve_info *info, double *timing);
 
+/* ---- Differentiation form: adjoints with the topology held fixed ----
+ * No reference counterpart (the reference has no derivatives); used by
+ * apcloud.adjoint (torch.autograd.Function) and its finite-difference tests.
+ * After apc_freeze_topology(ctx, 1) the tree, leaves, neighbour lists, GFD
+ * systems and the particle-to-leaf map stay as built; apc_step then only
+ * solves (fields), kicks and drifts. */
+int apc_freeze_topology(apc_ctx *ctx, int on);
+/* Particle state in caller order into the frozen device order. */
+int apc_set_state(apc_ctx *ctx, const double *x, const double *y, const double *z,
+                  const double *vx, const double *vy, const double *vz);
+/* Save (restore = 0) or restore device-side copies of (x, v), slot 0..63. */
+int apc_checkpoint(apc_ctx *ctx, int slot, int restore);
+/* Loss = sum over a periodic n^3 grid of (CIC density / mean - 1)^2 of the
+ * current positions; seeds the adjoint state with lambda_x = dL/dx,
+ * lambda_v = 0.  The deposit is sorted by cell and summed with compensation,
+ * so the loss is reproducible bit for bit. */
+int apc_adjoint_seed_cic(apc_ctx *ctx, int n, double *loss);
+/* Per-particle weights of the CIC loss's density (caller order, >= 0, not
+ * all zero; the mean is then the weight sum); NULL restores equal weights. */
+int apc_adjoint_cic_weights(apc_ctx *ctx, const double *w);
+/* Adjoint state (lambda_x, lambda_v) in caller order. */
+int apc_adjoint_get(apc_ctx *ctx, double *lx, double *ly, double *lz, double *lvx, double *lvy,
+                    double *lvz);
+int apc_adjoint_set(apc_ctx *ctx, const double *lx, const double *ly, const double *lz,
+                    const double *lvx, const double *lvy, const double *lvz);
+/* Adjoint of the drift (mover.cpp:91-153): lambda_v += Cx dt lambda_x. */
+int apc_adjoint_drift(apc_ctx *ctx, double Cx, double dt);
+/* Adjoint of the kick with the interpolation (mover.cpp:51-89,
+ * octree.cpp:3003-3031) using the node fields of the last solve:
+ * lambda_x += Cv dt H lambda_v; acc != 0 also accumulates the leaf-field
+ * adjoints dg, dH (cleared first when clear != 0). */
+int apc_adjoint_kick(apc_ctx *ctx, double Cv, double dt, int acc, int clear);
+/* Accumulated leaf-field adjoints, layouts as apc_get_fields (M x 3, M x 6). */
+int apc_adjoint_fields(apc_ctx *ctx, double *dg, double *dH);
+/* Transpose of Differentiate (solver.cpp:295-362): dphi = J^T (dg, dH). */
+int apc_gradient_adjoint(apc_ctx *ctx, const double *dg, const double *dH, double *dphi);
+/* out = A^-T rhs, A the density (which = 1) or Poisson (which = 2) matrix of
+ * the last solve (solver.cpp:182-293), by FP64 Jacobi GMRES on the explicit
+ * transpose. */
+int apc_solve_adjoint(apc_ctx *ctx, int which, const apc_ksp_opts *opts, const double *rhs,
+                      double *out, int *its, int *reason);
+
 /* Per-kernel device time of the timed calls (HIP events on the context's
  * stream): average ms and algorithmic bytes per launch, launch counts; names
  * are NUL-separated.  Returns the number of kernels. */
