@@ -34,6 +34,8 @@ EXPORTS = [
     "apc_stream_copy", "apc_freeze_topology", "apc_set_state", "apc_checkpoint",
     "apc_adjoint_seed_cic", "apc_adjoint_cic_weights", "apc_adjoint_get", "apc_adjoint_set", "apc_adjoint_drift",
     "apc_adjoint_kick", "apc_adjoint_fields", "apc_gradient_adjoint", "apc_solve_adjoint",
+    "apc_batch_create", "apc_batch_destroy", "apc_batch_size", "apc_batch_box",
+    "apc_batch_last_error", "apc_batch_set_particles", "apc_batch_run",
 ]
 
 
@@ -106,6 +108,16 @@ def lib():
         L.apc_step.argtypes = [C.c_void_p] + [C.c_double] * 6 + [C.POINTER(KspOpts)] * 2 + \
             [_D, _D, C.POINTER(SolveInfo), _D]
         L.apc_kernel_times.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _D, _I, _D, C.c_int]
+        L.apc_batch_create.argtypes = [C.POINTER(Params), C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.apc_batch_destroy.argtypes = [C.c_void_p]
+        L.apc_batch_size.argtypes = [C.c_void_p]
+        L.apc_batch_box.argtypes = [C.c_void_p, C.c_int]
+        L.apc_batch_box.restype = C.c_void_p
+        L.apc_batch_last_error.argtypes = [C.c_void_p]
+        L.apc_batch_last_error.restype = C.c_char_p
+        L.apc_batch_set_particles.argtypes = [C.c_void_p, C.c_int, C.c_int] + [_D] * 6
+        L.apc_batch_run.argtypes = [C.c_void_p, C.c_int, _D, _D, C.POINTER(KspOpts),
+                                    C.POINTER(KspOpts), C.POINTER(SolveInfo), _I]
         L.apc_stream_copy.argtypes = [C.c_void_p, C.c_double, C.c_int, _D]
         L.apc_reset_kernel_times.argtypes = [C.c_void_p]
         L.apc_set_timing.argtypes = [C.c_void_p, C.c_int]
@@ -162,22 +174,77 @@ def device_count():
     return n.value
 
 
-class Context:
-    """One simulation box on the current GPU."""
+class Batch:
+    """Many independent boxes stepped together (apc_batch_run): one context per
+    box, driven by a pool of host threads; `boxes[i]` is box i's Context."""
 
-    def __init__(self, params):
+    def __init__(self, params, nbox, nthreads=0):
         self.L = lib()
         self.p = params
         h = C.c_void_p()
-        r = self.L.apc_create(C.byref(params), C.byref(h))
+        r = self.L.apc_batch_create(C.byref(params), nbox, nthreads, C.byref(h))
         if r != 0:
-            raise ApcError("apc_create failed: %s" % ERRORS.get(r, r))
+            raise ApcError("apc_batch_create failed: %s" % ERRORS.get(r, r))
         self.h = h
+        self.boxes = [Context(params, _handle=C.c_void_p(self.L.apc_batch_box(h, i)))
+                      for i in range(nbox)]
+
+    def close(self):
+        if getattr(self, "h", None):
+            for b in self.boxes:
+                b.close()
+            self.L.apc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_particles(self, i, x, y, z, vx, vy, vz):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (x, y, z, vx, vy, vz)]
+        r = self.L.apc_batch_set_particles(self.h, i, len(arrs[0]), *[_p(a) for a in arrs])
+        if r != 0:
+            raise ApcError("apc_batch_set_particles failed (%s): %s"
+                           % (ERRORS.get(r, r), self.L.apc_batch_last_error(self.h).decode()))
+        self.boxes[i].n = len(arrs[0])
+
+    def run(self, coef, c_dens, rho_opts, phi_opts=None):
+        """coef: (steps, 5) rows of (c_pot, c_field, Cv, Cx, dt); c_dens per
+        box.  Returns (last SolveInfo per box, status per box)."""
+        phi_opts = rho_opts if phi_opts is None else phi_opts
+        coef = np.ascontiguousarray(coef, np.float64).reshape(-1, 5)
+        c_dens = np.ascontiguousarray(c_dens, np.float64)
+        nbox = len(self.boxes)
+        assert c_dens.shape == (nbox,)
+        info = (SolveInfo * nbox)()
+        status = np.zeros(nbox, np.int32)
+        r = self.L.apc_batch_run(self.h, len(coef), _p(coef), _p(c_dens), C.byref(rho_opts),
+                                 C.byref(phi_opts), info, _p(status, _I))
+        if r != 0:
+            raise ApcError("apc_batch_run failed (%s): %s"
+                           % (ERRORS.get(r, r), self.L.apc_batch_last_error(self.h).decode()))
+        return list(info), status
+
+
+class Context:
+    """One simulation box on the current GPU."""
+
+    def __init__(self, params, _handle=None):
+        self.L = lib()
+        self.p = params
+        self.owned = _handle is None
+        if _handle is None:
+            h = C.c_void_p()
+            r = self.L.apc_create(C.byref(params), C.byref(h))
+            if r != 0:
+                raise ApcError("apc_create failed: %s" % ERRORS.get(r, r))
+            _handle = h
+        self.h = _handle
         self.n = 0
 
     def close(self):
         if getattr(self, "h", None):
-            self.L.apc_destroy(self.h)
+            if self.owned:
+                self.L.apc_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -141,3 +141,24 @@ def write_param(path, **tags):
         f.write("% written by apcloud.ic.write_param\n")
         for k, v in tags.items():
             f.write("%-28s %s\n" % (k, repr(v) if isinstance(v, float) else v))
+
+
+MULTIBOX_EDGES = (16, 24, 32, 48, 64, 96)
+
+
+def multibox_specs(nbox=48, seed=3, rms=0.01, edges=MULTIBOX_EDGES, index_range=(-2.5, -1.5)):
+    """Independent periodic boxes for the multi-box form: lattice side drawn
+    uniformly from `edges`, spectral index uniformly from `index_range`, and
+    a per-box seed, all from one generator seeded with `seed`.  Returns a list
+    of dicts (n_side, spectral_index, seed, rms); `multibox_box` realises one."""
+    rng = np.random.default_rng(seed)
+    sides = rng.choice(np.asarray(edges), size=nbox)
+    index = rng.uniform(index_range[0], index_range[1], size=nbox)
+    seeds = rng.integers(0, 2 ** 31 - 1, size=nbox)
+    return [dict(n_side=int(sides[b]), spectral_index=float(index[b]), seed=int(seeds[b]),
+                 rms=rms) for b in range(nbox)]
+
+
+def multibox_box(spec):
+    """x, y, z, vx, vy, vz of one multibox_specs entry (a Zel'dovich lattice)."""
+    return zeldovich(spec["n_side"], spec["seed"], spec["rms"], spec["spectral_index"])

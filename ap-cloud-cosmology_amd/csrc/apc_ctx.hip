@@ -269,9 +269,12 @@ int apc_get_fields(apc_ctx *c, double *rho, double *phi, double *g, double *H) {
   if (!c->have_fields) return fail(c, APC_ERR_ARG, "get_fields: no fields");
   const int M = c->nc + c->nv;
   std::vector<double> gr((size_t)9 * M);
-  if (rho && c->rho.p) APC_HIP(c, hipMemcpy(rho, c->rho.p, 8ull * M, hipMemcpyDeviceToHost));
-  if (phi) APC_HIP(c, hipMemcpy(phi, c->phi.p, 8ull * M, hipMemcpyDeviceToHost));
-  APC_HIP(c, hipMemcpy(gr.data(), c->grad.p, 8ull * 9 * M, hipMemcpyDeviceToHost));
+  // the context's stream is non-blocking: copies go on it, then wait
+  if (rho && c->rho.p)
+    APC_HIP(c, hipMemcpyAsync(rho, c->rho.p, 8ull * M, hipMemcpyDeviceToHost, c->s));
+  if (phi) APC_HIP(c, hipMemcpyAsync(phi, c->phi.p, 8ull * M, hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipMemcpyAsync(gr.data(), c->grad.p, 8ull * 9 * M, hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
   for (int i = 0; i < M; ++i) {
     if (g)
       for (int q = 0; q < 3; ++q) g[3 * (size_t)i + q] = gr[(size_t)q * M + i];
@@ -291,8 +294,8 @@ int apc_set_gradients(apc_ctx *c, const double *g, const double *H) {
     tg.release(); th.release();
     return fail(c, APC_ERR_NOMEM, "set_gradients");
   }
-  hipError_t e = hipMemcpy(tg.p, g, 8ull * 3 * M, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(th.p, H, 8ull * 6 * M, hipMemcpyHostToDevice);
+  hipError_t e = hipMemcpyAsync(tg.p, g, 8ull * 3 * M, hipMemcpyHostToDevice, c->s);
+  if (e == hipSuccess) e = hipMemcpyAsync(th.p, H, 8ull * 6 * M, hipMemcpyHostToDevice, c->s);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_set_grad, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, tg.p, th.p, c->grad.p);
     e = hipStreamSynchronize(c->s);

@@ -558,7 +558,9 @@ static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, 
   auto monitor_flush = [&](int its) {
     if (!o.monitor) return 0;
     std::vector<double> hist(its + 1);
-    APC_HIP(c, hipMemcpy(hist.data(), g.hist, sizeof(double) * (its + 1), hipMemcpyDeviceToHost));
+    APC_HIP(c, hipMemcpyAsync(hist.data(), g.hist, sizeof(double) * (its + 1),
+                              hipMemcpyDeviceToHost, c->s));
+    APC_HIP(c, hipStreamSynchronize(c->s));
     for (int k = printed; k <= its; ++k) printf("%3d KSP Residual norm %14.12e \n", k, hist[k]);
     printed = its + 1;
     return 0;

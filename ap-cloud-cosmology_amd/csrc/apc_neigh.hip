@@ -407,8 +407,9 @@ extern "C" int apc_neighbors(apc_ctx *c, int *nbr, double *dist) {
   const int M = c->nc + c->nv, lm = c->prm.lm;
   std::vector<int> tn((size_t)M * lm);
   std::vector<double> td((size_t)M * lm);
-  APC_HIP(c, hipMemcpy(tn.data(), c->nbr.p, 4ull * M * lm, hipMemcpyDeviceToHost));
-  APC_HIP(c, hipMemcpy(td.data(), c->dist.p, 8ull * M * lm, hipMemcpyDeviceToHost));
+  APC_HIP(c, hipMemcpyAsync(tn.data(), c->nbr.p, 4ull * M * lm, hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipMemcpyAsync(td.data(), c->dist.p, 8ull * M * lm, hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
   for (int i = 0; i < M; ++i)
     for (int q = 0; q < lm; ++q) {
       if (nbr) nbr[(size_t)i * lm + q] = tn[(size_t)q * M + i];

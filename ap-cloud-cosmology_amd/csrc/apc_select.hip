@@ -646,7 +646,8 @@ extern "C" int apc_get_comp_sizes(apc_ctx *c, int *nc, int *nv, int *nb, int *nn
     // n24first holds the boundary-count scan; the Neigh24 total is summed on the host
     std::vector<int> cnt(M);
     if (M > 0) {
-      APC_HIP(c, hipMemcpy(cnt.data(), c->n24cnt.p, 4ull * M, hipMemcpyDeviceToHost));
+      APC_HIP(c, hipMemcpyAsync(cnt.data(), c->n24cnt.p, 4ull * M, hipMemcpyDeviceToHost, c->s));
+      APC_HIP(c, hipStreamSynchronize(c->s));
     }
     long tot = 0;
     for (int q = 0; q < M; ++q) tot += cnt[q];

@@ -232,14 +232,15 @@ extern "C" int apc_build_system(apc_ctx *c, int *cols, double *a1, double *a2, d
   const int M = c->nc + c->nv, lm = c->prm.lm;
   std::vector<int> tn((size_t)M * lm);
   std::vector<double> ts((size_t)M * lm), h(M), vd1(M), vd2(M), vr(M), vp(M), vf2(M);
-  APC_HIP(c, hipMemcpy(tn.data(), c->nbr.p, 4ull * M * lm, hipMemcpyDeviceToHost));
-  APC_HIP(c, hipMemcpy(ts.data(), c->sco.p, 8ull * M * lm, hipMemcpyDeviceToHost));
-  APC_HIP(c, hipMemcpy(h.data(), c->ch.p, 8ull * M, hipMemcpyDeviceToHost));
-  APC_HIP(c, hipMemcpy(vd1.data(), c->d1.p, 8ull * M, hipMemcpyDeviceToHost));
-  APC_HIP(c, hipMemcpy(vd2.data(), c->d2.p, 8ull * M, hipMemcpyDeviceToHost));
-  APC_HIP(c, hipMemcpy(vr.data(), c->rhs1.p, 8ull * M, hipMemcpyDeviceToHost));
-  APC_HIP(c, hipMemcpy(vp.data(), c->pbnd.p, 8ull * M, hipMemcpyDeviceToHost));
-  APC_HIP(c, hipMemcpy(vf2.data(), c->f2.p, 8ull * M, hipMemcpyDeviceToHost));
+  APC_HIP(c, hipMemcpyAsync(tn.data(), c->nbr.p, 4ull * M * lm, hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipMemcpyAsync(ts.data(), c->sco.p, 8ull * M * lm, hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipMemcpyAsync(h.data(), c->ch.p, 8ull * M, hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipMemcpyAsync(vd1.data(), c->d1.p, 8ull * M, hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipMemcpyAsync(vd2.data(), c->d2.p, 8ull * M, hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipMemcpyAsync(vr.data(), c->rhs1.p, 8ull * M, hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipMemcpyAsync(vp.data(), c->pbnd.p, 8ull * M, hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipMemcpyAsync(vf2.data(), c->f2.p, 8ull * M, hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
   for (int i = 0; i < M; ++i) {
     for (int q = 0; q < lm; ++q) {
       int j = tn[(size_t)q * M + i];

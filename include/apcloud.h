@@ -170,6 +170,29 @@ int apc_step(apc_ctx *ctx, double c_dens, double c_pot, double c_field, double C
              double dt, const apc_ksp_opts *rho, const apc_ksp_opts *phi, double *max_dv,
              double *max_dx, apc_solve_info *info, double *timing);
 
+/* ---- Multi-box form: many independent boxes stepped in one call ----
+ * The reference steps one box per process (controller.cpp:106-208); a batch
+ * holds one context per box (apc_batch_box gives it for the per-box entry
+ * points above) and apc_batch_run steps every box through the same
+ * coefficient schedule on a pool of host threads, largest box first.  Each
+ * box's result equals stepping it alone with apc_step. */
+typedef struct apc_batch apc_batch;
+/* nthreads <= 0: min(nbox, 8). */
+int apc_batch_create(const apc_params *p, int nbox, int nthreads, apc_batch **out);
+int apc_batch_destroy(apc_batch *b);
+int apc_batch_size(const apc_batch *b);
+apc_ctx *apc_batch_box(apc_batch *b, int i);
+const char *apc_batch_last_error(const apc_batch *b);
+int apc_batch_set_particles(apc_batch *b, int i, int n, const double *x, const double *y,
+                            const double *z, const double *vx, const double *vy, const double *vz);
+/* coef: nsteps x 5 (c_pot, c_field, Cv, Cx, dt) per step, as apc_step;
+ * c_dens: per box.  info (nbox, may be NULL) gets each box's last solve,
+ * status (nbox, may be NULL) each box's return code; the first failing
+ * box's code is returned and apc_batch_last_error names it. */
+int apc_batch_run(apc_batch *b, int nsteps, const double *coef, const double *c_dens,
+                  const apc_ksp_opts *rho, const apc_ksp_opts *phi, apc_solve_info *info,
+                  int *status);
+
 /* ---- Differentiation form: adjoints with the topology held fixed ----
  * No reference counterpart (the reference has no derivatives); used by
  * apcloud.adjoint (torch.autograd.Function) and its finite-difference tests.
