@@ -28,7 +28,7 @@ EXPORTS = [
     "apc_set_particles", "apc_get_particles", "apc_octree_build", "apc_get_sorted",
     "apc_octree_level", "apc_select", "apc_get_comp_sizes", "apc_get_comp", "apc_neighbors",
     "apc_solver_set_neighbors", "apc_solver_set_constants", "apc_build_system",
-    "apc_solver_solve", "apc_solver_amg_levels", "apc_differentiate",
+    "apc_solver_solve", "apc_solver_amg_levels", "apc_solver_amg_level", "apc_differentiate",
     "apc_get_fields", "apc_set_gradients", "apc_interp_kick", "apc_drift", "apc_max_density",
     "apc_step", "apc_kernel_times", "apc_reset_kernel_times", "apc_set_timing", "apc_sync",
     "apc_stream_copy", "apc_freeze_topology", "apc_set_state", "apc_checkpoint",
@@ -411,6 +411,24 @@ class Context:
         self._chk(self.L.apc_solver_amg_levels(self.h, _p(rows, _I), nnz.ctypes.data_as(C.c_void_p),
                                                32, C.byref(n)), "apc_solver_amg_levels")
         return [(int(rows[k]), int(nnz[k])) for k in range(min(n.value, 32))]
+
+    def amg_level(self, level):
+        """(CSR scipy matrix of level >= 1 or None for level 0, aggregate map
+        to the next level or None for the coarsest)."""
+        import scipy.sparse as sp
+        levels = self.amg_levels()
+        n, nnz = levels[level]
+        agg = np.zeros(n, np.int32) if level + 1 < len(levels) else None
+        if level == 0:
+            self._chk(self.L.apc_solver_amg_level(self.h, 0, None, None, None,
+                                                  _p(agg, _I) if agg is not None else None),
+                      "apc_solver_amg_level")
+            return None, agg
+        rp, col, val = np.zeros(n + 1, np.int32), np.zeros(nnz, np.int32), np.zeros(nnz)
+        self._chk(self.L.apc_solver_amg_level(self.h, level, _p(rp, _I), _p(col, _I), _p(val),
+                                              _p(agg, _I) if agg is not None else None),
+                  "apc_solver_amg_level")
+        return sp.csr_matrix((val, col, rp), shape=(n, n)), agg
 
     # ---- differentiation form (topology held fixed) ----
     def freeze_topology(self, on=True):

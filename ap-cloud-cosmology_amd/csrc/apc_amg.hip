@@ -87,12 +87,15 @@ struct AmgHier {
   DBuf<uint64_t> key, key2;
   DBuf<int> idx, idx2, sa, sb, flag, pos, cnt;
   DBuf<double> ev, smax, diag;
+  DBuf<int> gcol;     // Galerkin entries grouped by coarse row: coarse column
+  DBuf<double> gval;  // and value
   ~AmgHier() {
     for (auto &g : graphs) g.release();
     for (auto *l : L) { l->release(); delete l; }
     gj.release(); cinv.release(); colk.release(); cinv32.release(); key.release();
     key2.release(); idx.release(); idx2.release(); sa.release(); sb.release(); flag.release();
     pos.release(); cnt.release(); ev.release(); smax.release(); diag.release();
+    gcol.release(); gval.release();
   }
 };
 
@@ -101,6 +104,21 @@ struct AmgHier {
 struct EllView {
   Op<double> A;
   __device__ int n() const { return A.M; }
+  // the diagonal, then every slot with an interior column, in slot order
+  __device__ int count(int i) const {
+    int c = 1;
+    for (int q = 0; q < A.lm; ++q) c += (A.nbr[(size_t)q * A.M + i] < A.M) ? 1 : 0;
+    return c;
+  }
+  template <class F>
+  __device__ void entries(int i, F f) const {
+    f(i, A.d[i]);
+    const double fi = A.f[i];
+    for (int q = 0; q < A.lm; ++q) {
+      int m = A.nbr[(size_t)q * A.M + i];
+      if (m < A.M) f(m, fi * A.s[(size_t)q * A.M + i]);
+    }
+  }
   __device__ double diag(int i) const { return A.d[i]; }
   template <class F>
   __device__ void offdiag(int i, F f) const {
@@ -115,6 +133,12 @@ struct EllView {
 struct CsrView {
   Csr<double> A;
   __device__ int n() const { return A.n; }
+  // every stored entry of row i in storage order (the Galerkin generation order)
+  __device__ int count(int i) const { return A.rp[i + 1] - A.rp[i]; }
+  template <class F>
+  __device__ void entries(int i, F f) const {
+    for (int e = A.rp[i]; e < A.rp[i + 1]; ++e) f(A.col[e], A.val[e]);
+  }
   __device__ double diag(int i) const {
     double d = 0.0;
     for (int e = A.rp[i]; e < A.rp[i + 1]; ++e)
@@ -293,6 +317,130 @@ __global__ void __launch_bounds__(256) k_amg_reduce(int n, int nc, const uint64_
   val[p] = s;
   if (J == I) diag[I] = s;
   if (k == 0 || (int)(key[k - 1] / (uint64_t)nc) != I) rp[I] = p;
+}
+
+// Galerkin product by coarse row, in three row-parallel steps and no global
+// sort:
+//  1. every fine row's entry count; an exclusive scan of the counts taken in
+//     member order (the aggregates' rows grouped, ascending within each)
+//     gives each fine row an offset such that the entries of a coarse row's
+//     members are contiguous, members in row order;
+//  2. each fine row writes its entries there as (coarse column, value), in
+//     storage order (coalesced reads of the fine matrix);
+//  3. one 64-lane block per coarse row reads its span and takes the distinct
+//     coarse columns in increasing order by repeated wave minima, summing
+//     each over its entries with a fixed reduction tree (deterministic).  A
+//     span of up to GAL_CAP entries is held in registers; a longer one is
+//     re-read from memory for every column.
+//  Pass 0 of step 3 counts each coarse row's entries, pass 1 writes them at rp[I].
+constexpr int GAL_CAP = 1024;
+constexpr int GAL_R = GAL_CAP / 64;
+
+__device__ inline int wave_min_int(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// entry counts in row order (coalesced reads of the fine matrix) ...
+template <class V>
+__global__ void k_gal_count(V v, int n, int *__restrict__ cnt_row) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) cnt_row[i] = v.count(i);
+}
+
+// ... and gathered into member order
+__global__ void k_gal_gather(int n, const int *__restrict__ ml, const int *__restrict__ cnt_row,
+                             int *__restrict__ cnt_ml) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < n) cnt_ml[k] = cnt_row[ml[k]];
+}
+
+// off_row[ml[k]] = off_ml[k]
+__global__ void k_gal_rowoff(int n, const int *__restrict__ ml, const int *__restrict__ off_ml,
+                             int *__restrict__ off_row) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < n) off_row[ml[k]] = off_ml[k];
+}
+
+template <class V>
+__global__ void __launch_bounds__(256) k_gal_emit(V v, int n, const int *__restrict__ agg,
+                                                  const int *__restrict__ off_row,
+                                                  int *__restrict__ ej, double *__restrict__ ea) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  int e = off_row[i];
+  v.entries(i, [&](int j, double a) {
+    ej[e] = agg[j];
+    ea[e] = a;
+    ++e;
+  });
+}
+
+__global__ void __launch_bounds__(64) k_gal_rows(int nc, const int *__restrict__ mp,
+                                                 const int *__restrict__ off_ml,
+                                                 const int *__restrict__ ej,
+                                                 const double *__restrict__ ea, int pass,
+                                                 int *__restrict__ ucnt, const int *__restrict__ rp,
+                                                 int *__restrict__ col, double *__restrict__ val,
+                                                 double *__restrict__ diag) {
+  const int lane = threadIdx.x;
+  for (int I = blockIdx.x; I < nc; I += gridDim.x) {
+    const int e0 = off_ml[mp[I]], e1 = off_ml[mp[I + 1]];
+    const int E = e1 - e0;
+    const int o = (pass == 1) ? rp[I] : 0;
+    int u = 0;
+    auto emit = [&](int cur, double sum) {
+      if (pass == 1 && lane == 0) {
+        col[o + u] = cur;
+        val[o + u] = sum;
+        if (cur == I) diag[I] = sum;
+      }
+      ++u;
+    };
+    if (E <= GAL_CAP) {
+      int J[GAL_R];
+      double A[GAL_R];
+      int mn = INT_MAX;
+#pragma unroll
+      for (int r = 0; r < GAL_R; ++r) {
+        const int e = lane + 64 * r;
+        J[r] = (e < E) ? ej[e0 + e] : INT_MAX;
+        A[r] = (e < E) ? ea[e0 + e] : 0.0;
+        mn = min(mn, J[r]);
+      }
+      int cur = wave_min_int(mn);
+      while (cur != INT_MAX) {
+        double sum = 0.0;
+        int nxt = INT_MAX;
+#pragma unroll
+        for (int r = 0; r < GAL_R; ++r) {
+          if (J[r] == cur) sum += A[r];
+          else if (J[r] > cur) nxt = min(nxt, J[r]);
+        }
+        sum = wave_sum(sum);
+        emit(cur, sum);
+        cur = wave_min_int(nxt);
+      }
+    } else {  // long span: stream it for every distinct column
+      int mn = INT_MAX;
+      for (int e = lane; e < E; e += 64) mn = min(mn, ej[e0 + e]);
+      int cur = wave_min_int(mn);
+      while (cur != INT_MAX) {
+        double sum = 0.0;
+        int nxt = INT_MAX;
+        for (int e = lane; e < E; e += 64) {
+          const int j = ej[e0 + e];
+          if (j == cur) sum += ea[e0 + e];
+          else if (j > cur) nxt = min(nxt, j);
+        }
+        sum = wave_sum(sum);
+        emit(cur, sum);
+        cur = wave_min_int(nxt);
+      }
+    }
+    if (pass == 0 && lane == 0) ucnt[I] = u;
+  }
 }
 
 __global__ void k_amg_dinv(int n, const double *__restrict__ diag, double *__restrict__ dinv) {
@@ -551,6 +699,60 @@ static int group_members(apc_ctx *c, AmgHier *h, AmgLevel *Lf, int n, int nc) {
   return 0;
 }
 
+// Galerkin product into level Lc (CSR) by coarse rows (k_gal_count,
+// k_gal_rowoff, k_gal_emit, k_gal_rows).
+template <class V>
+static int galerkin_rows(apc_ctx *c, AmgHier *h, const V &v, AmgLevel *Lf, int n, int nc,
+                         AmgLevel *Lc) {
+  if ((size_t)Lf->nnz >= (size_t)INT32_MAX)  // entry offsets are int
+    return fail(c, APC_ERR_UNSUPPORTED, "amg: %d stored entries on one level", Lf->nnz);
+  APC_ALLOC(c, h->flag, (size_t)n + 1);  // entry counts, then coarse-row counts
+  APC_ALLOC(c, h->pos, (size_t)n + 1);   // entry offsets in member order
+  APC_ALLOC(c, h->sa, (size_t)n + 1);    // entry offsets by fine row
+  APC_ALLOC(c, h->cnt, 4);
+  const unsigned g = nblocks(n, 256);
+  hipLaunchKernelGGL((k_gal_count<V>), dim3(g), dim3(256), 0, c->s, v, n, h->sa.p);
+  hipLaunchKernelGGL(k_gal_gather, dim3(g), dim3(256), 0, c->s, n, Lf->ml.p, h->sa.p, h->flag.p);
+  int r = scan_exclusive(c, h->flag.p, h->pos.p, n, h->cnt.p);
+  if (r) return r;
+  int ne = 0;
+  APC_HIP(c, hipMemcpyAsync(&ne, h->cnt.p, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  APC_HIP(c, hipMemcpyAsync(h->pos.p + n, h->cnt.p, sizeof(int), hipMemcpyDeviceToDevice, c->s));
+  APC_ALLOC(c, h->gcol, (size_t)ne + 1);
+  APC_ALLOC(c, h->gval, (size_t)ne + 1);
+  hipLaunchKernelGGL(k_gal_rowoff, dim3(g), dim3(256), 0, c->s, n, Lf->ml.p, h->pos.p, h->sa.p);
+  hipLaunchKernelGGL((k_gal_emit<V>), dim3(g), dim3(256), 0, c->s, v, n, Lf->agg.p, h->sa.p,
+                     h->gcol.p, h->gval.p);
+  const unsigned grid = (unsigned)std::min(nc, 1 << 16);
+  hipLaunchKernelGGL(k_gal_rows, dim3(grid), dim3(64), 0, c->s, nc, Lf->mp.p, h->pos.p, h->gcol.p,
+                     h->gval.p, 0, h->flag.p, (const int *)nullptr, (int *)nullptr,
+                     (double *)nullptr, (double *)nullptr);
+  APC_ALLOC(c, h->sb, (size_t)nc + 1);
+  r = scan_exclusive(c, h->flag.p, h->sb.p, nc, h->cnt.p + 1);
+  if (r) return r;
+  int nnz = 0;
+  APC_HIP(c, hipMemcpyAsync(&nnz, h->cnt.p + 1, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  Lc->n = nc;
+  Lc->nnz = nnz;
+  APC_ALLOC(c, Lc->rp, nc + 1);
+  APC_ALLOC(c, Lc->col, nnz + 1);
+  APC_ALLOC(c, Lc->val, nnz + 1);
+  APC_ALLOC(c, Lc->dinv, nc + 1);
+  APC_ALLOC(c, h->diag, nc + 1);
+  APC_HIP(c, hipMemcpyAsync(Lc->rp.p, h->sb.p, sizeof(int) * nc, hipMemcpyDeviceToDevice, c->s));
+  APC_HIP(c, hipMemcpyAsync(Lc->rp.p + nc, h->cnt.p + 1, sizeof(int), hipMemcpyDeviceToDevice, c->s));
+  APC_HIP(c, hipMemsetAsync(h->diag.p, 0, sizeof(double) * nc, c->s));
+  hipLaunchKernelGGL(k_gal_rows, dim3(grid), dim3(64), 0, c->s, nc, Lf->mp.p, h->pos.p, h->gcol.p,
+                     h->gval.p, 1, (int *)nullptr, (const int *)Lc->rp.p, Lc->col.p, Lc->val.p,
+                     h->diag.p);
+  hipLaunchKernelGGL(k_amg_dinv, dim3(nblocks(nc, 256)), dim3(256), 0, c->s, nc, h->diag.p,
+                     Lc->dinv.p);
+  APC_HIP(c, hipGetLastError());
+  return 0;
+}
+
 // Galerkin product into level Lc (CSR) from `ne` generated entries
 static int galerkin_reduce(apc_ctx *c, AmgHier *h, size_t ne, int nc, AmgLevel *Lc) {
   if (ne >= (size_t)INT32_MAX) return fail(c, APC_ERR_UNSUPPORTED, "amg: too many entries");
@@ -583,6 +785,26 @@ static int galerkin_reduce(apc_ctx *c, AmgHier *h, size_t ne, int nc, AmgLevel *
                      Lc->dinv.p);
   APC_HIP(c, hipStreamSynchronize(c->s));  // Lc->nnz copied from the host stack above
   return 0;
+}
+
+// Galerkin product by a global sort of every fine entry's (I, J) key (the
+// first formulation; kept behind AmgOptions::galerkin_sorted as a check).
+static int galerkin_sorted(apc_ctx *c, AmgHier *h, const Op<double> &A, AmgLevel *Lf, int l, int n,
+                           int nc, AmgLevel *Lc) {
+  const size_t ne = (l == 0) ? (size_t)n * (A.lm + 1) : (size_t)Lf->nnz;
+  APC_ALLOC(c, h->key, ne + 1);
+  APC_ALLOC(c, h->key2, ne + 1);
+  APC_ALLOC(c, h->idx, ne + 1);
+  APC_ALLOC(c, h->idx2, ne + 1);
+  APC_ALLOC(c, h->ev, ne + 1);
+  if (l == 0)
+    hipLaunchKernelGGL(k_amg_entries_ell, dim3(nblocks(n, 256)), dim3(256), 0, c->s, A, Lf->agg.p,
+                       nc, h->key.p, h->ev.p, h->idx.p);
+  else
+    hipLaunchKernelGGL(k_amg_entries_csr, dim3(nblocks(n, 256)), dim3(256), 0, c->s,
+                       Csr<double>{Lf->rp.p, Lf->col.p, Lf->val.p, n}, Lf->agg.p, nc, h->key.p,
+                       h->ev.p, h->idx.p);
+  return galerkin_reduce(c, h, ne, nc, Lc);
 }
 
 static int ensure_work(apc_ctx *c, AmgLevel *L, int n, bool fp32) {
@@ -675,7 +897,11 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
   AmgLevel *L0 = level(0);
   L0->n = A.M;
   L0->nnz = A.M * (A.lm + 1);
-  int r = ensure_work(c, L0, A.M, fp32);
+  int r;
+  {
+    KScope ks1(c, "amg_setup_work0");
+    r = ensure_work(c, L0, A.M, fp32);
+  }
   if (r) return r;
   int n = A.M;
   int k = 0;  // deepest level built by this setup
@@ -684,34 +910,38 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
     // stop at the dense size (but coarsen level 0 at least once) or the cap
     if ((l > 0 && n <= o.coarse_max) || l + 1 >= o.max_levels) break;
     int nc = 0;
-    if (l == 0) r = aggregate_level(c, h, EllView{A}, n, Lf, &nc);
-    else r = aggregate_level(c, h, CsrView{Csr<double>{Lf->rp.p, Lf->col.p, Lf->val.p, n}}, n, Lf, &nc);
+    {
+      KScope ks1(c, "amg_setup_aggregate");
+      if (l == 0) r = aggregate_level(c, h, EllView{A}, n, Lf, &nc);
+      else r = aggregate_level(c, h, CsrView{Csr<double>{Lf->rp.p, Lf->col.p, Lf->val.p, n}}, n, Lf, &nc);
+    }
     if (r) return r;
     if (nc < 1 || nc >= n) break;  // no coarsening: this level is the coarsest
-    r = group_members(c, h, Lf, n, nc);
+    {
+      KScope ks1(c, "amg_setup_group");
+      r = group_members(c, h, Lf, n, nc);
+    }
     if (r) return r;
-    // Galerkin entries
-    size_t ne = (l == 0) ? (size_t)n * (A.lm + 1) : (size_t)Lf->nnz;
-    APC_ALLOC(c, h->key, ne + 1);
-    APC_ALLOC(c, h->key2, ne + 1);
-    APC_ALLOC(c, h->idx, ne + 1);
-    APC_ALLOC(c, h->idx2, ne + 1);
-    APC_ALLOC(c, h->ev, ne + 1);
-    if (l == 0)
-      hipLaunchKernelGGL(k_amg_entries_ell, dim3(nblocks(n, 256)), dim3(256), 0, c->s, A, Lf->agg.p,
-                         nc, h->key.p, h->ev.p, h->idx.p);
-    else
-      hipLaunchKernelGGL(k_amg_entries_csr, dim3(nblocks(n, 256)), dim3(256), 0, c->s,
-                         Csr<double>{Lf->rp.p, Lf->col.p, Lf->val.p, n}, Lf->agg.p, nc, h->key.p,
-                         h->ev.p, h->idx.p);
     AmgLevel *Lc = level(l + 1);
-    r = galerkin_reduce(c, h, ne, nc, Lc);
+    {
+      KScope ks1(c, "amg_setup_galerkin");
+      if (o.galerkin_sorted) r = galerkin_sorted(c, h, A, Lf, l, n, nc, Lc);
+      else if (l == 0) r = galerkin_rows(c, h, EllView{A}, Lf, n, nc, Lc);
+      else r = galerkin_rows(c, h, CsrView{Csr<double>{Lf->rp.p, Lf->col.p, Lf->val.p, n}}, Lf, n, nc, Lc);
+    }
     if (r) return r;
-    r = ensure_work(c, Lc, nc, fp32);
+    {
+      KScope ks1(c, "amg_setup_work");
+      r = ensure_work(c, Lc, nc, fp32);
+    }
     if (r) return r;
-    r = build_sell(c, h, Lc);
+    {
+      KScope ks1(c, "amg_setup_sell");
+      r = build_sell(c, h, Lc);
+    }
     if (r) return r;
     if (fp32) {
+      KScope ks1(c, "amg_setup_convert");
       APC_ALLOC(c, Lc->sval32, Lc->sell_len + 1);
       APC_ALLOC(c, Lc->dinv32, nc + 1);
       hipLaunchKernelGGL((k_amg_convert<double, float>), dim3(1024), dim3(256), 0, c->s,
@@ -726,6 +956,7 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
   AmgLevel *Lc = h->L[k];
   h->nc = Lc->n;
   if (k > 0 && Lc->n <= 1024) {  // else (coarsening stalled) smoothing only
+    KScope ks1(c, "amg_setup_dense");
     r = dense_inverse(c, h, Lc, periodic);
     if (r) return r;
     h->dense = true;
@@ -760,6 +991,25 @@ const float *cinv_of<float>(AmgHier *h) { return h->cinv32.p; }
 
 // x = cycle(b) on level l; x and b are device vectors of the level's size,
 // tmp a scratch vector of that size (distinct from x and b).
+// timer names of the coarse levels, one per level so profiles split by level
+static const char *const kSweepName[] = {"amg_sweep_l0", "amg_sweep_l1", "amg_sweep_l2",
+                                         "amg_sweep_l3", "amg_sweep_l4", "amg_sweep_l5",
+                                         "amg_sweep_l6", "amg_sweep_l7", "amg_sweep_l8",
+                                         "amg_sweep_l9", "amg_sweep_l10", "amg_sweep_l11"};
+static const char *const kResidName[] = {"amg_resid_l0", "amg_resid_l1", "amg_resid_l2",
+                                         "amg_resid_l3", "amg_resid_l4", "amg_resid_l5",
+                                         "amg_resid_l6", "amg_resid_l7", "amg_resid_l8",
+                                         "amg_resid_l9", "amg_resid_l10", "amg_resid_l11"};
+static const char *const kRestrictName[] = {
+    "amg_restrict_l0", "amg_restrict_l1", "amg_restrict_l2", "amg_restrict_l3",
+    "amg_restrict_l4", "amg_restrict_l5", "amg_restrict_l6", "amg_restrict_l7",
+    "amg_restrict_l8", "amg_restrict_l9", "amg_restrict_l10", "amg_restrict_l11"};
+static const char *const kProlongName[] = {
+    "amg_prolong_l0", "amg_prolong_l1", "amg_prolong_l2", "amg_prolong_l3",
+    "amg_prolong_l4", "amg_prolong_l5", "amg_prolong_l6", "amg_prolong_l7",
+    "amg_prolong_l8", "amg_prolong_l9", "amg_prolong_l10", "amg_prolong_l11"};
+static_assert(sizeof(kSweepName) / sizeof(kSweepName[0]) >= 12, "one name per level");
+
 template <class T>
 static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *x, T *tmp,
                  const GState *st) {
@@ -783,7 +1033,7 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
       hipLaunchKernelGGL((k_cy_sweep<T, true>), dim3(g), dim3(256), 0, c->s, E, C, dinv, b, xin,
                          xout, o.omega, st);
     } else {
-      KScope ks(c, "amg_sweep_coarse");
+      KScope ks(c, kSweepName[l]);
       hipLaunchKernelGGL((k_cy_sweep<T, false>), dim3(g), dim3(256), 0, c->s, E, C, dinv, b, xin,
                          xout, o.omega, st);
     }
@@ -821,11 +1071,11 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
       KScope ks(c, f32 ? "amg_resid0_32" : "amg_resid0", (double)n * (lm0 * (4 + 2 * sT) + 5 * sT));
       hipLaunchKernelGGL((k_cy_resid<T, true>), dim3(g), dim3(256), 0, c->s, E, C, b, cur, r, st);
     } else {
-      KScope ks(c, "amg_resid_coarse");
+      KScope ks(c, kResidName[l]);
       hipLaunchKernelGGL((k_cy_resid<T, false>), dim3(g), dim3(256), 0, c->s, E, C, b, cur, r, st);
     }
     {
-      KScope ks(c, ell ? (f32 ? "amg_restrict0_32" : "amg_restrict0") : "amg_restrict",
+      KScope ks(c, ell ? (f32 ? "amg_restrict0_32" : "amg_restrict0") : kRestrictName[l],
                 ell ? (double)n * (sT + 4) + (double)Lc->n * (4 + sT) : 0.0);
       hipLaunchKernelGGL((k_cy_restrict<T>), dim3(nblocks(Lc->n, 256)), dim3(256), 0, c->s, Lc->n,
                          L->mp.p, L->ml.p, r, Pc.b, st);
@@ -833,7 +1083,7 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
     int rr = cycle<T>(c, h, A0, l + 1, Pc.b, Pc.x, Pc.t, st);
     if (rr) return rr;
     {
-      KScope ks(c, ell ? (f32 ? "amg_prolong0_32" : "amg_prolong0") : "amg_prolong",
+      KScope ks(c, ell ? (f32 ? "amg_prolong0_32" : "amg_prolong0") : kProlongName[l],
                 ell ? (double)n * (4 + 3 * sT) : 0.0);
       hipLaunchKernelGGL((k_cy_prolong<T>), dim3(g), dim3(256), 0, c->s, n, L->agg.p, Pc.x, cur,
                          o.alpha, st);
@@ -909,6 +1159,20 @@ template int amg_apply<float>(apc_ctx *, AmgHier *, const Op<float> &, const flo
                               const GState *);
 
 void amg_free(AmgHier *h) { delete h; }
+
+int amg_level_copy(apc_ctx *c, const AmgHier *h, int l, int *rp, int *col, double *val, int *agg) {
+  if (!h || l < 0 || l >= h->nlev) return fail(c, APC_ERR_ARG, "amg level %d: no such level", l);
+  const AmgLevel *L = h->L[l];
+  if (l > 0 && (rp || col || val)) {
+    if (rp) APC_HIP(c, hipMemcpyAsync(rp, L->rp.p, sizeof(int) * (L->n + 1), hipMemcpyDeviceToHost, c->s));
+    if (col) APC_HIP(c, hipMemcpyAsync(col, L->col.p, sizeof(int) * L->nnz, hipMemcpyDeviceToHost, c->s));
+    if (val) APC_HIP(c, hipMemcpyAsync(val, L->val.p, sizeof(double) * L->nnz, hipMemcpyDeviceToHost, c->s));
+  }
+  if (agg && l + 1 < h->nlev)
+    APC_HIP(c, hipMemcpyAsync(agg, L->agg.p, sizeof(int) * L->n, hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  return 0;
+}
 
 int amg_levels(const AmgHier *h, int *rows, long long *nnz, int cap) {
   if (!h) return 0;

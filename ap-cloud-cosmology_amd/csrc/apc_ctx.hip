@@ -251,6 +251,11 @@ int apc_solver_amg_levels(apc_ctx *c, int *rows, long long *nnz, int cap, int *n
   return 0;
 }
 
+int apc_solver_amg_level(apc_ctx *c, int level, int *rp, int *col, double *val, int *agg) {
+  if (!c) return APC_ERR_ARG;
+  return amg_level_info(c, level, rp, col, val, agg);
+}
+
 int apc_differentiate(apc_ctx *c, const double *phi) {
   if (!c || !phi) return APC_ERR_ARG;
   if (!c->have_comp) return fail(c, APC_ERR_ARG, "differentiate: no computational particles");

@@ -679,6 +679,7 @@ static AmgOptions amg_options() {
   a.gamma = env_int("APC_AMG_GAMMA", a.gamma);
   a.wlevels = env_int("APC_AMG_WLEVELS", a.wlevels);
   a.coarse_max = env_int("APC_AMG_COARSE", a.coarse_max);
+  a.galerkin_sorted = env_int("APC_AMG_GALERKIN_SORTED", 0);
   if (const char *v = getenv("APC_AMG_OMEGA")) a.omega = atof(v);
   if (const char *v = getenv("APC_AMG_ALPHA")) a.alpha = atof(v);
   if (const char *v = getenv("APC_AMG_THETA")) a.theta = atof(v);
@@ -789,6 +790,11 @@ int solve_transposed(apc_ctx *c, int which, const apc_ksp_opts *o, const double 
   double rn = 0.0;
   int sw = 0;
   return gmres(c, A, b, out, *o, false, its, reason, &rn, &sw);
+}
+
+int amg_level_info(apc_ctx *c, int l, int *rp, int *col, double *val, int *agg) {
+  if (!c->gm || !c->gm->amg) return fail(c, APC_ERR_ARG, "amg level: no multigrid built");
+  return amg_level_copy(c, c->gm->amg, l, rp, col, val, agg);
 }
 
 int amg_info(apc_ctx *c, int *rows, long long *nnz, int cap) {

@@ -179,6 +179,7 @@ int solve_density_potential(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_op
                             apc_solve_info *info);  // apc_gmres.hip
 void free_solve_state(apc_ctx *c);              // apc_gmres.hip
 int amg_info(apc_ctx *c, int *rows, long long *nnz, int cap);  // apc_gmres.hip
+int amg_level_info(apc_ctx *c, int l, int *rp, int *col, double *val, int *agg);
 // A^T out = b (which 1: density, 2: Poisson), b and out device M-vectors
 int solve_transposed(apc_ctx *c, int which, const apc_ksp_opts *o, const double *b, double *out,
                      int *its, int *reason);  // apc_gmres.hip

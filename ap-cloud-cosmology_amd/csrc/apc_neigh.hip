@@ -346,7 +346,9 @@ int search_neighbors(apc_ctx *c) {
     APC_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
     c->search_slots = std::max(1, nb) * std::max(1, ncu) * 256;
   }
-  const int slots = std::max(1, std::min(M, c->search_slots));
+  int slot_cap = c->search_slots;
+  if (const char *v = getenv("APC_NEIGH_SLOTS")) slot_cap = std::max(256, atoi(v));
+  const int slots = std::max(1, std::min(M, std::min(slot_cap, c->search_slots)));
   const int slots_full = std::max(1, std::min(M, 1 << 16));
   const size_t elems = std::max((size_t)fast_cap * slots, (size_t)cap_full * slots_full);
   APC_ALLOC(c, c->nscratch, 2 * elems);

@@ -70,6 +70,7 @@ struct AmgOptions {
   double theta = 0.0;     // strength threshold (fraction of the row's largest |a_ij|)
   int coarse_max = 1024;  // dense solve at or below this many rows
   int max_levels = 12;
+  int galerkin_sorted = 0;  // form every coarse operator by the global sort (tests)
 };
 
 // Build the hierarchy for A (FP64); `periodic` shifts the near-null constant
@@ -84,5 +85,8 @@ template <class T>
 int amg_apply(apc_ctx *c, AmgHier *h, const Op<T> &A, const T *in, T *out, const GState *st);
 void amg_free(AmgHier *h);
 int amg_levels(const AmgHier *h, int *rows, long long *nnz, int cap);
+// Level l >= 1 as CSR (rp n+1, col/val nnz) and the aggregate map of level l
+// (n entries, not for the coarsest); any pointer may be null.
+int amg_level_copy(apc_ctx *c, const AmgHier *h, int l, int *rp, int *col, double *val, int *agg);
 
 }  // namespace apc

@@ -146,6 +146,11 @@ int apc_solver_solve(apc_ctx *ctx, const apc_ksp_opts *rho, const apc_ksp_opts *
  * matrix).  *nlev = 0 if no multigrid was built.  No reference counterpart
  * (PETSc's -ksp_view prints the same for gamg). */
 int apc_solver_amg_levels(apc_ctx *ctx, int *rows, long long *nnz, int cap, int *nlev);
+/* Level `level` >= 1 of that multigrid as CSR (rp: rows + 1, col and val:
+ * nnz entries, sizes from apc_solver_amg_levels) and the aggregate map from
+ * level `level` to the next (rows entries; level 0 included, the coarsest
+ * excluded).  Any pointer may be NULL.  For inspection and tests. */
+int apc_solver_amg_level(apc_ctx *ctx, int level, int *rp, int *col, double *val, int *agg);
 /* Solver::Differentiate alone on a caller-supplied potential (M values). */
 int apc_differentiate(apc_ctx *ctx, const double *phi);
 /* Node fields after a solve: rho, phi (M), g (M x 3), H (M x 6: xx yy zz xy xz yz). */

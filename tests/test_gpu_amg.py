@@ -85,3 +85,41 @@ def test_amg_rejects_unknown_pc_type():
     o.pc_type = 7
     with pytest.raises(RuntimeError):
         g.solve(o)
+
+
+@pytest.mark.parametrize("bc", [1, 0])
+@pytest.mark.parametrize("sorted_path", [0, 1])
+def test_coarse_operators_are_galerkin_products(bc, sorted_path, monkeypatch):
+    """Every coarse operator equals P^T A P of the level above it, with P the
+    piecewise-constant prolongation of the level's aggregate map (scipy), for
+    the coarse-row product and the global-sort product alike."""
+    import scipy.sparse as sp
+
+    if sorted_path:
+        monkeypatch.setenv("APC_AMG_GALERKIN_SORTED", "1")
+    g, _, gc, _ = _solve_pair("plummer" if bc else "ball", 30000, bc, 24)
+    info = g.solve(apcloud.ksp(rtol=1e-10, restart=60, max_it=2000, pc="amg"))
+    assert info.reason_phi > 0
+    sysm = g.build_system()
+    M = gc["nc"] + gc["nv"]
+    cols, vals = sysm["cols"], sysm["a2"]
+    rows = np.repeat(np.arange(M), cols.shape[1])
+    m = cols.ravel() < M
+    prev = (sp.csr_matrix((vals.ravel()[m], (rows[m], cols.ravel()[m])), shape=(M, M))
+            + sp.diags(sysm["d2"])).tocsr()
+    levels = g.amg_levels()
+    assert len(levels) >= 3, levels
+    for l in range(1, len(levels)):
+        _, agg = g.amg_level(l - 1)
+        Ac, _ = g.amg_level(l)
+        n, nc = prev.shape[0], Ac.shape[0]
+        assert agg.min() == 0 and agg.max() == nc - 1
+        P = sp.csr_matrix((np.ones(n), (np.arange(n), agg)), shape=(n, nc))
+        ref = (P.T @ prev @ P).tocsr()
+        err = abs(Ac - ref).max() / abs(ref).max()
+        assert err <= 1e-13, (l, err)
+        # rows sorted by column, no duplicate columns
+        for r in range(0, nc, max(1, nc // 50)):
+            cs = Ac.indices[Ac.indptr[r]:Ac.indptr[r + 1]]
+            assert np.all(np.diff(cs) > 0)
+        prev = Ac
