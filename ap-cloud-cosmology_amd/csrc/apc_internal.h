@@ -109,6 +109,7 @@ struct apc_ctx {
   std::vector<apc::DBuf<double>> ckpt;
   int nsearch_long = 0;
   int search_slots = 0;        // resident threads of the neighbour search
+  int search_waves = 0;        // resident waves of the wave-per-node search
   apc::DBuf<int> status_dev;
   apc::SolveState *gm = nullptr;
   // reductions

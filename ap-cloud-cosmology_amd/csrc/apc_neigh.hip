@@ -305,6 +305,394 @@ __global__ void __launch_bounds__(256) k_search(int M, int S, int nrows, const i
   }
 }
 
+// ---------------------------------------------------------------- one wave per node
+// The same search with the 64 lanes of a wave working on one node, its lists
+// in LDS:
+//  * expansion: a round's candidates (snapshot entry, Neigh24 slot) are
+//    flattened in the reference's order and taken 64 at a time; the lanes
+//    decide together which join the list (not the node itself, not listed —
+//    an LDS hash set — and not equal to an earlier lane's), appending in lane
+//    order, so the list order is the reference's; errors are raised at the
+//    first candidate (in that order) that hits one;
+//  * sorts: the reference's quicksort, with every partition computed by ranks
+//    instead of the two scans.  For a range [lo, hi] with pivot p = WD[lo], let
+//    g_1 < g_2 < ... be the positions in (lo, hi] holding a value > p and
+//    l_1 > l_2 > ... the positions holding a value <= p, lo last.  The scans
+//    swap g_k with l_k for every k with g_k < l_k (a prefix, K swaps), and the
+//    pivot then lands at max(l_{K+1}, g_K) (l_1 if K = 0).  Disjoint ranges are
+//    independent, so they are taken from a stack in any order.
+//  Nodes whose list would exceed WCAP entries are queued for the per-thread
+//  search with the full bound.
+constexpr int WCAP = 256;   // list entries held in LDS (< 65536: ranges pack in 16 bits)
+constexpr int WHASH = 512;  // membership hash slots (power of two, >= 2 WCAP)
+
+struct WaveLds {
+  int mo[WCAP];       // list: index | octant << 29 (insertion order); then WM (sort payload)
+  double d2[WCAP];    // list: squared distance; then WD (sort keys)
+  int snap[WCAP];     // snapshot in octant order; then the partition's g positions
+  int lasc[WCAP];     // the partition's <= positions, ascending
+  int stk[WCAP];      // pending sort ranges, lo | hi << 16
+  int ht[WHASH];      // membership hash of the list's indices (-1 = empty)
+  double pen[WCAP];   // 1e4^k by repeated multiplication, k = 0..WCAP-1
+};
+
+__device__ inline uint64_t lanes_below() {
+  const int lane = threadIdx.x & 63;
+  return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+// exclusive prefix sum over the 64 lanes; *total gets the sum
+__device__ inline int wave_excl_scan_i(int v, int *total) {
+  const int lane = threadIdx.x & 63;
+  int inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  *total = __shfl(inc, 63, 64);
+  return inc - v;
+}
+
+__device__ inline int hslot(uint32_t m) { return (int)((m * 0x9E3779B1u) >> 23) & (WHASH - 1); }
+
+__device__ inline bool ht_has(const int *ht, int m) {
+  for (int h = hslot((uint32_t)m);; h = (h + 1) & (WHASH - 1)) {
+    const int v = ht[h];
+    if (v == m) return true;
+    if (v < 0) return false;
+  }
+}
+
+// distinct keys per call (callers remove duplicates first); at most WCAP keys
+// in a table of 2 WCAP slots, so a free slot always exists
+__device__ inline void ht_put(int *ht, int m) {
+  for (int h = hslot((uint32_t)m);; h = (h + 1) & (WHASH - 1)) {
+    const int old = atomicCAS(&ht[h], -1, m);
+    if (old < 0 || old == m) return;
+  }
+}
+
+// wrapped offset to node m: octant in the rotated frame and squared distance
+__device__ inline int place_w(const Frame &F, const double4 *__restrict__ xyz, double x0, double y0,
+                              double z0, int m, double *dd) {
+  const double4 pm = xyz[m];
+  double ex = pwrap(pm.x - x0, F.len[0], F.bc);
+  double ey = pwrap(pm.y - y0, F.len[1], F.bc);
+  double ez = pwrap(pm.z - z0, F.len[2], F.bc);
+  double xt = F.A[0] * ex + F.A[1] * ey + F.A[2] * ez;
+  double yt = F.A[3] * ex + F.A[4] * ey + F.A[5] * ez;
+  double zt = F.A[6] * ex + F.A[7] * ey + F.A[8] * ez;
+  *dd = ex * ex + ey * ey + ez * ez;
+  return (xt > 0) + 2 * (yt > 0) + 4 * (zt > 0);
+}
+
+// the reference's quicksort of (WD, WM)[lo0..hi0], partitions by ranks
+__device__ void wave_qsort(WaveLds &L, int lo0, int hi0) {
+  const int lane = threadIdx.x & 63;
+  double *WD = L.d2;
+  int *WM = L.mo;
+  int *gp = L.snap, *la = L.lasc;
+  if (hi0 <= lo0) return;
+  if (lane == 0) L.stk[0] = lo0 | (hi0 << 16);
+  int top = 1;
+  __syncthreads();
+  while (top > 0) {
+    --top;
+    const int packed = L.stk[top];
+    const int lo = packed & 0xffff, hi = packed >> 16;
+    const double pivot = WD[lo];
+    int nG = 0, nL = 0;
+    for (int base = lo + 1; base <= hi; base += 64) {
+      const int p = base + lane;
+      const bool valid = p <= hi;
+      const bool gt = valid && (WD[valid ? p : lo] > pivot);
+      const bool le = valid && !gt;
+      const uint64_t mG = __ballot(gt), mL = __ballot(le);
+      const uint64_t lt = lanes_below();
+      if (gt) gp[nG + __popcll(mG & lt)] = p;
+      if (le) la[nL + __popcll(mL & lt)] = p;
+      nG += __popcll(mG);
+      nL += __popcll(mL);
+    }
+    __syncthreads();
+    const int kmax = min(nG, nL);
+    int K = 0;
+    for (int base = 0; base < kmax; base += 64) {
+      const int k = base + lane + 1;
+      const bool c = (k <= kmax) && (gp[k <= kmax ? k - 1 : 0] < la[k <= kmax ? nL - k : 0]);
+      K += __popcll(__ballot(c));
+    }
+    const int lK1 = (K + 1 <= nL) ? la[nL - K - 1] : lo;
+    const int jstar = (K > 0) ? max(lK1, gp[K - 1]) : lK1;
+    for (int base = 0; base < K; base += 64) {
+      const int k = base + lane + 1;
+      if (k <= K) {
+        const int a = gp[k - 1], b = la[nL - k];
+        const double wa = WD[a], wb = WD[b];
+        const int ma = WM[a], mb = WM[b];
+        WD[a] = wb;
+        WD[b] = wa;
+        WM[a] = mb;
+        WM[b] = ma;
+      }
+    }
+    __syncthreads();
+    if (lane == 0 && jstar != lo) {
+      const double t = WD[lo];
+      WD[lo] = WD[jstar];
+      WD[jstar] = t;
+      const int tm = WM[lo];
+      WM[lo] = WM[jstar];
+      WM[jstar] = tm;
+    }
+    const bool left = jstar - 1 > lo, right = hi > jstar + 1;
+    if (lane == 0) {
+      int t = top;
+      if (left) L.stk[t++] = lo | ((jstar - 1) << 16);
+      if (right) L.stk[t++] = (jstar + 1) | (hi << 16);
+    }
+    top += (left ? 1 : 0) + (right ? 1 : 0);
+    __syncthreads();
+  }
+}
+
+// per-octant rank of every list entry and its position in octant order
+// (octants ascending, insertion order within an octant); calls
+// f(r, e, m, o, rank, pos) for entry e = 64 r + lane
+template <class F>
+__device__ inline void octant_order(const WaveLds &L, int l, const int (&off)[8], F f) {
+  const int lane = threadIdx.x & 63;
+  int run[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int r = 0; r < WCAP / 64; ++r) {
+    if (64 * r >= l) break;
+    const int e = 64 * r + lane;
+    const bool valid = e < l;
+    const uint32_t v = valid ? (uint32_t)L.mo[e] : 0u;
+    const int o = valid ? (int)(v >> 29) : 8;
+    int rank = 0, pos = 0;
+    const uint64_t lt = lanes_below();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const uint64_t mk = __ballot(o == q);
+      if (o == q) {
+        rank = run[q] + __popcll(mk & lt);
+        pos = off[q] + rank;
+      }
+      run[q] += __popcll(mk);
+    }
+    if (valid) f(r, e, (int)(v & 0x1fffffffu), o, rank, pos);
+  }
+}
+
+__global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int cap, Frame F,
+                                                    const double4 *__restrict__ xyz,
+                                                    const int *__restrict__ n24,
+                                                    const int *__restrict__ n24cnt,
+                                                    int *__restrict__ nbr, double *__restrict__ dist,
+                                                    int *__restrict__ status, int *__restrict__ ovf) {
+  __shared__ WaveLds L;
+  const int lane = threadIdx.x;
+  if (lane == 0) {
+    double p = 1.0;
+    for (int k = 0; k < WCAP; ++k) {
+      L.pen[k] = p;
+      p = p * 10000.0;
+    }
+  }
+  __syncthreads();
+  const int nq = 10 * lm;
+  for (int i0 = blockIdx.x; i0 < M; i0 += gridDim.x) {
+    for (int h = lane; h < WHASH; h += 64) L.ht[h] = -1;
+    const double4 p0 = xyz[i0];
+    const double x0 = p0.x, y0 = p0.y, z0 = p0.z;
+    int nn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    __syncthreads();
+    // the node's own Neigh24, all of it (solver.cpp:481-492)
+    int l;
+    {
+      const int cnt = n24cnt[i0];
+      int m = -1, o = 8;
+      if (lane < cnt) {
+        m = n24[(size_t)i0 * 24 + lane];
+        double dd;
+        o = place_w(F, xyz, x0, y0, z0, m, &dd);
+        L.mo[lane] = m | (o << 29);
+        L.d2[lane] = dd;
+      }
+      // duplicates (if any) share a hash slot: insert the first of each
+      bool first = lane < cnt;
+      for (int t = 0; t < 24; ++t) {
+        const int mt = __shfl(m, t, 64);
+        if (t < lane && mt == m) first = false;
+      }
+      if (first) ht_put(L.ht, m);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) nn[q] += __popcll(__ballot(o == q));
+      l = cnt;
+    }
+    __syncthreads();
+    // expansion through the neighbours' Neigh24 until >= lm (solver.cpp:495-551)
+    int ite = 0, err = 0;
+    bool overflow = false;
+    while (l < lm) {
+      int off[8];
+      off[0] = 0;
+#pragma unroll
+      for (int q = 1; q < 8; ++q) off[q] = off[q - 1] + nn[q - 1];
+      octant_order(L, l, off, [&](int, int, int m, int, int, int pos) { L.snap[pos] = m; });
+      __syncthreads();
+      // the round's candidates, (snapshot entry a, Neigh24 slot q) in the
+      // reference's order, flattened: coff[a] = first candidate of entry a
+      const int lsnap = l;
+      int *coff = L.lasc;
+      int T = 0;
+      for (int base = 0; base < lsnap; base += 64) {
+        const int a = base + lane;
+        int cnt = 0;
+        if (a < lsnap) {
+          const int src = L.snap[a];
+          cnt = (src < M) ? n24cnt[src] : 0;
+        }
+        int tot = 0;
+        const int ex = wave_excl_scan_i(cnt, &tot);
+        if (a < lsnap) coff[a] = T + ex;
+        T += tot;
+      }
+      if (lane == 0) coff[lsnap] = T;
+      __syncthreads();
+      for (int tb = 0; tb < T; tb += 64) {
+        const int t = tb + lane;
+        const bool live = t < T;
+        int m = -1;
+        if (live) {
+          int lo = 0, hi = lsnap;  // last entry a with coff[a] <= t
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (coff[mid] <= t) lo = mid;
+            else hi = mid - 1;
+          }
+          m = n24[(size_t)L.snap[lo] * 24 + (t - coff[lo])];
+        }
+        const bool invalid = live && m >= S;
+        bool fresh = live && !invalid && m != i0 && !ht_has(L.ht, m);
+        for (int u = 0; u < 64; ++u) {  // an earlier candidate of this chunk with the same index
+          const int mu = __shfl(m, u, 64);
+          if (u < lane && mu == m) fresh = false;
+        }
+        int o = 8;
+        double dd = 0.0;
+        if (fresh) o = place_w(F, xyz, x0, y0, z0, m, &dd);
+        const uint64_t lt = lanes_below();
+        const uint64_t mf = __ballot(fresh);
+        const int rank = __popcll(mf & lt);  // position among this chunk's new entries
+        int before = 0;                      // same-octant new entries before it
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const uint64_t mk = __ballot(fresh && o == q);
+          if (o == q) before = nn[q] + __popcll(mk & lt);
+        }
+        // the reference's checks in its order: invalid index, octant full, list full
+        int code = 0;
+        if (invalid) code = APC_ERR_NEIGH_INVALID;
+        else if (fresh && before >= nq) code = APC_ERR_NEIGH_MANY;
+        else if (fresh && l + rank >= cap) code = 1;  // overflow: longer list needed
+        const uint64_t me = __ballot(code != 0);
+        if (me) {
+          const int fl = __ffsll((unsigned long long)me) - 1;
+          const int c0 = __shfl(code, fl, 64);
+          if (c0 == 1) overflow = true;
+          else err = c0;
+          break;
+        }
+        if (fresh) {
+          L.mo[l + rank] = m | (o << 29);
+          L.d2[l + rank] = dd;
+          ht_put(L.ht, m);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) nn[q] += __popcll(__ballot(fresh && o == q));
+        l += __popcll(mf);
+        __syncthreads();
+      }
+      if (err || overflow) break;
+      ite++;
+      if (ite > 5) {
+        err = APC_ERR_NEIGH_FEW;
+        break;
+      }
+    }
+    if (err) {
+      if (lane == 0) status[0] = err;
+      return;
+    }
+    if (overflow) {
+      if (lane == 0) ovf[atomicAdd(&status[2], 1)] = i0;
+      __syncthreads();
+      continue;
+    }
+    // the reference sorts octant o's range but reads octant 0's slots
+    // (solver.cpp:557-561): ranges that reach slot nq are not supported
+    {
+      int ko = nn[0];
+      bool bad = false;
+#pragma unroll
+      for (int o = 1; o < 8; ++o) {
+        if (nn[o] >= 2 && ko + nn[o] - 1 >= nq) bad = true;
+        ko += nn[o];
+      }
+      if (bad) {
+        if (lane == 0) {
+          status[0] = APC_ERR_UNSUPPORTED;
+          status[1] = i0;
+        }
+        return;
+      }
+    }
+    // keys: octant 0 first (raw distances), then octants 1..7, each in
+    // insertion order with 1e4^rank; built in registers, then written over
+    // the list storage
+    {
+      int off[8];
+      off[0] = 0;
+#pragma unroll
+      for (int q = 1; q < 8; ++q) off[q] = off[q - 1] + nn[q - 1];
+      double key[WCAP / 64];
+      int km[WCAP / 64], kp[WCAP / 64];
+#pragma unroll
+      for (int r = 0; r < WCAP / 64; ++r) kp[r] = -1;
+      octant_order(L, l, off, [&](int r, int e, int m, int o, int rank, int pos) {
+        const double d = L.d2[e];
+        key[r] = (o != 0) ? L.pen[rank] * d : d;
+        km[r] = m;
+        kp[r] = pos;
+      });
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < WCAP / 64; ++r)
+        if (kp[r] >= 0) {
+          L.d2[kp[r]] = key[r];
+          L.mo[kp[r]] = km[r];
+        }
+      __syncthreads();
+    }
+    // octant 0 quicksorted on the squared distance, then penalised
+    if (nn[0] > 1) wave_qsort(L, 0, nn[0] - 1);
+    for (int q = lane; q < nn[0]; q += 64) L.d2[q] = L.pen[q] * L.d2[q];
+    __syncthreads();
+    wave_qsort(L, 0, l - 1);
+    for (int q = lane; q < lm; q += 64) {
+      const int m = L.mo[q];
+      double dd;
+      (void)place_w(F, xyz, x0, y0, z0, m, &dd);
+      nbr[(size_t)q * M + i0] = m;
+      dist[(size_t)q * M + i0] = sqrt(dd);
+    }
+    __syncthreads();
+  }
+}
+
 int search_neighbors(apc_ctx *c) {
   if (!c->have_comp) return fail(c, APC_ERR_ARG, "neighbours: no computational particles");
   const int M = c->nc + c->nv, S = M + c->nb, lm = c->prm.lm;
@@ -340,17 +728,28 @@ int search_neighbors(apc_ctx *c) {
   int fast_cap = FAST_CAP;
   if (const char *v = getenv("APC_NEIGH_FAST_CAP"))
     fast_cap = std::min(FAST_CAP, std::max(32, atoi(v))) & ~7;
+  // first pass: one wave per node (k_search_wave) when the list bound fits
+  // its LDS lists, else one thread per node with lists in device scratch
+  const char *wenv = getenv("APC_NEIGH_WAVE");
+  const bool wave = lm <= WCAP / 2 && !(wenv && atoi(wenv) == 0);
+  int ncu = 0;
+  APC_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
   if (c->search_slots == 0) {  // resident threads of k_search on this device
-    int nb = 0, ncu = 0;
+    int nb = 0;
     APC_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_search, 256, 0));
-    APC_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
     c->search_slots = std::max(1, nb) * std::max(1, ncu) * 256;
+  }
+  if (c->search_waves == 0) {  // resident waves of k_search_wave
+    int nb = 0;
+    APC_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_search_wave, 64, 0));
+    c->search_waves = std::max(1, nb) * std::max(1, ncu);
   }
   int slot_cap = c->search_slots;
   if (const char *v = getenv("APC_NEIGH_SLOTS")) slot_cap = std::max(256, atoi(v));
   const int slots = std::max(1, std::min(M, std::min(slot_cap, c->search_slots)));
   const int slots_full = std::max(1, std::min(M, 1 << 16));
-  const size_t elems = std::max((size_t)fast_cap * slots, (size_t)cap_full * slots_full);
+  const size_t elems = wave ? (size_t)cap_full * slots_full
+                            : std::max((size_t)fast_cap * slots, (size_t)cap_full * slots_full);
   APC_ALLOC(c, c->nscratch, 2 * elems);
   APC_ALLOC(c, c->nscratch_i, 2 * elems);
   APC_ALLOC(c, c->nscratch_f, elems / 2 + 4);  // 16-bit fingerprints as int
@@ -364,7 +763,13 @@ int search_neighbors(apc_ctx *c) {
   double4 *xyz = (double4 *)c->nxyz.p;
   hipLaunchKernelGGL(k_pack_xyz, dim3(nblocks(S, 256)), dim3(256), 0, c->s, S, c->cx.p, c->cy.p,
                      c->cz.p, xyz);
-  {
+  if (wave) {
+    KScope ks(c, "neighbor_search");
+    const int grid = std::max(1, std::min(M, c->search_waves));
+    hipLaunchKernelGGL(k_search_wave, dim3(grid), dim3(64), 0, c->s, M, S, lm,
+                       std::min(fast_cap, WCAP), F, (const double4 *)xyz, c->n24.p, c->n24cnt.p,
+                       c->nbr.p, c->dist.p, c->status_dev.p, c->novf.p);
+  } else {
     KScope ks(c, "neighbor_search");
     hipLaunchKernelGGL(k_search, dim3(nblocks(slots, 256)), dim3(256), 0, c->s, M, S, M,
                        (const int *)nullptr, lm, fast_cap, slots, F, (const double4 *)xyz, c->n24.p, c->n24cnt.p, mo, d2, wd, wm, fpw, c->nbr.p, c->dist.p,
