@@ -110,6 +110,7 @@ struct apc_ctx {
   int nsearch_long = 0;
   int search_slots = 0;        // resident threads of the neighbour search
   int search_waves = 0;        // resident waves of the wave-per-node search
+  int search_pen = 0;          // its penalty table is on the device
   apc::DBuf<int> status_dev;
   apc::SolveState *gm = nullptr;
   // reductions

@@ -19,6 +19,7 @@
 //    tails, scratch sized to residency).  The expansion lists are
 //    element-major (the scans run in lockstep and coalesce); the sort keys
 //    are thread-major, so a thread's partition scans stay in its own lines.
+#include <array>
 #include <cstdlib>
 
 #include "apc_internal.h"
@@ -314,8 +315,9 @@ __global__ void __launch_bounds__(256) k_search(int M, int S, int nrows, const i
 //    an LDS hash set — and not equal to an earlier lane's), appending in lane
 //    order, so the list order is the reference's; errors are raised at the
 //    first candidate (in that order) that hits one;
-//  * sorts: the reference's quicksort, with every partition computed by ranks
-//    instead of the two scans.  For a range [lo, hi] with pivot p = WD[lo], let
+//  * sorts: the reference's quicksort, with every partition of a range longer
+//    than WSMALL computed by ranks instead of the two scans; the short ranges
+//    left over are disjoint and each is sorted by one lane with the scans.  For a range [lo, hi] with pivot p = WD[lo], let
 //    g_1 < g_2 < ... be the positions in (lo, hi] holding a value > p and
 //    l_1 > l_2 > ... the positions holding a value <= p, lo last.  The scans
 //    swap g_k with l_k for every k with g_k < l_k (a prefix, K swaps), and the
@@ -330,11 +332,17 @@ struct WaveLds {
   int mo[WCAP];       // list: index | octant << 29 (insertion order); then WM (sort payload)
   double d2[WCAP];    // list: squared distance; then WD (sort keys)
   int snap[WCAP];     // snapshot in octant order; then the partition's g positions
-  int lasc[WCAP];     // the partition's <= positions, ascending
-  int stk[WCAP];      // pending sort ranges, lo | hi << 16
-  int ht[WHASH];      // membership hash of the list's indices (-1 = empty)
-  double pen[WCAP];   // 1e4^k by repeated multiplication, k = 0..WCAP-1
+  int lasc[WCAP];     // candidate offsets of a round; then the partition's <= positions
+  int smallr[WCAP / 2];  // short sort ranges left to single lanes, lo | hi << 16
+  union {
+    int ht[WHASH];    // membership hash of the list's indices (-1 = empty)
+    int stk[WHASH];   // pending sort ranges, lo | hi << 16; then per-lane stacks
+  };
 };
+
+// 1e4^k by repeated multiplication from 1 (the reference's running penalty),
+// k = 0..WCAP-1; written by the host before the search
+__constant__ double c_pen[WCAP];
 
 __device__ inline uint64_t lanes_below() {
   const int lane = threadIdx.x & 63;
@@ -387,21 +395,72 @@ __device__ inline int place_w(const Frame &F, const double4 *__restrict__ xyz, d
   return (xt > 0) + 2 * (yt > 0) + 4 * (zt > 0);
 }
 
-// the reference's quicksort of (WD, WM)[lo0..hi0], partitions by ranks
-__device__ void wave_qsort(WaveLds &L, int lo0, int hi0) {
+// The reference's quicksort on one short range by one lane (apartition /
+// aquickSort, solver.cpp:600-632, as qpart / qsort_ref above), on the LDS
+// arrays; stk: this lane's stack of WLSTK entries.
+constexpr int WSMALL = 8;  // ranges up to this length go to single lanes (sweep: 6-8 fastest)
+constexpr int WLSTK = 8;    // > log2(WSMALL) + 1 pending ranges per lane
+static_assert(64 * WLSTK <= WHASH, "per-lane stacks fit the stack storage");
+
+__device__ int lane_qpart(double *WD, int *WM, int l, int r) {
+  const double pivot = WD[l];
+  int i = l, j = r + 1;
+  for (;;) {
+    do ++i; while (i <= r && WD[i] <= pivot);
+    do --j; while (WD[j] > pivot);
+    if (i >= j) break;
+    const double t = WD[i]; WD[i] = WD[j]; WD[j] = t;
+    const int ti = WM[i]; WM[i] = WM[j]; WM[j] = ti;
+  }
+  const double t = WD[l]; WD[l] = WD[j]; WD[j] = t;
+  const int ti = WM[l]; WM[l] = WM[j]; WM[j] = ti;
+  return j;
+}
+
+__device__ void lane_qsort(double *WD, int *WM, int l, int r, int need, int *stk) {
+  int top = 0;
+  stk[top++] = l | (r << 16);
+  while (top > 0) {
+    const int pk = stk[--top];
+    int lo = pk & 0xffff, hi = pk >> 16;
+    while (lo < hi && lo < need) {
+      const int j = lane_qpart(WD, WM, lo, hi);
+      if (j - lo < hi - j) {
+        stk[top++] = (j + 1) | (hi << 16);
+        hi = j - 1;
+      } else {
+        stk[top++] = lo | ((j - 1) << 16);
+        lo = j + 1;
+      }
+    }
+  }
+}
+
+// the reference's quicksort of (WD, WM)[lo0..hi0], partitions by ranks; only
+// positions below `need` are wanted: a range lying wholly at or above it is
+// left unsorted (a range's elements stay within it, so nothing below changes)
+__device__ void wave_qsort(WaveLds &L, int lo0, int hi0, int need) {
   const int lane = threadIdx.x & 63;
   double *WD = L.d2;
   int *WM = L.mo;
   int *gp = L.snap, *la = L.lasc;
   if (hi0 <= lo0) return;
   if (lane == 0) L.stk[0] = lo0 | (hi0 << 16);
-  int top = 1;
+  int top = 1, nsmall = 0;
   __syncthreads();
   while (top > 0) {
     --top;
     const int packed = L.stk[top];
     const int lo = packed & 0xffff, hi = packed >> 16;
+    if (hi - lo + 1 <= WSMALL) {  // disjoint from every other range: any lane, any time
+      if (lane == 0) L.smallr[nsmall] = packed;
+      ++nsmall;
+      __syncthreads();
+      continue;
+    }
     const double pivot = WD[lo];
+    int jstar;
+    {
     int nG = 0, nL = 0;
     for (int base = lo + 1; base <= hi; base += 64) {
       const int p = base + lane;
@@ -424,7 +483,7 @@ __device__ void wave_qsort(WaveLds &L, int lo0, int hi0) {
       K += __popcll(__ballot(c));
     }
     const int lK1 = (K + 1 <= nL) ? la[nL - K - 1] : lo;
-    const int jstar = (K > 0) ? max(lK1, gp[K - 1]) : lK1;
+    jstar = (K > 0) ? max(lK1, gp[K - 1]) : lK1;
     for (int base = 0; base < K; base += 64) {
       const int k = base + lane + 1;
       if (k <= K) {
@@ -446,7 +505,8 @@ __device__ void wave_qsort(WaveLds &L, int lo0, int hi0) {
       WM[lo] = WM[jstar];
       WM[jstar] = tm;
     }
-    const bool left = jstar - 1 > lo, right = hi > jstar + 1;
+    }
+    const bool left = jstar - 1 > lo, right = hi > jstar + 1 && jstar + 1 < need;
     if (lane == 0) {
       int t = top;
       if (left) L.stk[t++] = lo | ((jstar - 1) << 16);
@@ -455,6 +515,11 @@ __device__ void wave_qsort(WaveLds &L, int lo0, int hi0) {
     top += (left ? 1 : 0) + (right ? 1 : 0);
     __syncthreads();
   }
+  for (int k = lane; k < nsmall; k += 64) {
+    const int pk = L.smallr[k];
+    lane_qsort(WD, WM, pk & 0xffff, pk >> 16, need, &L.stk[lane * WLSTK]);
+  }
+  __syncthreads();
 }
 
 // per-octant rank of every list entry and its position in octant order
@@ -494,14 +559,6 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
                                                     int *__restrict__ status, int *__restrict__ ovf) {
   __shared__ WaveLds L;
   const int lane = threadIdx.x;
-  if (lane == 0) {
-    double p = 1.0;
-    for (int k = 0; k < WCAP; ++k) {
-      L.pen[k] = p;
-      p = p * 10000.0;
-    }
-  }
-  __syncthreads();
   const int nq = 10 * lm;
   for (int i0 = blockIdx.x; i0 < M; i0 += gridDim.x) {
     for (int h = lane; h < WHASH; h += 64) L.ht[h] = -1;
@@ -577,7 +634,11 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
         }
         const bool invalid = live && m >= S;
         bool fresh = live && !invalid && m != i0 && !ht_has(L.ht, m);
-        for (int u = 0; u < 64; ++u) {  // an earlier candidate of this chunk with the same index
+        // an earlier candidate of this chunk with the same index: only the
+        // candidates still new can matter (an equal earlier one that is not
+        // new makes this one not new either)
+        for (uint64_t mk = __ballot(fresh); mk; mk &= mk - 1) {
+          const int u = __ffsll((unsigned long long)mk) - 1;
           const int mu = __shfl(m, u, 64);
           if (u < lane && mu == m) fresh = false;
         }
@@ -664,7 +725,7 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
       for (int r = 0; r < WCAP / 64; ++r) kp[r] = -1;
       octant_order(L, l, off, [&](int r, int e, int m, int o, int rank, int pos) {
         const double d = L.d2[e];
-        key[r] = (o != 0) ? L.pen[rank] * d : d;
+        key[r] = (o != 0) ? c_pen[rank] * d : d;
         km[r] = m;
         kp[r] = pos;
       });
@@ -678,10 +739,10 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
       __syncthreads();
     }
     // octant 0 quicksorted on the squared distance, then penalised
-    if (nn[0] > 1) wave_qsort(L, 0, nn[0] - 1);
-    for (int q = lane; q < nn[0]; q += 64) L.d2[q] = L.pen[q] * L.d2[q];
+    if (nn[0] > 1) wave_qsort(L, 0, nn[0] - 1, nn[0]);
+    for (int q = lane; q < nn[0]; q += 64) L.d2[q] = c_pen[q] * L.d2[q];
     __syncthreads();
-    wave_qsort(L, 0, l - 1);
+    wave_qsort(L, 0, l - 1, lm);  // positions 0..lm-1 are the output
     for (int q = lane; q < lm; q += 64) {
       const int m = L.mo[q];
       double dd;
@@ -764,6 +825,20 @@ int search_neighbors(apc_ctx *c) {
   hipLaunchKernelGGL(k_pack_xyz, dim3(nblocks(S, 256)), dim3(256), 0, c->s, S, c->cx.p, c->cy.p,
                      c->cz.p, xyz);
   if (wave) {
+    if (!c->search_pen) {  // per context: the constant table lives on its device
+      static const std::array<double, WCAP> pen = [] {
+        std::array<double, WCAP> t{};
+        double p = 1.0;
+        for (int k = 0; k < WCAP; ++k) {
+          t[k] = p;
+          p = p * 10000.0;
+        }
+        return t;
+      }();
+      APC_HIP(c, hipMemcpyToSymbolAsync(HIP_SYMBOL(c_pen), pen.data(), sizeof(double) * WCAP, 0,
+                                        hipMemcpyHostToDevice, c->s));
+      c->search_pen = 1;
+    }
     KScope ks(c, "neighbor_search");
     const int grid = std::max(1, std::min(M, c->search_waves));
     hipLaunchKernelGGL(k_search_wave, dim3(grid), dim3(64), 0, c->s, M, S, lm,
