@@ -635,6 +635,66 @@ __global__ void __launch_bounds__(64) k_cy_dense(int n, const T *__restrict__ in
   if (lane == 0) x[i] = (T)s;
 }
 
+// ---- small coarse levels: G lanes per row (per coarse row for the
+// restriction), so each lane's chain of dependent gathers is G times shorter;
+// the lanes' partial sums meet in a fixed shuffle tree (deterministic)
+template <class T, int G>
+__device__ inline double sell_apply_g(const Sell<T> &A, const T *__restrict__ x, int i, int sub) {
+  const int sl = i >> 6;
+  const long long base = (long long)A.soff[sl] * 64 + (i & 63);
+  const int w = A.wid[sl];
+  double acc = 0.0;
+  for (int k = sub; k < w; k += G) {
+    const long long e = base + (long long)k * 64;
+    acc += (double)A.val[e] * (double)x[A.col[e]];
+  }
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
+  return acc;
+}
+
+template <class T, int G>
+__global__ void __launch_bounds__(256) k_cy_sweep_g(Sell<T> C, const T *__restrict__ dinv,
+                                                    const T *__restrict__ b,
+                                                    const T *__restrict__ xin,
+                                                    T *__restrict__ xout, double omega,
+                                                    const GState *st) {
+  if (st && halted(st)) return;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int i = t / G, sub = t % G;
+  if (i >= C.n) return;  // whole groups (G divides 64)
+  const double ax = sell_apply_g<T, G>(C, xin, i, sub);
+  if (sub == 0) xout[i] = (T)((double)xin[i] + omega * (double)dinv[i] * ((double)b[i] - ax));
+}
+
+template <class T, int G>
+__global__ void __launch_bounds__(256) k_cy_resid_g(Sell<T> C, const T *__restrict__ b,
+                                                    const T *__restrict__ x, T *__restrict__ r,
+                                                    const GState *st) {
+  if (st && halted(st)) return;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int i = t / G, sub = t % G;
+  if (i >= C.n) return;
+  const double ax = sell_apply_g<T, G>(C, x, i, sub);
+  if (sub == 0) r[i] = (T)((double)b[i] - ax);
+}
+
+template <class T, int G>
+__global__ void __launch_bounds__(256) k_cy_restrict_g(int nc, const int *__restrict__ mp,
+                                                       const int *__restrict__ ml,
+                                                       const T *__restrict__ r, T *__restrict__ bc,
+                                                       const GState *st) {
+  if (st && halted(st)) return;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int I = t / G, sub = t % G;
+  if (I >= nc) return;
+  double s = 0.0;
+  for (int e = mp[I] + sub; e < mp[I + 1]; e += G) s += (double)r[ml[e]];
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+  if (sub == 0) bc[I] = (T)s;
+}
+
 // ---------------------------------------------------------------- host side
 static int ilog2_ceil(uint64_t v) {
   int b = 0;
@@ -1027,7 +1087,20 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
   const double sT = sizeof(T);
   const double lm0 = A0.lm;
   const bool f32 = sizeof(T) == 4;
+  // lanes per row on this level's coarse kernels (and per coarse row on the
+  // restriction into the next): more on the small, latency-bound levels
+  const int gl = ell ? 1 : (n <= o.g8_rows ? 8 : (n <= o.g4_rows ? 4 : 1));
   auto sweep = [&](const T *xin, T *xout) {
+    if (!ell && gl > 1) {
+      KScope ks(c, kSweepName[l]);
+      if (gl == 8)
+        hipLaunchKernelGGL((k_cy_sweep_g<T, 8>), dim3(nblocks((long)n * 8, 256)), dim3(256), 0,
+                           c->s, C, dinv, b, xin, xout, o.omega, st);
+      else
+        hipLaunchKernelGGL((k_cy_sweep_g<T, 4>), dim3(nblocks((long)n * 4, 256)), dim3(256), 0,
+                           c->s, C, dinv, b, xin, xout, o.omega, st);
+      return;
+    }
     if (ell) {
       KScope ks(c, f32 ? "amg_sweep0_32" : "amg_sweep0", (double)n * (lm0 * (4 + 2 * sT) + 6 * sT));
       hipLaunchKernelGGL((k_cy_sweep<T, true>), dim3(g), dim3(256), 0, c->s, E, C, dinv, b, xin,
@@ -1070,6 +1143,14 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
     if (ell) {
       KScope ks(c, f32 ? "amg_resid0_32" : "amg_resid0", (double)n * (lm0 * (4 + 2 * sT) + 5 * sT));
       hipLaunchKernelGGL((k_cy_resid<T, true>), dim3(g), dim3(256), 0, c->s, E, C, b, cur, r, st);
+    } else if (gl == 8) {
+      KScope ks(c, kResidName[l]);
+      hipLaunchKernelGGL((k_cy_resid_g<T, 8>), dim3(nblocks((long)n * 8, 256)), dim3(256), 0, c->s,
+                         C, b, cur, r, st);
+    } else if (gl == 4) {
+      KScope ks(c, kResidName[l]);
+      hipLaunchKernelGGL((k_cy_resid_g<T, 4>), dim3(nblocks((long)n * 4, 256)), dim3(256), 0, c->s,
+                         C, b, cur, r, st);
     } else {
       KScope ks(c, kResidName[l]);
       hipLaunchKernelGGL((k_cy_resid<T, false>), dim3(g), dim3(256), 0, c->s, E, C, b, cur, r, st);
@@ -1077,8 +1158,12 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
     {
       KScope ks(c, ell ? (f32 ? "amg_restrict0_32" : "amg_restrict0") : kRestrictName[l],
                 ell ? (double)n * (sT + 4) + (double)Lc->n * (4 + sT) : 0.0);
-      hipLaunchKernelGGL((k_cy_restrict<T>), dim3(nblocks(Lc->n, 256)), dim3(256), 0, c->s, Lc->n,
-                         L->mp.p, L->ml.p, r, Pc.b, st);
+      if (Lc->n <= o.rg8_rows)
+        hipLaunchKernelGGL((k_cy_restrict_g<T, 8>), dim3(nblocks((long)Lc->n * 8, 256)), dim3(256),
+                           0, c->s, Lc->n, L->mp.p, L->ml.p, r, Pc.b, st);
+      else
+        hipLaunchKernelGGL((k_cy_restrict<T>), dim3(nblocks(Lc->n, 256)), dim3(256), 0, c->s,
+                           Lc->n, L->mp.p, L->ml.p, r, Pc.b, st);
     }
     int rr = cycle<T>(c, h, A0, l + 1, Pc.b, Pc.x, Pc.t, st);
     if (rr) return rr;

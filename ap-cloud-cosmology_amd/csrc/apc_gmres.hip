@@ -680,6 +680,9 @@ static AmgOptions amg_options() {
   a.wlevels = env_int("APC_AMG_WLEVELS", a.wlevels);
   a.coarse_max = env_int("APC_AMG_COARSE", a.coarse_max);
   a.galerkin_sorted = env_int("APC_AMG_GALERKIN_SORTED", 0);
+  a.g8_rows = env_int("APC_AMG_G8", a.g8_rows);
+  a.g4_rows = env_int("APC_AMG_G4", a.g4_rows);
+  a.rg8_rows = env_int("APC_AMG_RG8", a.rg8_rows);
   if (const char *v = getenv("APC_AMG_OMEGA")) a.omega = atof(v);
   if (const char *v = getenv("APC_AMG_ALPHA")) a.alpha = atof(v);
   if (const char *v = getenv("APC_AMG_THETA")) a.theta = atof(v);
