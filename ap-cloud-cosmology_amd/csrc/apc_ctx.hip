@@ -158,9 +158,9 @@ int apc_destroy(apc_ctx *c) {
                           &c->status_dev, &c->tr_pos, &c->tr_pos2, &c->tr_rp, &c->cic_v,
                           &c->cic_v2, &c->cic_rng};
   for (auto b : is) b->release();
-  for (auto &b : c->ckpt) b.release();
-  c->key.release(); c->key2.release(); c->nprefix.release(); c->red_u64.release();
-  c->tr_key.release(); c->tr_key2.release(); c->cic_key.release(); c->cic_key2.release();
+  for (auto &b : c->ckpt) b.release(c->s);
+  c->key.release(c->s); c->key2.release(c->s); c->nprefix.release(c->s); c->red_u64.release(c->s);
+  c->tr_key.release(c->s); c->tr_key2.release(c->s); c->cic_key.release(c->s); c->cic_key2.release(c->s);
   hipStreamDestroy(c->s);
   delete c;
   return APC_OK;
@@ -295,8 +295,8 @@ int apc_set_gradients(apc_ctx *c, const double *g, const double *H) {
   const int M = c->nc + c->nv;
   APC_ALLOC(c, c->grad, (size_t)9 * M + 1);
   apc::DBuf<double> tg, th;
-  if (tg.ensure((size_t)3 * M + 1) || th.ensure((size_t)6 * M + 1)) {
-    tg.release(); th.release();
+  if (tg.ensure((size_t)3 * M + 1, c->s) || th.ensure((size_t)6 * M + 1, c->s)) {
+    tg.release(c->s); th.release(c->s);
     return fail(c, APC_ERR_NOMEM, "set_gradients");
   }
   hipError_t e = hipMemcpyAsync(tg.p, g, 8ull * 3 * M, hipMemcpyHostToDevice, c->s);
@@ -305,7 +305,7 @@ int apc_set_gradients(apc_ctx *c, const double *g, const double *H) {
     hipLaunchKernelGGL(k_set_grad, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, tg.p, th.p, c->grad.p);
     e = hipStreamSynchronize(c->s);
   }
-  tg.release(); th.release();
+  tg.release(c->s); th.release(c->s);
   if (e != hipSuccess) return hip_fail(c, e, "set_gradients");
   c->have_fields = 1;
   return APC_OK;
@@ -385,8 +385,8 @@ int apc_stream_copy(apc_ctx *c, double bytes, int reps, double *gbs) {
   if (!c || bytes < 1024 || reps < 1 || !gbs) return APC_ERR_ARG;
   size_t n = (size_t)(bytes / 16);  // double2 elements
   apc::DBuf<double> a, b;
-  if (a.ensure(2 * n) != hipSuccess || b.ensure(2 * n) != hipSuccess) {
-    a.release(); b.release();
+  if (a.ensure(2 * n, c->s) != hipSuccess || b.ensure(2 * n, c->s) != hipSuccess) {
+    a.release(c->s); b.release(c->s);
     return apc::fail(c, APC_ERR_NOMEM, "stream_copy");
   }
   hipEvent_t e0, e1;
@@ -405,8 +405,8 @@ int apc_stream_copy(apc_ctx *c, double bytes, int reps, double *gbs) {
   hipEventElapsedTime(&ms, e0, e1);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  a.release();
-  b.release();
+  a.release(c->s);
+  b.release(c->s);
   if (e != hipSuccess) return apc::hip_fail(c, e, "stream_copy");
   *gbs = 2.0 * 16.0 * n * reps / (ms * 1e-3) / 1e9;
   return APC_OK;

@@ -46,7 +46,7 @@ struct SolveState {
   GState *hst = nullptr;  // pinned mirror
   AmgHier *amg = nullptr;
   int mcap = 0, histcap = 0;
-  ~SolveState() {
+  ~SolveState() {  // the owner has drained its stream (apc_destroy)
     V64.release(); dscratch.release(); r64.release(); part.release(); z64.release();
     u64.release(); V32.release(); s32.release(); d32.release(); f32.release();
     dinv32.release(); z32.release(); u32.release(); Z32.release(); hess.release(); st.release();

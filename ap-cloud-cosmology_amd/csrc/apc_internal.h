@@ -11,15 +11,21 @@
 
 namespace apc {
 
-// Grow-only device buffer.
+// Grow-only device buffer.  Growing frees the old allocation: `s`, the stream
+// whose queued work may still use it, is drained first, so that memory is never
+// handed to another context's thread while a kernel here still reads it.
 template <class T>
 struct DBuf {
   T *p = nullptr;
   size_t cap = 0;
-  hipError_t ensure(size_t n) {
+  hipError_t ensure(size_t n, hipStream_t s) {
     if (n <= cap) return hipSuccess;
     size_t want = n + n / 4 + 256;
-    if (p) hipFree(p);
+    if (p) {
+      hipError_t e = hipStreamSynchronize(s);
+      if (e != hipSuccess) return e;
+      hipFree(p);
+    }
     p = nullptr;
     cap = 0;
     hipError_t e = hipMalloc((void **)&p, want * sizeof(T));
@@ -27,8 +33,13 @@ struct DBuf {
     cap = want;
     return hipSuccess;
   }
-  void release() {
-    if (p) hipFree(p);
+  // s: as for ensure; none where the owner has drained its stream already
+  // (context destruction)
+  void release(hipStream_t s = nullptr) {
+    if (p) {
+      if (s) (void)hipStreamSynchronize(s);
+      hipFree(p);
+    }
     p = nullptr;
     cap = 0;
   }
@@ -135,7 +146,7 @@ int hip_fail(apc_ctx *c, hipError_t e, const char *what);
 
 #define APC_ALLOC(ctx, buf, n)                                          \
   do {                                                                  \
-    if ((buf).ensure(n) != hipSuccess)                                  \
+    if ((buf).ensure(n, (ctx)->s) != hipSuccess)                        \
       return apc::fail(ctx, APC_ERR_NOMEM, "hipMalloc failed for %s (%zu elements)", #buf, (size_t)(n)); \
   } while (0)
 

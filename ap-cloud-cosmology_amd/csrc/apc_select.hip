@@ -433,9 +433,9 @@ int select_tree(apc_ctx *c, double cc, int order, int mindepth, int maxdepth) {
       Tree old = t;
       size_t ncap = need + need / 2;
       DBuf<uint64_t> pr; DBuf<int> i6[7]; DBuf<double> d3[3];
-      if (pr.ensure(ncap) != hipSuccess) return fail(c, APC_ERR_NOMEM, "select: grow");
-      for (auto &q : i6) if (q.ensure(ncap) != hipSuccess) return fail(c, APC_ERR_NOMEM, "select: grow");
-      for (auto &q : d3) if (q.ensure(ncap) != hipSuccess) return fail(c, APC_ERR_NOMEM, "select: grow");
+      if (pr.ensure(ncap, c->s) != hipSuccess) return fail(c, APC_ERR_NOMEM, "select: grow");
+      for (auto &q : i6) if (q.ensure(ncap, c->s) != hipSuccess) return fail(c, APC_ERR_NOMEM, "select: grow");
+      for (auto &q : d3) if (q.ensure(ncap, c->s) != hipSuccess) return fail(c, APC_ERR_NOMEM, "select: grow");
       APC_HIP(c, hipMemcpyAsync(pr.p, old.prefix, 8ull * b, hipMemcpyDeviceToDevice, c->s));
       const int *srcs[7] = {old.level, old.first, old.np, old.status, old.child, old.parent, c->nout.p};
       for (int q = 0; q < 7; ++q)
@@ -449,9 +449,9 @@ int select_tree(apc_ctx *c, double cc, int order, int mindepth, int maxdepth) {
       std::swap(c->nstatus, i6[3]); std::swap(c->nchild, i6[4]); std::swap(c->nparent, i6[5]);
       std::swap(c->nout, i6[6]);
       std::swap(c->ncx, d3[0]); std::swap(c->ncy, d3[1]); std::swap(c->ncz, d3[2]);
-      pr.release();
-      for (auto &q : i6) q.release();
-      for (auto &q : d3) q.release();
+      pr.release(c->s);
+      for (auto &q : i6) q.release(c->s);
+      for (auto &q : d3) q.release(c->s);
       t = tree_of(c);
     }
     {
@@ -495,9 +495,9 @@ int select_tree(apc_ctx *c, double cc, int order, int mindepth, int maxdepth) {
         // grow preserving content (rare: capacity carries over between steps)
         DBuf<uint64_t> pr; DBuf<int> i6[7]; DBuf<double> d3[3];
         size_t ncap = need + need / 2;
-        if (pr.ensure(ncap) != hipSuccess) return fail(c, APC_ERR_NOMEM, "select: grow");
-        for (auto &q : i6) if (q.ensure(ncap) != hipSuccess) return fail(c, APC_ERR_NOMEM, "select: grow");
-        for (auto &q : d3) if (q.ensure(ncap) != hipSuccess) return fail(c, APC_ERR_NOMEM, "select: grow");
+        if (pr.ensure(ncap, c->s) != hipSuccess) return fail(c, APC_ERR_NOMEM, "select: grow");
+        for (auto &q : i6) if (q.ensure(ncap, c->s) != hipSuccess) return fail(c, APC_ERR_NOMEM, "select: grow");
+        for (auto &q : d3) if (q.ensure(ncap, c->s) != hipSuccess) return fail(c, APC_ERR_NOMEM, "select: grow");
         APC_HIP(c, hipMemcpyAsync(pr.p, t.prefix, 8ull * T, hipMemcpyDeviceToDevice, c->s));
         const int *srcs[7] = {t.level, t.first, t.np, t.status, t.child, t.parent, c->nout.p};
         for (int q = 0; q < 7; ++q)
@@ -511,9 +511,9 @@ int select_tree(apc_ctx *c, double cc, int order, int mindepth, int maxdepth) {
         std::swap(c->nstatus, i6[3]); std::swap(c->nchild, i6[4]); std::swap(c->nparent, i6[5]);
         std::swap(c->nout, i6[6]);
         std::swap(c->ncx, d3[0]); std::swap(c->ncy, d3[1]); std::swap(c->ncz, d3[2]);
-        pr.release();
-        for (auto &q : i6) q.release();
-        for (auto &q : d3) q.release();
+        pr.release(c->s);
+        for (auto &q : i6) q.release(c->s);
+        for (auto &q : d3) q.release(c->s);
         t = tree_of(c);
       }
       {
@@ -553,8 +553,8 @@ int select_tree(apc_ctx *c, double cc, int order, int mindepth, int maxdepth) {
     if (nv > 0) {
       DBuf<uint64_t> code, code2;
       DBuf<int> node, node2;
-      if (code.ensure(nv) || code2.ensure(nv) || node.ensure(nv) || node2.ensure(nv)) {
-        code.release(); code2.release(); node.release(); node2.release();
+      if (code.ensure(nv, c->s) || code2.ensure(nv, c->s) || node.ensure(nv, c->s) || node2.ensure(nv, c->s)) {
+        code.release(c->s); code2.release(c->s); node.release(c->s); node2.release(c->s);
         return fail(c, APC_ERR_NOMEM, "select: vacuum sort");
       }
       hipLaunchKernelGGL(k_vac_codes, dim3(nblocks(m, 256)), dim3(256), 0, c->s, t, Ttd, T,
@@ -564,7 +564,7 @@ int select_tree(apc_ctx *c, double cc, int order, int mindepth, int maxdepth) {
         hipLaunchKernelGGL(k_vac_emit, dim3(nblocks(nv, 256)), dim3(256), 0, c->s, nv, nc, node.p,
                            c->leaf_node.p, c->nout.p);
       APC_HIP(c, hipStreamSynchronize(c->s));
-      code.release(); code2.release(); node.release(); node2.release();
+      code.release(c->s); code2.release(c->s); node.release(c->s); node2.release(c->s);
       if (r) return r;
     }
   }

@@ -242,8 +242,8 @@ extern "C" int apc_octree_level(apc_ctx *c, int level, int *count, uint64_t *key
   if (!key && !first && !np) return 0;
   DBuf<uint64_t> ok;
   DBuf<int> of, on;
-  if (ok.ensure(m) != hipSuccess || of.ensure(m) != hipSuccess || on.ensure(m) != hipSuccess) {
-    ok.release(); of.release(); on.release();
+  if (ok.ensure(m, c->s) != hipSuccess || of.ensure(m, c->s) != hipSuccess || on.ensure(m, c->s) != hipSuccess) {
+    ok.release(c->s); of.release(c->s); on.release(c->s);
     return fail(c, APC_ERR_NOMEM, "apc_octree_level: allocation");
   }
   hipLaunchKernelGGL(k_level_emit, dim3(nblocks(n, 256)), dim3(256), 0, c->s, c->key.p, n, sh,
@@ -254,7 +254,7 @@ extern "C" int apc_octree_level(apc_ctx *c, int level, int *count, uint64_t *key
   if (first && e == hipSuccess) e = hipMemcpyAsync(first, of.p, 4ull * m, hipMemcpyDeviceToHost, c->s);
   if (np && e == hipSuccess) e = hipMemcpyAsync(np, on.p, 4ull * m, hipMemcpyDeviceToHost, c->s);
   if (e == hipSuccess) e = hipStreamSynchronize(c->s);
-  ok.release(); of.release(); on.release();
+  ok.release(c->s); of.release(c->s); on.release(c->s);
   if (e != hipSuccess) return hip_fail(c, e, "apc_octree_level copies");
   return 0;
 }
