@@ -525,7 +525,9 @@ __global__ void __launch_bounds__(256) k_gj_pivot(int n, int k, double *__restri
       g[(size_t)p * w + j] = a;
     }
   __syncthreads();
+  // every wave reads the pivot before any thread overwrites it with 1
   const double inv = 1.0 / g[(size_t)k * w + k];
+  __syncthreads();
   for (int j = t; j < w; j += 256) g[(size_t)k * w + j] *= inv;
   __syncthreads();
   for (int i = t; i < n; i += 256) colk[i] = g[(size_t)i * w + k];

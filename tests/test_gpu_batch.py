@@ -97,3 +97,31 @@ def test_batch_reports_the_failing_box():
     with pytest.raises(apcloud.ApcError, match="box 1"):
         B.run(schedule(1), [1.0 / len(x)] * 3, apcloud.ksp(rtol=1e-8))
     B.close()
+
+
+def test_batch_under_load_is_repeatable():
+    """48 boxes (16^3 to 96^3) on 16 host threads, multigrid on the Poisson
+    solve: two runs end bit-identical and every solve converges.  Concurrent
+    boxes desynchronise the waves of a workgroup, which exposed an intra-block
+    race in the coarsest-level Gauss-Jordan pivot scaling (some boxes' solves
+    then ran to max_it)."""
+    specs = ic.multibox_specs()
+    boxes = [ic.multibox_box(s) for s in specs]
+    params = apcloud.make_params(**PARAMS)
+    o1 = apcloud.ksp(rtol=1e-8, restart=200, max_it=1000)
+    o2 = apcloud.ksp(rtol=1e-8, restart=200, max_it=1000, pc="amg")
+    coef = schedule(2)
+    cd = np.array([1.0 / len(b[0]) for b in boxes])
+    runs = []
+    for _ in range(2):
+        B = apcloud.Batch(params, len(boxes), nthreads=16)
+        for i, b in enumerate(boxes):
+            B.set_particles(i, *b)
+        info, status = B.run(coef, cd, o1, o2)
+        assert np.all(status == 0)
+        bad = [i for i, x in enumerate(info) if not (x.reason_rho > 0 and x.reason_phi > 0)]
+        assert not bad, bad
+        runs.append([np.array(B.boxes[i].particles()) for i in range(len(boxes))])
+        B.close()
+    diff = [i for i in range(len(boxes)) if not np.array_equal(runs[0][i], runs[1][i])]
+    assert not diff, diff
