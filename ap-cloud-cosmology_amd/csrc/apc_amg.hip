@@ -646,7 +646,24 @@ __device__ inline double sell_apply_g(const Sell<T> &A, const T *__restrict__ x,
   const long long base = (long long)A.soff[sl] * 64 + (i & 63);
   const int w = A.wid[sl];
   double acc = 0.0;
-  for (int k = sub; k < w; k += G) {
+  // a lane's slots sub, sub+G, ... in groups of four: the group's column and
+  // value loads, then its gathers, then the additions in slot order
+  int k = sub;
+  for (; k + 3 * G < w; k += 4 * G) {
+    int cj[4];
+    T av[4], xv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long e = base + (long long)(k + u * G) * 64;
+      cj[u] = A.col[e];
+      av[u] = A.val[e];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xv[u] = x[cj[u]];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc += (double)av[u] * (double)xv[u];
+  }
+  for (; k < w; k += G) {
     const long long e = base + (long long)k * 64;
     acc += (double)A.val[e] * (double)x[A.col[e]];
   }

@@ -23,6 +23,8 @@
 // matrix, inside FP64 iterative refinement — every restart computes the true
 // residual with the FP64 operator, the FP32 cycle solves for the correction
 // and the correction is accumulated into the FP64 solution.
+#include <algorithm>
+#include <cmath>
 #include <cstdlib>
 
 #include "apc_solve.h"
@@ -135,11 +137,7 @@ __global__ void __launch_bounds__(256) k_spmv_z(Op<double> A, const float *__res
   const double s = g.sig[j];
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= A.M) return;
-  double acc = 0.0;
-  for (int q = 0; q < A.lm; ++q) {
-    int m = A.nbr[(size_t)q * A.M + i];
-    if (m < A.M) acc += A.s[(size_t)q * A.M + i] * (double)z[m];
-  }
+  const double acc = row_offdiag_sum<double, float>(A, z, i);
   const float zi = z[i];
   zj[i] = zi;
   w[i] = s * (A.d[i] * (double)zi + A.f[i] * acc);
@@ -505,6 +503,26 @@ static GDev gdev_of(SolveState *S, int m) {
   return g;
 }
 
+// Arnoldi steps to launch before the host next reads the solve's state.
+// Between reads the launches of a solve that has converged return at once (a
+// few microseconds each, but a multigrid cycle is tens of launches), so the
+// next read is placed where the residual, extrapolated at the rate observed
+// in this restart cycle, reaches the target (at most 16 steps ahead).  With
+// kernel timing on the state is read after every step, so that no timed
+// launch is one that returned at once.
+static int steps_to_next_read(const GState &h, bool timing) {
+  constexpr int kMax = 16;
+  if (timing) return 1;
+  double target = h.ttol;
+  if (h.fp32) target = std::max(target, 2e-6 * h.cyc_res0);  // the FP32 cycle's floor
+  if (h.it < 2 || !(h.res > 0.0) || !(h.cyc_res0 > h.res)) return kMax;
+  if (h.res <= target) return 1;
+  const double rate = std::pow(h.res / h.cyc_res0, 1.0 / h.it);
+  const double need = std::log(target / h.res) / std::log(rate);
+  if (!(need < kMax)) return kMax;
+  return std::max(1, (int)std::ceil(need));
+}
+
 static int read_state(apc_ctx *c, SolveState *S) {
   APC_HIP(c, hipMemcpyAsync(S->hst, S->st.p, sizeof(GState), hipMemcpyDeviceToHost, c->s));
   APC_HIP(c, hipStreamSynchronize(c->s));
@@ -578,7 +596,7 @@ static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, 
     int r = read_state(c, S);
     if (r) return r;
     if (S->hst->reason) { reason = S->hst->reason; monitor_flush(S->hst->its); break; }
-    const int batch = 16;
+    int to_read = steps_to_next_read(*S->hst, c->timing);
     for (int j = 0; j < m && j < budget; ++j) {
       const T *vj = V + (size_t)j * ld;
       T *w = V + (size_t)(j + 1) * ld;
@@ -623,10 +641,11 @@ static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, 
                            (int)gaxpy, g);
       }
       ++it_launched;
-      if ((j + 1) % batch == 0) {
+      if (--to_read == 0) {
         r = read_state(c, S);
         if (r) return r;
         if (S->hst->reason || S->hst->cycle_done) break;
+        to_read = steps_to_next_read(*S->hst, c->timing);
       }
     }
     r = read_state(c, S);

@@ -31,12 +31,35 @@ struct GState {
 
 __device__ inline bool halted(const GState *st) { return st->reason != 0 || st->cycle_done != 0; }
 
-template <class T>
-__device__ inline T row_offdiag_sum(const Op<T> &A, const T *__restrict__ x, int i) {
+// Sum over the row's neighbour slots in slot order.  The slots are taken in
+// groups of RG: the group's index and weight loads are issued together, then
+// its gathers (an empty slot gathers the row's own entry and adds nothing), so
+// a thread keeps a group's loads in flight instead of one slot's.  The
+// additions are the slot-order sequence of the plain loop.  X: the vector's
+// type (widened to T before the product).
+template <class T, class X = T>
+__device__ inline T row_offdiag_sum(const Op<T> &A, const X *__restrict__ x, int i) {
+  constexpr int RG = 27;
   T acc = 0;
-  for (int j = 0; j < A.lm; ++j) {
+  int j = 0;
+  for (; j + RG <= A.lm; j += RG) {
+    int m[RG];
+    T s[RG];
+    X xv[RG];
+#pragma unroll
+    for (int u = 0; u < RG; ++u) {
+      m[u] = A.nbr[(size_t)(j + u) * A.M + i];
+      s[u] = A.s[(size_t)(j + u) * A.M + i];
+    }
+#pragma unroll
+    for (int u = 0; u < RG; ++u) xv[u] = x[m[u] < A.M ? m[u] : i];
+#pragma unroll
+    for (int u = 0; u < RG; ++u)
+      if (m[u] < A.M) acc += s[u] * (T)xv[u];
+  }
+  for (; j < A.lm; ++j) {
     int m = A.nbr[(size_t)j * A.M + i];
-    if (m < A.M) acc += A.s[(size_t)j * A.M + i] * x[m];
+    if (m < A.M) acc += A.s[(size_t)j * A.M + i] * (T)x[m];
   }
   return acc;
 }
@@ -58,17 +81,20 @@ __device__ inline double row_apply(const Op<T> &A, const T *__restrict__ x, int 
 struct AmgHier;  // apc_amg.hip
 
 // Defaults from sweeps on the 256^3 Zel'dovich + Plummer system (7.5M rows):
-// a W-cycle on every level with one sweep each side, damping 0.85, a 1.5x
-// coarse correction and a dense solve once a level has <= 1024 rows (5
-// levels) converged in the fewest seconds (55 GMRES steps to rtol 1e-8).
+// one Jacobi sweep each side, damping 0.85, a 1.5x coarse correction, a
+// W-cycle on the three finest levels and V below, and a dense solve once a
+// level has <= 256 rows (6 levels: 7.5M, 611k, 58k, 6.7k, 838, ~100) took the
+// step from 287 to 248 ms (40 -> 33 GMRES steps to rtol 1e-8) against a
+// W-cycle everywhere with a dense level of <= 1024 rows.  A larger correction
+// scale is faster still up to 1.7 and diverges at 1.9, so 1.5 keeps a margin.
 struct AmgOptions {
   int nu = 1;             // Jacobi sweeps before and after the coarse correction
   double omega = 0.85;    // Jacobi damping
   double alpha = 1.5;     // scaling of the coarse correction
   int gamma = 2;          // 1: V-cycle, 2: W-cycle
-  int wlevels = 64;       // gamma applies on the first wlevels levels, V below
+  int wlevels = 3;        // gamma applies on the first wlevels levels, V below
   double theta = 0.0;     // strength threshold (fraction of the row's largest |a_ij|)
-  int coarse_max = 1024;  // dense solve at or below this many rows
+  int coarse_max = 256;   // dense solve at or below this many rows
   int max_levels = 12;
   int galerkin_sorted = 0;  // form every coarse operator by the global sort (tests)
   // coarse levels with at most g8_rows / g4_rows rows run their sweeps and

@@ -70,10 +70,13 @@ def eds(a):
     return 1.5 / a, -1.0, cx, cx / a / a
 
 
-def traffic_from_profiles(kernel, workload):
+def traffic_from_profiles(kernel, workload, alg_bytes):
     """HBM bytes per launch of `kernel` from the committed PMC summary
     (profiles/pmc_traffic.json, written from rocprofv3 FETCH_SIZE and
-    WRITE_SIZE runs scaled by a copy kernel of known size), or None."""
+    WRITE_SIZE runs scaled by a copy kernel of known size), or None.  Where
+    the summary holds the measured traffic over the algorithmic bytes (per
+    row, so independent of the step's row count) it is applied to this
+    launch's algorithmic bytes `alg_bytes`."""
     path = os.path.join(REPO, "profiles", "pmc_traffic.json")
     try:
         with open(path) as f:
@@ -81,6 +84,8 @@ def traffic_from_profiles(kernel, workload):
         e = d["kernels"][kernel]
         if d.get("workload") != workload:
             return None
+        if "traffic_over_algorithmic" in e:
+            return round(float(e["traffic_over_algorithmic"]) * alg_bytes)
         return float(e["hbm_bytes_per_launch"])
     except (OSError, KeyError, ValueError, TypeError):
         return None
@@ -236,7 +241,7 @@ def main():
         ach = b / (avg_ms * 1e-3) / 1e9
         roof = {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles(name, args.workload),
+                "traffic": traffic_from_profiles(name, args.workload, b),
                 "bytes_per_launch": int(b), "avg_launch_ms": round(avg_ms, 4), "launches": cnt,
                 "measured_copy_GBs": round(copy_gbs, 1)}
 
