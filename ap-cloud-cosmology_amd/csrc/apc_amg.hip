@@ -21,6 +21,7 @@
 //    amplify it (GMRES resolves that one direction itself).
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "apc_solve.h"
@@ -128,6 +129,35 @@ struct EllView {
       if (m < A.M) f(m, fi * A.s[(size_t)q * A.M + i]);
     }
   }
+  // every slot in slot order as f(j, a, ok): ok for an interior column j;
+  // otherwise j = i (a valid index to gather from) and the slot is to be
+  // ignored.  Slots are loaded nine at a time so that f's gathers at j do
+  // not wait on one another.
+  template <class F>
+  __device__ void offdiag_all(int i, F f) const {
+    const double fi = A.f[i];
+    constexpr int RG = 9;
+    int q = 0;
+    for (; q + RG <= A.lm; q += RG) {
+      int m[RG];
+      double sv[RG];
+#pragma unroll
+      for (int u = 0; u < RG; ++u) {
+        m[u] = A.nbr[(size_t)(q + u) * A.M + i];
+        sv[u] = A.s[(size_t)(q + u) * A.M + i];
+      }
+#pragma unroll
+      for (int u = 0; u < RG; ++u) {
+        const bool ok = m[u] < A.M;
+        f(ok ? m[u] : i, fi * sv[u], ok);
+      }
+    }
+    for (; q < A.lm; ++q) {
+      const int m = A.nbr[(size_t)q * A.M + i];
+      const bool ok = m < A.M;
+      f(ok ? m : i, fi * A.s[(size_t)q * A.M + i], ok);
+    }
+  }
 };
 
 struct CsrView {
@@ -150,6 +180,10 @@ struct CsrView {
     for (int e = A.rp[i]; e < A.rp[i + 1]; ++e)
       if (A.col[e] != i) f(A.col[e], A.val[e]);
   }
+  template <class F>
+  __device__ void offdiag_all(int i, F f) const {
+    for (int e = A.rp[i]; e < A.rp[i + 1]; ++e) f(A.col[e], A.val[e], A.col[e] != i);
+  }
 };
 
 __device__ inline uint32_t prio_of(uint32_t i) {  // bijective on 32 bits
@@ -166,7 +200,9 @@ __global__ void __launch_bounds__(256) k_amg_smax(V v, double *__restrict__ smax
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= v.n()) return;
   double m = 0.0;
-  v.offdiag(i, [&](int, double a) { m = fmax(m, fabs(a)); });
+  v.offdiag_all(i, [&](int, double a, bool ok) {
+    if (ok) m = fmax(m, fabs(a));
+  });
   smax[i] = m;
 }
 
@@ -183,8 +219,9 @@ __global__ void __launch_bounds__(256) k_amg_pick(V v, double theta, const doubl
   const uint32_t pi = prio_of((uint32_t)i);
   const double lim = theta * smax[i];
   bool best = true;
-  v.offdiag(i, [&](int j, double x) {
-    if (fabs(x) >= lim && a[j] == 0 && prio_of((uint32_t)j) > pi) best = false;
+  v.offdiag_all(i, [&](int j, double x, bool ok) {
+    const int aj = a[j];
+    if (ok && fabs(x) >= lim && aj == 0 && prio_of((uint32_t)j) > pi) best = false;
   });
   b[i] = best ? 1 : 0;
 }
@@ -199,8 +236,9 @@ __global__ void __launch_bounds__(256) k_amg_cover(V v, double theta, const doub
   if (b[i] != 0) return;
   const double lim = theta * smax[i];
   bool cov = false;
-  v.offdiag(i, [&](int j, double x) {
-    if (fabs(x) >= lim && b[j] == 1) cov = true;
+  v.offdiag_all(i, [&](int j, double x, bool ok) {
+    const int bj = b[j];
+    if (ok && fabs(x) >= lim && bj == 1) cov = true;
   });
   if (cov) b[i] = 2;
   else atomicAdd(undecided, 1);
@@ -224,9 +262,10 @@ __global__ void __launch_bounds__(256) k_amg_agg(V v, double theta, const double
   const double lim = theta * smax[i];
   double bv = -1.0;
   int bj = -1;
-  v.offdiag(i, [&](int j, double x) {
-    double ax = fabs(x);
-    if (ax >= lim && st[j] == 1 && (ax > bv || (ax == bv && j < bj))) { bv = ax; bj = j; }
+  v.offdiag_all(i, [&](int j, double x, bool ok) {
+    const double ax = fabs(x);
+    const int sj = st[j];
+    if (ok && ax >= lim && sj == 1 && (ax > bv || (ax == bv && j < bj))) { bv = ax; bj = j; }
   });
   if (bj < 0) { atomicAdd(bad, 1); agg[i] = 0; return; }
   agg[i] = rank[bj];
@@ -330,9 +369,13 @@ __global__ void __launch_bounds__(256) k_amg_reduce(int n, int nc, const uint64_
 //  3. one 64-lane block per coarse row reads its span and takes the distinct
 //     coarse columns in increasing order by repeated wave minima, summing
 //     each over its entries with a fixed reduction tree (deterministic).  A
-//     span of up to GAL_CAP entries is held in registers; a longer one is
-//     re-read from memory for every column.
-//  Pass 0 of step 3 counts each coarse row's entries, pass 1 writes them at rp[I].
+//     span of up to GAL_CAP entries is held in registers (as many per lane as
+//     the span needs); a longer one is re-read from memory for every column.
+//  Pass 0 of step 3 counts each coarse row's distinct columns; a span held in
+//  registers also gets its result (column, sum) written over its own start,
+//  the whole span having been read first.  Pass 1 copies those results to
+//  rp[I], and runs the streamed reduction again for the longer spans, writing
+//  at rp[I].
 constexpr int GAL_CAP = 1024;
 constexpr int GAL_R = GAL_CAP / 64;
 
@@ -377,69 +420,154 @@ __global__ void __launch_bounds__(256) k_gal_emit(V v, int n, const int *__restr
   });
 }
 
+// Step 2 on the ELL level: the generic emission (k_gal_emit<EllView>) with
+// the row's slots taken in groups of nine, each group's index and weight loads
+// issued together, then its aggregate gathers (an empty slot gathers the row's
+// own entry and emits nothing), then the stores in slot order: the same
+// entries in the same order, without a dependent load chain per slot.
+__global__ void __launch_bounds__(256) k_gal_emit_ell(Op<double> A, const int *__restrict__ agg,
+                                                      const int *__restrict__ off_row,
+                                                      int *__restrict__ ej,
+                                                      double *__restrict__ ea) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= A.M) return;
+  int e = off_row[i];
+  ej[e] = agg[i];
+  ea[e] = A.d[i];
+  ++e;
+  const double fi = A.f[i];
+  constexpr int RG = 9;
+  int q = 0;
+  for (; q + RG <= A.lm; q += RG) {
+    int m[RG], a[RG];
+    double sv[RG];
+#pragma unroll
+    for (int u = 0; u < RG; ++u) {
+      m[u] = A.nbr[(size_t)(q + u) * A.M + i];
+      sv[u] = A.s[(size_t)(q + u) * A.M + i];
+    }
+#pragma unroll
+    for (int u = 0; u < RG; ++u) a[u] = agg[m[u] < A.M ? m[u] : i];
+#pragma unroll
+    for (int u = 0; u < RG; ++u)
+      if (m[u] < A.M) {
+        ej[e] = a[u];
+        ea[e] = fi * sv[u];
+        ++e;
+      }
+  }
+  for (; q < A.lm; ++q) {
+    const int m = A.nbr[(size_t)q * A.M + i];
+    if (m < A.M) {
+      ej[e] = agg[m];
+      ea[e] = fi * A.s[(size_t)q * A.M + i];
+      ++e;
+    }
+  }
+}
+
+// one coarse row's span [e0, e0 + E), E <= 64 R, held in R registers per
+// lane: distinct columns ascending, each summed; the results (column, sum)
+// are written over the span's start and the count returned
+template <int R>
+__device__ inline int gal_row_regs(int I, int e0, int E, int *ej, double *ea, double *diag) {
+  const int lane = threadIdx.x;
+  int J[R];
+  double A[R];
+  int mn = INT_MAX;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int e = lane + 64 * r;
+    J[r] = (e < E) ? ej[e0 + e] : INT_MAX;
+    A[r] = (e < E) ? ea[e0 + e] : 0.0;
+    mn = min(mn, J[r]);
+  }
+  // every lane's loads have completed here (the minimum needs them), so the
+  // writes over the span below cannot overtake a read of it
+  int cur = wave_min_int(mn);
+  int u = 0;
+  while (cur != INT_MAX) {
+    double sum = 0.0;
+    int nxt = INT_MAX;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (J[r] == cur) sum += A[r];
+      else if (J[r] > cur) nxt = min(nxt, J[r]);
+    }
+    sum = wave_sum(sum);
+    if (lane == 0) {
+      ej[e0 + u] = cur;
+      ea[e0 + u] = sum;
+      if (cur == I) diag[I] = sum;
+    }
+    ++u;
+    cur = wave_min_int(nxt);
+  }
+  return u;
+}
+
+// a span longer than GAL_CAP, streamed once per distinct column; pass 1
+// writes (column, sum) at o
+__device__ inline int gal_row_stream(int I, int e0, int E, const int *ej, const double *ea,
+                                     bool write, int o, int *col, double *val, double *diag) {
+  const int lane = threadIdx.x;
+  int mn = INT_MAX;
+  for (int e = lane; e < E; e += 64) mn = min(mn, ej[e0 + e]);
+  int cur = wave_min_int(mn);
+  int u = 0;
+  while (cur != INT_MAX) {
+    double sum = 0.0;
+    int nxt = INT_MAX;
+    for (int e = lane; e < E; e += 64) {
+      const int j = ej[e0 + e];
+      if (j == cur) sum += ea[e0 + e];
+      else if (j > cur) nxt = min(nxt, j);
+    }
+    sum = wave_sum(sum);
+    if (write && lane == 0) {
+      col[o + u] = cur;
+      val[o + u] = sum;
+      if (cur == I) diag[I] = sum;
+    }
+    ++u;
+    cur = wave_min_int(nxt);
+  }
+  return u;
+}
+
+// ej, ea: the emitted entries, overwritten by pass 0 (not __restrict__: read
+// and written through the same pointers)
 __global__ void __launch_bounds__(64) k_gal_rows(int nc, const int *__restrict__ mp,
-                                                 const int *__restrict__ off_ml,
-                                                 const int *__restrict__ ej,
-                                                 const double *__restrict__ ea, int pass,
-                                                 int *__restrict__ ucnt, const int *__restrict__ rp,
+                                                 const int *__restrict__ off_ml, int *ej,
+                                                 double *ea, int pass, int *__restrict__ ucnt,
+                                                 const int *__restrict__ rp,
                                                  int *__restrict__ col, double *__restrict__ val,
                                                  double *__restrict__ diag) {
   const int lane = threadIdx.x;
   for (int I = blockIdx.x; I < nc; I += gridDim.x) {
     const int e0 = off_ml[mp[I]], e1 = off_ml[mp[I + 1]];
     const int E = e1 - e0;
-    const int o = (pass == 1) ? rp[I] : 0;
-    int u = 0;
-    auto emit = [&](int cur, double sum) {
-      if (pass == 1 && lane == 0) {
-        col[o + u] = cur;
-        val[o + u] = sum;
-        if (cur == I) diag[I] = sum;
-      }
-      ++u;
-    };
-    if (E <= GAL_CAP) {
-      int J[GAL_R];
-      double A[GAL_R];
-      int mn = INT_MAX;
-#pragma unroll
-      for (int r = 0; r < GAL_R; ++r) {
-        const int e = lane + 64 * r;
-        J[r] = (e < E) ? ej[e0 + e] : INT_MAX;
-        A[r] = (e < E) ? ea[e0 + e] : 0.0;
-        mn = min(mn, J[r]);
-      }
-      int cur = wave_min_int(mn);
-      while (cur != INT_MAX) {
-        double sum = 0.0;
-        int nxt = INT_MAX;
-#pragma unroll
-        for (int r = 0; r < GAL_R; ++r) {
-          if (J[r] == cur) sum += A[r];
-          else if (J[r] > cur) nxt = min(nxt, J[r]);
+    if (pass == 0) {
+      int u;
+      if (E <= 128) u = gal_row_regs<2>(I, e0, E, ej, ea, diag);
+      else if (E <= 256) u = gal_row_regs<4>(I, e0, E, ej, ea, diag);
+      else if (E <= 384) u = gal_row_regs<6>(I, e0, E, ej, ea, diag);
+      else if (E <= 512) u = gal_row_regs<8>(I, e0, E, ej, ea, diag);
+      else if (E <= GAL_CAP) u = gal_row_regs<GAL_R>(I, e0, E, ej, ea, diag);
+      else u = gal_row_stream(I, e0, E, ej, ea, false, 0, nullptr, nullptr, nullptr);
+      if (lane == 0) ucnt[I] = u;
+    } else {
+      const int o = rp[I];
+      if (E <= GAL_CAP) {
+        const int u = rp[I + 1] - o;
+        for (int q = lane; q < u; q += 64) {
+          col[o + q] = ej[e0 + q];
+          val[o + q] = ea[e0 + q];
         }
-        sum = wave_sum(sum);
-        emit(cur, sum);
-        cur = wave_min_int(nxt);
-      }
-    } else {  // long span: stream it for every distinct column
-      int mn = INT_MAX;
-      for (int e = lane; e < E; e += 64) mn = min(mn, ej[e0 + e]);
-      int cur = wave_min_int(mn);
-      while (cur != INT_MAX) {
-        double sum = 0.0;
-        int nxt = INT_MAX;
-        for (int e = lane; e < E; e += 64) {
-          const int j = ej[e0 + e];
-          if (j == cur) sum += ea[e0 + e];
-          else if (j > cur) nxt = min(nxt, j);
-        }
-        sum = wave_sum(sum);
-        emit(cur, sum);
-        cur = wave_min_int(nxt);
+      } else {
+        (void)gal_row_stream(I, e0, E, ej, ea, true, o, col, val, diag);
       }
     }
-    if (pass == 0 && lane == 0) ucnt[I] = u;
   }
 }
 
@@ -801,12 +929,18 @@ static int galerkin_rows(apc_ctx *c, AmgHier *h, const V &v, AmgLevel *Lf, int n
   APC_ALLOC(c, h->gcol, (size_t)ne + 1);
   APC_ALLOC(c, h->gval, (size_t)ne + 1);
   hipLaunchKernelGGL(k_gal_rowoff, dim3(g), dim3(256), 0, c->s, n, Lf->ml.p, h->pos.p, h->sa.p);
-  hipLaunchKernelGGL((k_gal_emit<V>), dim3(g), dim3(256), 0, c->s, v, n, Lf->agg.p, h->sa.p,
-                     h->gcol.p, h->gval.p);
+  if constexpr (std::is_same<V, EllView>::value)
+    hipLaunchKernelGGL(k_gal_emit_ell, dim3(g), dim3(256), 0, c->s, v.A, Lf->agg.p, h->sa.p,
+                       h->gcol.p, h->gval.p);
+  else
+    hipLaunchKernelGGL((k_gal_emit<V>), dim3(g), dim3(256), 0, c->s, v, n, Lf->agg.p, h->sa.p,
+                       h->gcol.p, h->gval.p);
   const unsigned grid = (unsigned)std::min(nc, 1 << 16);
+  APC_ALLOC(c, h->diag, nc + 1);
+  APC_HIP(c, hipMemsetAsync(h->diag.p, 0, sizeof(double) * nc, c->s));  // both passes write it
   hipLaunchKernelGGL(k_gal_rows, dim3(grid), dim3(64), 0, c->s, nc, Lf->mp.p, h->pos.p, h->gcol.p,
                      h->gval.p, 0, h->flag.p, (const int *)nullptr, (int *)nullptr,
-                     (double *)nullptr, (double *)nullptr);
+                     (double *)nullptr, h->diag.p);
   APC_ALLOC(c, h->sb, (size_t)nc + 1);
   r = scan_exclusive(c, h->flag.p, h->sb.p, nc, h->cnt.p + 1);
   if (r) return r;
@@ -819,10 +953,8 @@ static int galerkin_rows(apc_ctx *c, AmgHier *h, const V &v, AmgLevel *Lf, int n
   APC_ALLOC(c, Lc->col, nnz + 1);
   APC_ALLOC(c, Lc->val, nnz + 1);
   APC_ALLOC(c, Lc->dinv, nc + 1);
-  APC_ALLOC(c, h->diag, nc + 1);
   APC_HIP(c, hipMemcpyAsync(Lc->rp.p, h->sb.p, sizeof(int) * nc, hipMemcpyDeviceToDevice, c->s));
   APC_HIP(c, hipMemcpyAsync(Lc->rp.p + nc, h->cnt.p + 1, sizeof(int), hipMemcpyDeviceToDevice, c->s));
-  APC_HIP(c, hipMemsetAsync(h->diag.p, 0, sizeof(double) * nc, c->s));
   hipLaunchKernelGGL(k_gal_rows, dim3(grid), dim3(64), 0, c->s, nc, Lf->mp.p, h->pos.p, h->gcol.p,
                      h->gval.p, 1, (int *)nullptr, (const int *)Lc->rp.p, Lc->col.p, Lc->val.p,
                      h->diag.p);
