@@ -2,12 +2,14 @@
 // Hessian fit (solver.cpp:295-362), one thread per computational/vacuum row.
 //
 // The reference solves each row's lm x 9 weighted least-squares problem with
-// LAPACKE_dgels (QR).  Here every row streams its lm weighted basis rows
-// through Givens rotations into the 9x9 triangular factor R (kept in
-// registers), so B^T B = R^T R without forming the normal matrix:
+// LAPACKE_dgels (QR).  Here every row keeps a 9x9 triangular factor R with
+// B^T B = R^T R in registers:
 //   Laplacian weights  s_j = w_j^2 * basis_j . y,  y = R^-1 R^-T (e3+e4+e5)
-//   (the reference's X3+X4+X5 column sums of X = (WB)^+ W), and
-//   gradient/Hessian   c = R^-1 Q^T b with b carried as a tenth column.
+//   (the reference's X3+X4+X5 column sums of X = (WB)^+ W), R from the
+//   Cholesky factorisation of the Gram matrix B^T B; and
+//   gradient/Hessian   c = R^-1 Q^T b, R and Q^T b from streaming the lm
+//   weighted basis rows, b carried as a tenth column, through Givens
+//   rotations (a least-squares solution, where QR is the accurate form).
 // Matrices are stored slot-major (slot j of all rows contiguous) so the
 // SpMV and the stencil writes coalesce.
 #include "apc_gfd.h"
@@ -27,7 +29,12 @@ __global__ void __launch_bounds__(256) k_stencil(
   if (i >= M) return;
   const double xi = cx[i], yi = cy[i], zi = cz[i], h = ch[i];
   const double maxdis = row_maxdis(dist, M, i, lm);
-  double R[45];
+  // The weights need only y = (B^T B)^-1 u (B: the weighted basis rows), and
+  // a solve with B^T B has error of order cond(B)^2 u through any factor of
+  // it, so R comes from the Cholesky factorisation of the Gram matrix rather
+  // than from a QR of B: nine square roots per row instead of a rotation
+  // (square root and division) per basis entry.
+  double R[45];  // the Gram matrix (upper, packed), then its factor in place
 #pragma unroll
   for (int q = 0; q < 45; ++q) R[q] = 0.0;
   for (int j = 0; j < lm; ++j) {
@@ -38,11 +45,29 @@ __global__ void __launch_bounds__(256) k_stencil(
               pwrap_s(cz[m] - zi, G.len[2], G.bc) / h, r);
 #pragma unroll
     for (int k = 0; k < 9; ++k) r[k] *= w;
-    givens_add<false>(R, nullptr, r, 0.0);
-  }
-  bool deficient = false;
 #pragma unroll
-  for (int k = 0; k < 9; ++k) deficient |= (R[RI(k, k)] == 0.0);
+    for (int k = 0; k < 9; ++k)
+#pragma unroll
+      for (int q = k; q < 9; ++q) R[RI(k, q)] += r[k] * r[q];
+  }
+  bool deficient = false;  // a pivot that is not positive: B has no full column rank
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    double d = R[RI(k, k)];
+#pragma unroll
+    for (int q = 0; q < k; ++q) d -= R[RI(q, k)] * R[RI(q, k)];
+    deficient |= !(d > 0.0);
+    const double rk = sqrt(fmax(d, 0.0));
+    R[RI(k, k)] = rk;
+    const double inv = 1.0 / rk;
+#pragma unroll
+    for (int m2 = k + 1; m2 < 9; ++m2) {
+      double e = R[RI(k, m2)];
+#pragma unroll
+      for (int q = 0; q < k; ++q) e -= R[RI(q, k)] * R[RI(q, m2)];
+      R[RI(k, m2)] = e * inv;
+    }
+  }
   if (deficient) { status[0] = APC_ERR_RANK; status[1] = i; return; }
   // y = R^-1 R^-T u, u = e3+e4+e5
   double z[9];
