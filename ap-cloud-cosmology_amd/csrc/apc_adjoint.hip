@@ -255,7 +255,8 @@ int build_transpose(apc_ctx *c) {
 }
 
 // Transpose of Differentiate (solver.cpp:295-362): for row i the fit is
-// cf = R^-1 R^-T B^T W^2 (v - phi_i) c_field with R from the QR of W B, g =
+// cf = R^-1 R^-T B^T W^2 (v - phi_i) c_field with R^T R = (W B)^T (W B) (the
+// Cholesky factor the forward kernel k_gradient uses), g =
 // cf[0..2]/h, H = cf[3..8]/h^2.  Given dG = (dg, dH) per row, z = R^-1 R^-T
 // dcf and each neighbour j receives c_field w_j^2 (b_j . z); the row's own
 // phi_i receives minus their sum.  Scattered with atomics.
@@ -268,11 +269,9 @@ __global__ void __launch_bounds__(256) k_grad_adjoint(
   if (i >= M) return;
   const double xi = cx[i], yi = cy[i], zi = cz[i], h = ch[i];
   const double maxdis = row_maxdis(dist, M, i, lm);
-  double R[45], Qtb[9];
+  double R[45];
 #pragma unroll
   for (int q = 0; q < 45; ++q) R[q] = 0.0;
-#pragma unroll
-  for (int q = 0; q < 9; ++q) Qtb[q] = 0.0;
   for (int j = 0; j < lm; ++j) {
     int m = nbr[(size_t)j * M + i];
     double w = weight_of(dist[(size_t)j * M + i], maxdis);
@@ -281,12 +280,9 @@ __global__ void __launch_bounds__(256) k_grad_adjoint(
               pwrap_s(cz[m] - zi, G.len[2], G.bc) / h, r);
 #pragma unroll
     for (int k = 0; k < 9; ++k) r[k] *= w;
-    givens_add<false>(R, Qtb, r, 0.0);
+    gram_add(R, r);
   }
-  bool deficient = false;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) deficient |= (R[RI(k, k)] == 0.0);
-  if (deficient) { status[0] = APC_ERR_RANK; status[1] = i; return; }
+  if (chol_inplace(R)) { status[0] = APC_ERR_RANK; status[1] = i; return; }
   double dcf[9];
   dcf[0] = dG[i] / h;
   dcf[1] = dG[(size_t)M + i] / h;

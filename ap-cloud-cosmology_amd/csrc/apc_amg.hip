@@ -768,8 +768,10 @@ __global__ void __launch_bounds__(64) k_cy_dense(int n, const T *__restrict__ in
 // ---- small coarse levels: G lanes per row (per coarse row for the
 // restriction), so each lane's chain of dependent gathers is G times shorter;
 // the lanes' partial sums meet in a fixed shuffle tree (deterministic)
-template <class T, int G>
-__device__ inline double sell_apply_g(const Sell<T> &A, const T *__restrict__ x, int i, int sub) {
+// (A x)_i with G lanes per row; xf(j) gives x_j (a load, or a value formed
+// from loads at j)
+template <class T, int G, class XF>
+__device__ inline double sell_apply_gx(const Sell<T> &A, XF xf, int i, int sub) {
   const int sl = i >> 6;
   const long long base = (long long)A.soff[sl] * 64 + (i & 63);
   const int w = A.wid[sl];
@@ -787,17 +789,43 @@ __device__ inline double sell_apply_g(const Sell<T> &A, const T *__restrict__ x,
       av[u] = A.val[e];
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) xv[u] = x[cj[u]];
+    for (int u = 0; u < 4; ++u) xv[u] = xf(cj[u]);
 #pragma unroll
     for (int u = 0; u < 4; ++u) acc += (double)av[u] * (double)xv[u];
   }
   for (; k < w; k += G) {
     const long long e = base + (long long)k * 64;
-    acc += (double)A.val[e] * (double)x[A.col[e]];
+    acc += (double)A.val[e] * (double)xf(A.col[e]);
   }
 #pragma unroll
   for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
   return acc;
+}
+
+template <class T, int G>
+__device__ inline double sell_apply_g(const Sell<T> &A, const T *__restrict__ x, int i, int sub) {
+  return sell_apply_gx<T, G>(A, [&](int j) { return x[j]; }, i, sub);
+}
+
+// The first smoothing step and the residual after it, in one launch: x =
+// omega D^-1 b and r = b - A x, each neighbour's x_j formed from b_j and
+// dinv_j exactly as the first step stores it (so r is the same as from the
+// two launches), the row's own x_i stored for the correction that follows.
+template <class T, int G>
+__global__ void __launch_bounds__(256) k_cy_first_resid_g(Sell<T> C, const T *__restrict__ dinv,
+                                                          const T *__restrict__ b,
+                                                          T *__restrict__ x, T *__restrict__ r,
+                                                          double omega, const GState *st) {
+  if (st && halted(st)) return;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int i = t / G, sub = t % G;
+  if (i >= C.n) return;  // whole groups (G divides 64)
+  auto xf = [&](int j) { return (T)(omega * (double)dinv[j] * (double)b[j]); };
+  const double ax = sell_apply_gx<T, G>(C, xf, i, sub);
+  if (sub == 0) {
+    x[i] = xf(i);
+    r[i] = (T)((double)b[i] - ax);
+  }
 }
 
 template <class T, int G>
@@ -1278,7 +1306,10 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
   // smoothing writes alternate between tmp and x; with 2*nu - 1 sweeps after
   // the first (odd), starting in tmp ends in x
   T *cur = tmp, *alt = x;
-  {
+  // coarse levels with one pre-smoothing step: the first step is formed
+  // inside the first residual (k_cy_first_resid_g), one launch fewer
+  const bool fuse_first = !ell && gl > 1 && o.nu == 1;
+  if (!fuse_first) {
     KScope ks(c, ell ? "amg_first0" : "amg_first_coarse");
     hipLaunchKernelGGL((k_cy_first<T>), dim3(g), dim3(256), 0, c->s, n, dinv, b, cur, o.omega, st);
   }
@@ -1294,6 +1325,14 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
     if (ell) {
       KScope ks(c, f32 ? "amg_resid0_32" : "amg_resid0", (double)n * (lm0 * (4 + 2 * sT) + 5 * sT));
       hipLaunchKernelGGL((k_cy_resid<T, true>), dim3(g), dim3(256), 0, c->s, E, C, b, cur, r, st);
+    } else if (fuse_first && q == 0) {
+      KScope ks(c, kResidName[l]);
+      if (gl == 8)
+        hipLaunchKernelGGL((k_cy_first_resid_g<T, 8>), dim3(nblocks((long)n * 8, 256)), dim3(256),
+                           0, c->s, C, dinv, b, cur, r, o.omega, st);
+      else
+        hipLaunchKernelGGL((k_cy_first_resid_g<T, 4>), dim3(nblocks((long)n * 4, 256)), dim3(256),
+                           0, c->s, C, dinv, b, cur, r, o.omega, st);
     } else if (gl == 8) {
       KScope ks(c, kResidName[l]);
       hipLaunchKernelGGL((k_cy_resid_g<T, 8>), dim3(nblocks((long)n * 8, 256)), dim3(256), 0, c->s,

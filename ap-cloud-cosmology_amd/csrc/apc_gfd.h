@@ -28,28 +28,55 @@ __device__ inline void basis_row(double p1, double p2, double p3, double *r) {
   r[6] = p1 * p2; r[7] = p1 * p3; r[8] = p2 * p3;
 }
 
-// Rotate row r (with optional RHS rb) into R (and Qtb).
-template <bool RHS>
-__device__ inline void givens_add(double *R, double *Qtb, double *r, double rb) {
+// Add the outer product r r^T to the Gram matrix R (upper, packed as RI).
+__device__ inline void gram_add(double *R, const double *r) {
+#pragma unroll
+  for (int k = 0; k < 9; ++k)
+#pragma unroll
+    for (int q = k; q < 9; ++q) R[RI(k, q)] += r[k] * r[q];
+}
+
+// Cholesky factorisation in place: the Gram matrix B^T B becomes the upper
+// triangular R with R^T R = B^T B.  Returns true if a pivot is not positive
+// (B has no full column rank).
+__device__ inline bool chol_inplace(double *R) {
+  bool deficient = false;
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    double a = R[RI(k, k)], b = r[k];
-    if (b != 0.0) {
-      double rr = sqrt(a * a + b * b);
-      double cs = a / rr, sn = b / rr;
-      R[RI(k, k)] = rr;
+    double d = R[RI(k, k)];
 #pragma unroll
-      for (int m = k + 1; m < 9; ++m) {
-        double u = R[RI(k, m)], v = r[m];
-        R[RI(k, m)] = cs * u + sn * v;
-        r[m] = cs * v - sn * u;
-      }
-      if (RHS) {
-        double u = Qtb[k];
-        Qtb[k] = cs * u + sn * rb;
-        rb = cs * rb - sn * u;
-      }
+    for (int q = 0; q < k; ++q) d -= R[RI(q, k)] * R[RI(q, k)];
+    deficient |= !(d > 0.0);
+    const double rk = sqrt(fmax(d, 0.0));
+    R[RI(k, k)] = rk;
+    const double inv = 1.0 / rk;
+#pragma unroll
+    for (int m = k + 1; m < 9; ++m) {
+      double e = R[RI(k, m)];
+#pragma unroll
+      for (int q = 0; q < k; ++q) e -= R[RI(q, k)] * R[RI(q, m)];
+      R[RI(k, m)] = e * inv;
     }
+  }
+  return deficient;
+}
+
+// z = R^-1 R^-T u (the semi-normal solve (B^T B) z = u)
+__device__ inline void seminormal_solve(const double *R, const double *u, double *z) {
+  double y[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    double s = u[k];
+#pragma unroll
+    for (int q = 0; q < k; ++q) s -= R[RI(q, k)] * y[q];
+    y[k] = s / R[RI(k, k)];
+  }
+#pragma unroll
+  for (int k = 8; k >= 0; --k) {
+    double s = y[k];
+#pragma unroll
+    for (int q = k + 1; q < 9; ++q) s -= R[RI(k, q)] * z[q];
+    z[k] = s / R[RI(k, k)];
   }
 }
 

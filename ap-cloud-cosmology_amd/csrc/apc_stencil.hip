@@ -5,11 +5,11 @@
 // LAPACKE_dgels (QR).  Here every row keeps a 9x9 triangular factor R with
 // B^T B = R^T R in registers:
 //   Laplacian weights  s_j = w_j^2 * basis_j . y,  y = R^-1 R^-T (e3+e4+e5)
-//   (the reference's X3+X4+X5 column sums of X = (WB)^+ W), R from the
-//   Cholesky factorisation of the Gram matrix B^T B; and
-//   gradient/Hessian   c = R^-1 Q^T b, R and Q^T b from streaming the lm
-//   weighted basis rows, b carried as a tenth column, through Givens
-//   rotations (a least-squares solution, where QR is the accurate form).
+//   (the reference's X3+X4+X5 column sums of X = (WB)^+ W), and
+//   gradient/Hessian   c = R^-1 R^-T B^T b (the semi-normal equations),
+//   R from the Cholesky factorisation of the Gram matrix B^T B, accumulated
+//   in one pass over the neighbours.  Both agree with the reference's QR to
+//   rounding of order cond(B)^2 u.
 // Matrices are stored slot-major (slot j of all rows contiguous) so the
 // SpMV and the stencil writes coalesce.
 #include "apc_gfd.h"
@@ -45,30 +45,9 @@ __global__ void __launch_bounds__(256) k_stencil(
               pwrap_s(cz[m] - zi, G.len[2], G.bc) / h, r);
 #pragma unroll
     for (int k = 0; k < 9; ++k) r[k] *= w;
-#pragma unroll
-    for (int k = 0; k < 9; ++k)
-#pragma unroll
-      for (int q = k; q < 9; ++q) R[RI(k, q)] += r[k] * r[q];
+    gram_add(R, r);
   }
-  bool deficient = false;  // a pivot that is not positive: B has no full column rank
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    double d = R[RI(k, k)];
-#pragma unroll
-    for (int q = 0; q < k; ++q) d -= R[RI(q, k)] * R[RI(q, k)];
-    deficient |= !(d > 0.0);
-    const double rk = sqrt(fmax(d, 0.0));
-    R[RI(k, k)] = rk;
-    const double inv = 1.0 / rk;
-#pragma unroll
-    for (int m2 = k + 1; m2 < 9; ++m2) {
-      double e = R[RI(k, m2)];
-#pragma unroll
-      for (int q = 0; q < k; ++q) e -= R[RI(q, k)] * R[RI(q, m2)];
-      R[RI(k, m2)] = e * inv;
-    }
-  }
-  if (deficient) { status[0] = APC_ERR_RANK; status[1] = i; return; }
+  if (chol_inplace(R)) { status[0] = APC_ERR_RANK; status[1] = i; return; }
   // y = R^-1 R^-T u, u = e3+e4+e5
   double z[9];
 #pragma unroll
@@ -118,10 +97,7 @@ __global__ void __launch_bounds__(256) k_stencil(
   pbnd[i] = pinned ? 0.0 : p3;
 }
 
-// Two waves per SIMD (measured at 256^3: 8.7 ms against 12.5 at one wave with
-// no spill; three or four waves spill the factor and are slower, as is the
-// same hint on k_stencil).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_gradient(
+__global__ void __launch_bounds__(256) k_gradient(
     int M, int lm, Geo G, double c_field, const double *__restrict__ cx,
     const double *__restrict__ cy, const double *__restrict__ cz, const double *__restrict__ ch,
     const int *__restrict__ nbr, const double *__restrict__ dist, const double *__restrict__ phi,
@@ -131,11 +107,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   const double xi = cx[i], yi = cy[i], zi = cz[i], h = ch[i];
   const double maxdis = row_maxdis(dist, M, i, lm);
   const double p4 = phi[i];
-  double R[45], Qtb[9];
+  // cf = (B^T B)^-1 B^T b (B: the weighted basis rows, b: the weighted
+  // differences), by the semi-normal equations with R from the Cholesky
+  // factorisation of the Gram matrix: one pass over the neighbours, nine
+  // square roots per row.  The adjoint (k_grad_adjoint) transposes exactly
+  // this form.
+  double R[45], Btb[9];
 #pragma unroll
   for (int q = 0; q < 45; ++q) R[q] = 0.0;
 #pragma unroll
-  for (int q = 0; q < 9; ++q) Qtb[q] = 0.0;
+  for (int q = 0; q < 9; ++q) Btb[q] = 0.0;
   for (int j = 0; j < lm; ++j) {
     int m = nbr[(size_t)j * M + i];
     double w = weight_of(dist[(size_t)j * M + i], maxdis);
@@ -146,20 +127,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     for (int k = 0; k < 9; ++k) r[k] *= w;
     double v = (m < M) ? phi[m] : bv[m - M];
     double rb = w * (v - p4) * c_field;
-    givens_add<true>(R, Qtb, r, rb);
-  }
-  bool deficient = false;
+    gram_add(R, r);
 #pragma unroll
-  for (int k = 0; k < 9; ++k) deficient |= (R[RI(k, k)] == 0.0);
-  if (deficient) { status[0] = APC_ERR_RANK; status[1] = i; return; }
+    for (int k = 0; k < 9; ++k) Btb[k] += r[k] * rb;
+  }
+  if (chol_inplace(R)) { status[0] = APC_ERR_RANK; status[1] = i; return; }
   double cf[9];
-#pragma unroll
-  for (int k = 8; k >= 0; --k) {
-    double s = Qtb[k];
-#pragma unroll
-    for (int q = k + 1; q < 9; ++q) s -= R[RI(k, q)] * cf[q];
-    cf[k] = s / R[RI(k, k)];
-  }
+  seminormal_solve(R, Btb, cf);
   grad[i] = cf[0] / h;
   grad[(size_t)M + i] = cf[1] / h;
   grad[2 * (size_t)M + i] = cf[2] / h;
