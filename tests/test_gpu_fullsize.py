@@ -12,6 +12,13 @@ Rows are sampled at random.
   every quadratic, with zero first and mixed moments and unit pure second
   moments.  A wrong index, sign, scale or basis entry breaks it.
 * The density and Poisson rows carry the same weights (s/24 and s/h^2).
+* The solves (solver.cpp:182-293) return solutions of the systems they
+  built, recomputed here on every row.  For the Poisson solve (FP32
+  multigrid in FP64 flexible GMRES, as the bench runs it, right
+  preconditioned) the true residual is within the tolerance.  The density
+  solve (Jacobi, left preconditioned) meets it on the Jacobi-scaled residual.
+  The Poisson right-hand side is the density minus its volume-weighted mean,
+  times C_pot (solver.cpp:233-261).
 """
 import numpy as np
 import pytest
@@ -22,6 +29,7 @@ from apcloud import ic
 pytestmark = pytest.mark.gpu
 
 SAMPLE = 100_000
+RTOL = 1e-8
 
 
 def _wrap(d):
@@ -47,11 +55,16 @@ def full():
     comp = g.comp()
     M = comp["nc"] + comp["nv"]
     print("rows", M, flush=True)
+    info = g.solve(apcloud.ksp(rtol=RTOL, restart=200, max_it=1000),
+                   apcloud.ksp(rtol=RTOL, restart=200, max_it=1000, pc="amg", precision=1))
+    fields = g.fields()
+    print("its", info.its_rho, info.its_phi, flush=True)
     rows = np.random.default_rng(0).choice(M, size=min(SAMPLE, M), replace=False)
-    out = dict(M=M, rows=rows, nbr=nbr[rows], dist=dist[rows], cols=sysm["cols"][rows],
-               a1=sysm["a1"][rows], a2=sysm["a2"][rows], x=comp["x"], y=comp["y"],
-               z=comp["z"], h=comp["h"])
-    del sysm, nbr, dist
+    out = dict(M=M, rows=rows, nbr=nbr[rows], dist=dist[rows], sys=sysm, info=info,
+               rho=fields["rho"], phi=fields["phi"], x=comp["x"], y=comp["y"], z=comp["z"],
+               h=comp["h"])
+    out["cols"], out["a1"], out["a2"] = sysm["cols"][rows], sysm["a1"][rows], sysm["a2"][rows]
+    del nbr, dist
     g.close()
     return out
 
@@ -91,3 +104,32 @@ def test_full_size_stencils_reproduce_quadratics(full):
     live = a2.any(axis=1)
     assert live.sum() >= len(rows) - 1
     np.testing.assert_allclose(a2[live] * hi[live] ** 2, s[live], rtol=1e-13, atol=0)
+
+
+def _apply(diag, off, cols, v):
+    """(A v)_i = diag_i v_i + sum_j off_ij v_cols_ij, in row blocks."""
+    out = np.empty_like(v)
+    for a in range(0, len(v), 1 << 20):
+        b = min(len(v), a + (1 << 20))
+        out[a:b] = diag[a:b] * v[a:b] + (off[a:b] * v[cols[a:b]]).sum(axis=1)
+    return out
+
+
+def test_full_size_solves_satisfy_their_systems(full):
+    S, info, h = full["sys"], full["info"], full["h"]
+    M = full["M"]
+    assert info.reason_rho > 0 and info.reason_phi > 0
+    cols = S["cols"]
+    assert cols.max() < M  # periodic: no boundary nodes
+    # density: Jacobi-scaled residual (the quantity the left-preconditioned solve tests)
+    rho = full["rho"]
+    r1 = S["rhs1"] - _apply(S["d1"], S["a1"], cols, rho)
+    rel1 = np.linalg.norm(r1 / S["d1"]) / np.linalg.norm(S["rhs1"] / S["d1"])
+    assert rel1 < 2 * RTOL, rel1
+    # Poisson: the true residual (right preconditioning tests it directly)
+    h3 = h[:M] ** 3
+    avg = (rho * h3).sum() / h3.sum()
+    rhs2 = (rho - avg) * (1.5 / 0.02)
+    r2 = rhs2 - _apply(S["d2"], S["a2"], cols, full["phi"])
+    rel2 = np.linalg.norm(r2) / np.linalg.norm(rhs2)
+    assert rel2 < 2 * RTOL, rel2
