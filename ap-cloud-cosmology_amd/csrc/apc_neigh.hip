@@ -20,6 +20,7 @@
 //    element-major (the scans run in lockstep and coalesce); the sort keys
 //    are thread-major, so a thread's partition scans stay in its own lines.
 #include <array>
+#include <climits>
 #include <cstdlib>
 
 #include "apc_internal.h"
@@ -338,6 +339,8 @@ struct WaveLds {
     int ht[WHASH];    // membership hash of the list's indices (-1 = empty)
     int stk[WHASH];   // pending sort ranges, lo | hi << 16; then per-lane stacks
   };
+  int own[WHASH];     // per hash slot: the tag (chunk << 6 | lane) of the earliest
+                      // candidate holding that index; 0 for the node's Neigh24
 };
 
 // 1e4^k by repeated multiplication from 1 (the reference's running penalty),
@@ -364,20 +367,14 @@ __device__ inline int wave_excl_scan_i(int v, int *total) {
 
 __device__ inline int hslot(uint32_t m) { return (int)((m * 0x9E3779B1u) >> 23) & (WHASH - 1); }
 
-__device__ inline bool ht_has(const int *ht, int m) {
-  for (int h = hslot((uint32_t)m);; h = (h + 1) & (WHASH - 1)) {
-    const int v = ht[h];
-    if (v == m) return true;
-    if (v < 0) return false;
-  }
-}
-
-// distinct keys per call (callers remove duplicates first); at most WCAP keys
-// in a table of 2 WCAP slots, so a free slot always exists
-__device__ inline void ht_put(int *ht, int m) {
+// the slot holding m, claimed (CAS from -1) if m is absent; lanes of one call
+// with equal m meet in one slot.  The table holds the list (at most WCAP
+// keys) and one chunk's claims (at most 64) in 2 WCAP slots, so a free slot
+// always exists.
+__device__ inline int ht_claim(int *ht, int m) {
   for (int h = hslot((uint32_t)m);; h = (h + 1) & (WHASH - 1)) {
     const int old = atomicCAS(&ht[h], -1, m);
-    if (old < 0 || old == m) return;
+    if (old < 0 || old == m) return h;
   }
 }
 
@@ -577,7 +574,10 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
   for (long k = blockIdx.x / nx; k < mine; k += per) {
     const int i0 = (int)(((k / xchunk) * nx + xcd) * xchunk + k % xchunk);
     if (i0 >= M) continue;  // the last chunk is short
-    for (int h = lane; h < WHASH; h += 64) L.ht[h] = -1;
+    for (int h = lane; h < WHASH; h += 64) {
+      L.ht[h] = -1;
+      L.own[h] = INT_MAX;
+    }
     const double4 p0 = xyz[i0];
     const double x0 = p0.x, y0 = p0.y, z0 = p0.z;
     int nn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -594,13 +594,8 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
         L.mo[lane] = m | (o << 29);
         L.d2[lane] = dd;
       }
-      // duplicates (if any) share a hash slot: insert the first of each
-      bool first = lane < cnt;
-      for (int t = 0; t < 24; ++t) {
-        const int mt = __shfl(m, t, 64);
-        if (t < lane && mt == m) first = false;
-      }
-      if (first) ht_put(L.ht, m);
+      // in the list (duplicates, if any, share a slot): owner 0, before every chunk
+      if (lane < cnt) L.own[ht_claim(L.ht, m)] = 0;
 #pragma unroll
       for (int q = 0; q < 8; ++q) nn[q] += __popcll(__ballot(o == q));
       l = cnt;
@@ -608,6 +603,7 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
     __syncthreads();
     // expansion through the neighbours' Neigh24 until >= lm (solver.cpp:495-551)
     int ite = 0, err = 0;
+    int ctag = 1;  // chunk counter of the node (owner tags; 0 is the Neigh24)
     bool overflow = false;
     while (l < lm) {
       int off[8];
@@ -649,15 +645,20 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
           m = n24[(size_t)L.snap[lo] * 24 + (t - coff[lo])];
         }
         const bool invalid = live && m >= S;
-        bool fresh = live && !invalid && m != i0 && !ht_has(L.ht, m);
-        // an earlier candidate of this chunk with the same index: only the
-        // candidates still new can matter (an equal earlier one that is not
-        // new makes this one not new either)
-        for (uint64_t mk = __ballot(fresh); mk; mk &= mk - 1) {
-          const int u = __ffsll((unsigned long long)mk) - 1;
-          const int mu = __shfl(m, u, 64);
-          if (u < lane && mu == m) fresh = false;
+        // new to the list: the index is in no earlier chunk (nor the node's
+        // Neigh24) and no earlier lane of this chunk holds it.  Every
+        // candidate claims its index's slot and offers its tag to the slot's
+        // owner word; the smallest tag is the earliest candidate, and a slot
+        // owned before this chunk holds a list member.
+        const bool cand = live && !invalid && m != i0;
+        const int tag = (ctag << 6) | lane;
+        int slot = 0;
+        if (cand) {
+          slot = ht_claim(L.ht, m);
+          atomicMin(&L.own[slot], tag);
         }
+        bool fresh = cand && L.own[slot] == tag;
+        ++ctag;
         int o = 8;
         double dd = 0.0;
         if (fresh) o = place_w(F, xyz, x0, y0, z0, m, &dd);
@@ -686,7 +687,6 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
         if (fresh) {
           L.mo[l + rank] = m | (o << 29);
           L.d2[l + rank] = dd;
-          ht_put(L.ht, m);
         }
 #pragma unroll
         for (int q = 0; q < 8; ++q) nn[q] += __popcll(__ballot(fresh && o == q));
