@@ -132,10 +132,11 @@ struct EllView {
   // every slot in slot order as f(j, a, ok): ok for an interior column j;
   // otherwise j = i (a valid index to gather from) and the slot is to be
   // ignored.  Slots are loaded nine at a time so that f's gathers at j do
-  // not wait on one another.
-  template <class F>
+  // not wait on one another.  VAL false: the values are not read (a = 0),
+  // for callers that use only the pattern.
+  template <bool VAL = true, class F>
   __device__ void offdiag_all(int i, F f) const {
-    const double fi = A.f[i];
+    const double fi = VAL ? A.f[i] : 0.0;
     constexpr int RG = 9;
     int q = 0;
     for (; q + RG <= A.lm; q += RG) {
@@ -144,7 +145,7 @@ struct EllView {
 #pragma unroll
       for (int u = 0; u < RG; ++u) {
         m[u] = A.nbr[(size_t)(q + u) * A.M + i];
-        sv[u] = A.s[(size_t)(q + u) * A.M + i];
+        sv[u] = VAL ? A.s[(size_t)(q + u) * A.M + i] : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < RG; ++u) {
@@ -155,7 +156,7 @@ struct EllView {
     for (; q < A.lm; ++q) {
       const int m = A.nbr[(size_t)q * A.M + i];
       const bool ok = m < A.M;
-      f(ok ? m : i, fi * A.s[(size_t)q * A.M + i], ok);
+      f(ok ? m : i, VAL ? fi * A.s[(size_t)q * A.M + i] : 0.0, ok);
     }
   }
 };
@@ -180,9 +181,9 @@ struct CsrView {
     for (int e = A.rp[i]; e < A.rp[i + 1]; ++e)
       if (A.col[e] != i) f(A.col[e], A.val[e]);
   }
-  template <class F>
+  template <bool VAL = true, class F>
   __device__ void offdiag_all(int i, F f) const {
-    for (int e = A.rp[i]; e < A.rp[i + 1]; ++e) f(A.col[e], A.val[e], A.col[e] != i);
+    for (int e = A.rp[i]; e < A.rp[i + 1]; ++e) f(A.col[e], VAL ? A.val[e] : 0.0, A.col[e] != i);
   }
 };
 
@@ -219,10 +220,12 @@ __global__ void __launch_bounds__(256) k_amg_pick(V v, double theta, const doubl
   const uint32_t pi = prio_of((uint32_t)i);
   const double lim = theta * smax[i];
   bool best = true;
-  v.offdiag_all(i, [&](int j, double x, bool ok) {
+  auto f = [&](int j, double x, bool ok) {
     const int aj = a[j];
     if (ok && fabs(x) >= lim && aj == 0 && prio_of((uint32_t)j) > pi) best = false;
-  });
+  };
+  if (theta > 0) v.template offdiag_all<true>(i, f);
+  else v.template offdiag_all<false>(i, f);  // every coupling is strong: pattern only
   b[i] = best ? 1 : 0;
 }
 
@@ -236,10 +239,12 @@ __global__ void __launch_bounds__(256) k_amg_cover(V v, double theta, const doub
   if (b[i] != 0) return;
   const double lim = theta * smax[i];
   bool cov = false;
-  v.offdiag_all(i, [&](int j, double x, bool ok) {
+  auto f = [&](int j, double x, bool ok) {
     const int bj = b[j];
     if (ok && fabs(x) >= lim && bj == 1) cov = true;
-  });
+  };
+  if (theta > 0) v.template offdiag_all<true>(i, f);
+  else v.template offdiag_all<false>(i, f);  // every coupling is strong: pattern only
   if (cov) b[i] = 2;
   else atomicAdd(undecided, 1);
 }
