@@ -156,11 +156,12 @@ int apc_destroy(apc_ctx *c) {
                           &c->cnp, &c->cfirst, &c->ctype, &c->n24, &c->n24cnt, &c->n24first,
                           &c->owner, &c->nbr, &c->nscratch_i, &c->nscratch_f, &c->novf,
                           &c->status_dev, &c->tr_pos, &c->tr_pos2, &c->tr_rp, &c->cic_v,
-                          &c->cic_v2, &c->cic_rng};
+                          &c->cic_v2, &c->cic_rng, &c->vnode, &c->vnode2};
   for (auto b : is) b->release();
   for (auto &b : c->ckpt) b.release(c->s);
   c->key.release(c->s); c->key2.release(c->s); c->nprefix.release(c->s); c->red_u64.release(c->s);
   c->tr_key.release(c->s); c->tr_key2.release(c->s); c->cic_key.release(c->s); c->cic_key2.release(c->s);
+  c->vcode.release(c->s); c->vcode2.release(c->s);
   hipStreamDestroy(c->s);
   delete c;
   return APC_OK;

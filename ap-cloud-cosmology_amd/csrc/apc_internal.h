@@ -106,6 +106,8 @@ struct apc_ctx {
   apc::DBuf<int> nscratch_i;
   apc::DBuf<int> nscratch_f;   // candidate fingerprints
   apc::DBuf<int> novf;         // nodes searched again with the full list bound
+  apc::DBuf<uint64_t> vcode, vcode2;  // vacuum leaves sorted by (level, Morton)
+  apc::DBuf<int> vnode, vnode2;
   apc::DBuf<double> nxyz;      // node centres packed 4 doubles per node
   // differentiation form (apc_adjoint.hip): topology held fixed
   int frozen = 0, have_tr = 0;

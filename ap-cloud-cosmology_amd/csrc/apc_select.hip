@@ -550,22 +550,17 @@ int select_tree(apc_ctx *c, double cc, int order, int mindepth, int maxdepth) {
     r = read_ints(c, c->dcount.p, hc, 1);
     if (r) return r;
     nv = hc[0];
-    if (nv > 0) {
-      DBuf<uint64_t> code, code2;
-      DBuf<int> node, node2;
-      if (code.ensure(nv, c->s) || code2.ensure(nv, c->s) || node.ensure(nv, c->s) || node2.ensure(nv, c->s)) {
-        code.release(c->s); code2.release(c->s); node.release(c->s); node2.release(c->s);
-        return fail(c, APC_ERR_NOMEM, "select: vacuum sort");
-      }
+    if (nv > 0) {  // the sort buffers stay with the context (no allocation per step)
+      APC_ALLOC(c, c->vcode, nv);
+      APC_ALLOC(c, c->vcode2, nv);
+      APC_ALLOC(c, c->vnode, nv);
+      APC_ALLOC(c, c->vnode2, nv);
       hipLaunchKernelGGL(k_vac_codes, dim3(nblocks(m, 256)), dim3(256), 0, c->s, t, Ttd, T,
-                         c->nflag.p, c->nrank.p, code.p, node.p);
-      r = radix_sort_pairs(c, code.p, node.p, code2.p, node2.p, nv, 3 * p.depth + 1);
-      if (r == 0)
-        hipLaunchKernelGGL(k_vac_emit, dim3(nblocks(nv, 256)), dim3(256), 0, c->s, nv, nc, node.p,
-                           c->leaf_node.p, c->nout.p);
-      APC_HIP(c, hipStreamSynchronize(c->s));
-      code.release(c->s); code2.release(c->s); node.release(c->s); node2.release(c->s);
+                         c->nflag.p, c->nrank.p, c->vcode.p, c->vnode.p);
+      r = radix_sort_pairs(c, c->vcode.p, c->vnode.p, c->vcode2.p, c->vnode2.p, nv, 3 * p.depth + 1);
       if (r) return r;
+      hipLaunchKernelGGL(k_vac_emit, dim3(nblocks(nv, 256)), dim3(256), 0, c->s, nv, nc,
+                         c->vnode.p, c->leaf_node.p, c->nout.p);
     }
   }
   const int M = nc + nv;
