@@ -7,7 +7,7 @@
  * The serial / OpenMP split follows the reference, so this file is also the
  * CPU baseline that bench.py times.
  *
- * Deliberate differences from the reference (all documented in DESIGN.md):
+ * Deliberate differences from the reference:
  *  - sort by (key, index): one legal outcome of the reference's unstable
  *    std::sort on key alone (octree.cpp:148);
  *  - selection arrays are heap vectors that grow, so the nmax/-1 retry of

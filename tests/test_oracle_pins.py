@@ -200,8 +200,8 @@ def test_uniform_lattice_density_is_exact():
 
 
 def test_uniform_ball_potential():
-    # configs[0]-like: Dirichlet boundary with the analytic potential on the
-    # box faces, C_dens = m = 1/N, C_pot = 4 pi G
+    # the reference's uniform-sphere case: Dirichlet boundary with the analytic
+    # potential on the box faces, C_dens = m = 1/N, C_pot = 4 pi G
     n = 20000
     x, y, z = ic.uniform_ball(n, 0)
     o = _oracle(x, y, z, lm=27, bc=0, c=0.0, leaf_capacity=8, bv_kind=1, boundary_at_faces=1,
