@@ -230,23 +230,26 @@ __global__ void __launch_bounds__(256) k_amg_pick(V v, double theta, const doubl
 }
 
 // MIS round, part 2: an undecided row with a strong root neighbour is
-// covered; the rest are counted.
+// covered; the rest are counted, one atomic per wave.
 template <class V>
 __global__ void __launch_bounds__(256) k_amg_cover(V v, double theta, const double *__restrict__ smax,
                                                    int *__restrict__ b, int *__restrict__ undecided) {
   int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= v.n()) return;
-  if (b[i] != 0) return;
-  const double lim = theta * smax[i];
-  bool cov = false;
-  auto f = [&](int j, double x, bool ok) {
-    const int bj = b[j];
-    if (ok && fabs(x) >= lim && bj == 1) cov = true;
-  };
-  if (theta > 0) v.template offdiag_all<true>(i, f);
-  else v.template offdiag_all<false>(i, f);  // every coupling is strong: pattern only
-  if (cov) b[i] = 2;
-  else atomicAdd(undecided, 1);
+  bool und = false;
+  if (i < v.n() && b[i] == 0) {
+    const double lim = theta * smax[i];
+    bool cov = false;
+    auto f = [&](int j, double x, bool ok) {
+      const int bj = b[j];
+      if (ok && fabs(x) >= lim && bj == 1) cov = true;
+    };
+    if (theta > 0) v.template offdiag_all<true>(i, f);
+    else v.template offdiag_all<false>(i, f);  // every coupling is strong: pattern only
+    if (cov) b[i] = 2;
+    else und = true;
+  }
+  const uint64_t m = __ballot(und);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(undecided, __popcll(m));
 }
 
 __global__ void k_amg_rootflag(int n, const int *__restrict__ st, int *__restrict__ flag) {
