@@ -666,10 +666,12 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
         const uint64_t mf = __ballot(fresh);
         const int rank = __popcll(mf & lt);  // position among this chunk's new entries
         int before = 0;                      // same-octant new entries before it
+        int add[8];                          // new entries per octant (wave-uniform)
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
           const uint64_t mk = __ballot(fresh && o == q);
           if (o == q) before = nn[q] + __popcll(mk & lt);
+          add[q] = __popcll(mk);
         }
         // the reference's checks in its order: invalid index, octant full, list full
         int code = 0;
@@ -689,7 +691,7 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
           L.d2[l + rank] = dd;
         }
 #pragma unroll
-        for (int q = 0; q < 8; ++q) nn[q] += __popcll(__ballot(fresh && o == q));
+        for (int q = 0; q < 8; ++q) nn[q] += add[q];
         l += __popcll(mf);
         __syncthreads();
       }
