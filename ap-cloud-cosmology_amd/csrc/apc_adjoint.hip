@@ -257,7 +257,7 @@ int build_transpose(apc_ctx *c) {
 // Transpose of Differentiate (solver.cpp:295-362): for row i the fit is
 // cf = R^-1 R^-T B^T W^2 (v - phi_i) c_field with R^T R = (W B)^T (W B) (the
 // Cholesky factor the forward kernel k_gradient uses), g =
-// cf[0..2]/h, H = cf[3..8]/h^2.  Given dG = (dg, dH) per row, z = R^-1 R^-T
+// cf[0..2] (1/h), H = cf[3..8] (1/h)^2.  Given dG = (dg, dH) per row, z = R^-1 R^-T
 // dcf and each neighbour j receives c_field w_j^2 (b_j . z); the row's own
 // phi_i receives minus their sum.  Scattered with atomics.
 __global__ void __launch_bounds__(256) k_grad_adjoint(
@@ -267,28 +267,26 @@ __global__ void __launch_bounds__(256) k_grad_adjoint(
     double *__restrict__ dphi, int *__restrict__ status) {
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= M) return;
-  const double xi = cx[i], yi = cy[i], zi = cz[i], h = ch[i];
-  const double maxdis = row_maxdis(dist, M, i, lm);
+  const FitRow f = fit_row(cx, cy, cz, ch, dist, M, i, lm);
   double R[45];
 #pragma unroll
   for (int q = 0; q < 45; ++q) R[q] = 0.0;
-  for (int j = 0; j < lm; ++j) {
-    int m = nbr[(size_t)j * M + i];
-    double w = weight_of(dist[(size_t)j * M + i], maxdis);
-    double r[9];
-    basis_row(pwrap_s(cx[m] - xi, G.len[0], G.bc) / h, pwrap_s(cy[m] - yi, G.len[1], G.bc) / h,
-              pwrap_s(cz[m] - zi, G.len[2], G.bc) / h, r);
+  for_slots(M, lm, i, nbr, dist, cx, cy, cz, NoExtra{},
+            [&](int, int, double d, double x, double y, double z, double) {
+              double r[9];
+              const double w = fit_slot(G, f, d, x, y, z, r);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) r[k] *= w;
-    gram_add(R, r);
-  }
+              for (int k = 0; k < 9; ++k) r[k] *= w;
+              gram_add(R, r);
+            });
   if (chol_inplace(R)) { status[0] = APC_ERR_RANK; status[1] = i; return; }
+  const double ih2 = f.ih * f.ih;
   double dcf[9];
-  dcf[0] = dG[i] / h;
-  dcf[1] = dG[(size_t)M + i] / h;
-  dcf[2] = dG[2 * (size_t)M + i] / h;
+  dcf[0] = dG[i] * f.ih;
+  dcf[1] = dG[(size_t)M + i] * f.ih;
+  dcf[2] = dG[2 * (size_t)M + i] * f.ih;
 #pragma unroll
-  for (int q = 3; q < 9; ++q) dcf[q] = dG[(size_t)q * M + i] / h / h;
+  for (int q = 3; q < 9; ++q) dcf[q] = dG[(size_t)q * M + i] * ih2;
   // y = R^-T dcf (forward substitution), z = R^-1 y (back substitution)
   double yv[9], zv[9];
 #pragma unroll
@@ -306,19 +304,17 @@ __global__ void __launch_bounds__(256) k_grad_adjoint(
     zv[k] = s / R[RI(k, k)];
   }
   double self = 0.0;
-  for (int j = 0; j < lm; ++j) {
-    int m = nbr[(size_t)j * M + i];
-    double w = weight_of(dist[(size_t)j * M + i], maxdis);
-    double r[9];
-    basis_row(pwrap_s(cx[m] - xi, G.len[0], G.bc) / h, pwrap_s(cy[m] - yi, G.len[1], G.bc) / h,
-              pwrap_s(cz[m] - zi, G.len[2], G.bc) / h, r);
-    double bz = 0.0;
+  for_slots(M, lm, i, nbr, dist, cx, cy, cz, NoExtra{},
+            [&](int, int m, double d, double x, double y, double z, double) {
+              double r[9];
+              const double w = fit_slot(G, f, d, x, y, z, r);
+              double bz = 0.0;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) bz += r[k] * zv[k];
-    const double coef = c_field * w * w * bz;
-    if (m < M) atomicAdd(&dphi[m], coef);
-    self -= coef;
-  }
+              for (int k = 0; k < 9; ++k) bz += r[k] * zv[k];
+              const double coef = c_field * w * w * bz;
+              if (m < M) atomicAdd(&dphi[m], coef);
+              self -= coef;
+            });
   atomicAdd(&dphi[i], self);
 }
 

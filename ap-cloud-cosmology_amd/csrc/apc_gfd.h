@@ -6,10 +6,19 @@
 
 namespace apc {
 
+// Periodic minimum image of a coordinate difference: subtract len while x >=
+// len/2, then add it while x < -len/2.  A difference of two positions inside
+// the box lies in (-len, len), where one conditional step of each kind is the
+// whole loop; the loop runs only for anything farther out.
 __device__ inline double pwrap_s(double x, double len, int bc) {
-  if (bc == 1 && isfinite(x)) {  // a non-finite value would never leave the loop
-    while (x >= (0.5 * len)) x -= len;
-    while (x < -(0.5 * len)) x += len;
+  if (bc == 1) {
+    const double hl = 0.5 * len;
+    x = (x >= hl) ? x - len : x;
+    x = (x < -hl) ? x + len : x;
+    if (!(x >= -hl && x < hl) && isfinite(x)) {  // a non-finite value would never leave the loop
+      while (x >= hl) x -= len;
+      while (x < -hl) x += len;
+    }
   }
   return x;
 }
@@ -89,9 +98,75 @@ __device__ inline double row_maxdis(const double *__restrict__ dist, int M, int 
   }
   return maxdis * 0.7;
 }
-__device__ inline double weight_of(double d, double maxdis) {
-  return (exp(-d * d / maxdis / maxdis) - exp(-4.0)) / (1 - exp(-4.0));
+
+// Per-row constants of a fit: the row's position, 1/h and 1/maxdis^2.  The
+// reference divides by h and twice by maxdis for every neighbour; here each is
+// one product with a reciprocal taken once per row (a rounding-level
+// difference; a double division costs about ten instructions).
+struct FitRow {
+  double xi, yi, zi, ih, inv_md2;
+};
+
+__device__ inline FitRow fit_row(const double *__restrict__ cx, const double *__restrict__ cy,
+                                 const double *__restrict__ cz, const double *__restrict__ ch,
+                                 const double *__restrict__ dist, int M, int i, int lm) {
+  FitRow f;
+  f.xi = cx[i];
+  f.yi = cy[i];
+  f.zi = cz[i];
+  f.ih = 1.0 / ch[i];
+  const double md = row_maxdis(dist, M, i, lm);
+  f.inv_md2 = 1.0 / (md * md);
+  return f;
 }
+
+// w = (exp(-d^2/maxdis^2) - e^-4) / (1 - e^-4)
+__device__ inline double weight_of(double d, double inv_md2) {
+  constexpr double E4 = 0x1.2c155b8213cf4p-6;  // exp(-4) rounded to double
+  constexpr double INV_NORM = 1.0 / (1.0 - E4);
+  return (exp(-(d * d) * inv_md2) - E4) * INV_NORM;
+}
+
+// Neighbour slot of a fit: the weight and the unweighted basis row of the
+// wrapped offset (x_m - x_i) / h.
+__device__ inline double fit_slot(const Geo &G, const FitRow &f, double d, double x, double y,
+                                  double z, double *r) {
+  basis_row(pwrap_s(x - f.xi, G.len[0], G.bc) * f.ih, pwrap_s(y - f.yi, G.len[1], G.bc) * f.ih,
+            pwrap_s(z - f.zi, G.len[2], G.bc) * f.ih, r);
+  return weight_of(d, f.inv_md2);
+}
+
+// Visit the lm neighbour slots of row i in order, body(j, m, d, x, y, z, e)
+// with m = nbr_ij, d = dist_ij, (x, y, z) the position of m and e =
+// extra(m).  The loads run two slots ahead: slot j+2's index and distance and
+// slot j+1's gathers are in flight while slot j is computed, so a thread does
+// not wait for its chain of dependent loads (index, then gather) on every
+// slot.  Loads past the last slot repeat the last slot.
+template <class Extra, class Body>
+__device__ inline void for_slots(int M, int lm, int i, const int *__restrict__ nbr,
+                                 const double *__restrict__ dist, const double *__restrict__ cx,
+                                 const double *__restrict__ cy, const double *__restrict__ cz,
+                                 Extra &&extra, Body &&body) {
+  const int last = lm - 1;
+  auto at = [&](int j) { return (size_t)(j < last ? j : last) * M + i; };
+  int mC = nbr[at(0)];
+  double dC = dist[at(0)];
+  int mB = nbr[at(1)];
+  double dB = dist[at(1)];
+  double xC = cx[mC], yC = cy[mC], zC = cz[mC], eC = extra(mC);
+  for (int j = 0; j < lm; ++j) {
+    const int mA = nbr[at(j + 2)];
+    const double dA = dist[at(j + 2)];
+    const double xB = cx[mB], yB = cy[mB], zB = cz[mB], eB = extra(mB);
+    body(j, mC, dC, xC, yC, zC, eC);
+    mC = mB; dC = dB; xC = xB; yC = yB; zC = zB; eC = eB;
+    mB = mA; dB = dA;
+  }
+}
+
+struct NoExtra {
+  __device__ double operator()(int) const { return 0.0; }
+};
 
 inline Geo geo_of(apc_ctx *c) {
   Geo G;

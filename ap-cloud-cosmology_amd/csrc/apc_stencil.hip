@@ -27,8 +27,7 @@ __global__ void __launch_bounds__(256) k_stencil(
     double *__restrict__ pbnd, int *__restrict__ status) {
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= M) return;
-  const double xi = cx[i], yi = cy[i], zi = cz[i], h = ch[i];
-  const double maxdis = row_maxdis(dist, M, i, lm);
+  const FitRow f = fit_row(cx, cy, cz, ch, dist, M, i, lm);
   // The weights need only y = (B^T B)^-1 u (B: the weighted basis rows), and
   // a solve with B^T B has error of order cond(B)^2 u through any factor of
   // it, so R comes from the Cholesky factorisation of the Gram matrix rather
@@ -37,16 +36,14 @@ __global__ void __launch_bounds__(256) k_stencil(
   double R[45];  // the Gram matrix (upper, packed), then its factor in place
 #pragma unroll
   for (int q = 0; q < 45; ++q) R[q] = 0.0;
-  for (int j = 0; j < lm; ++j) {
-    int m = nbr[(size_t)j * M + i];
-    double w = weight_of(dist[(size_t)j * M + i], maxdis);
-    double r[9];
-    basis_row(pwrap_s(cx[m] - xi, G.len[0], G.bc) / h, pwrap_s(cy[m] - yi, G.len[1], G.bc) / h,
-              pwrap_s(cz[m] - zi, G.len[2], G.bc) / h, r);
+  for_slots(M, lm, i, nbr, dist, cx, cy, cz, NoExtra{},
+            [&](int, int, double d, double x, double y, double z, double) {
+              double r[9];
+              const double w = fit_slot(G, f, d, x, y, z, r);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) r[k] *= w;
-    gram_add(R, r);
-  }
+              for (int k = 0; k < 9; ++k) r[k] *= w;
+              gram_add(R, r);
+            });
   if (chol_inplace(R)) { status[0] = APC_ERR_RANK; status[1] = i; return; }
   // y = R^-1 R^-T u, u = e3+e4+e5
   double z[9];
@@ -65,37 +62,45 @@ __global__ void __launch_bounds__(256) k_stencil(
     for (int q = k + 1; q < 9; ++q) s -= R[RI(k, q)] * y[q];
     y[k] = s / R[RI(k, k)];
   }
-  // coefficients, A1 / A2 rows (solver.cpp:116-172)
+  // coefficients, A1 / A2 rows (solver.cpp:116-172); s/24 and s/h^2 as
+  // products with the reciprocals
+  constexpr double I24 = 1.0 / 24.0;
   double p2a = 0.0, p2b = 0.0, p3 = 0.0;
-  const double ih2 = 1.0 / h / h;
+  const double ih2 = f.ih * f.ih;
   const bool pinned = pin_row0 && i == 0;
-  for (int j = 0; j < lm; ++j) {
-    int m = nbr[(size_t)j * M + i];
-    double w = weight_of(dist[(size_t)j * M + i], maxdis);
-    double r[9];
-    basis_row(pwrap_s(cx[m] - xi, G.len[0], G.bc) / h, pwrap_s(cy[m] - yi, G.len[1], G.bc) / h,
-              pwrap_s(cz[m] - zi, G.len[2], G.bc) / h, r);
-    double s = 0.0;
+  for_slots(M, lm, i, nbr, dist, cx, cy, cz, NoExtra{},
+            [&](int j, int m, double d, double x, double yy, double zz, double) {
+              double r[9];
+              const double w = fit_slot(G, f, d, x, yy, zz, r);
+              double s = 0.0;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) s += r[k] * y[k];
-    s *= w * w;
-    sco[(size_t)j * M + i] = s;
-    p2a = p2a - s / 24.0;
-    double a2 = s / h / h;
-    p2b = p2b - a2;
-    if (m >= M) p3 = p3 - bv[m - M] * a2;
-  }
+              for (int k = 0; k < 9; ++k) s += r[k] * y[k];
+              s *= w * w;
+              sco[(size_t)j * M + i] = s;
+              p2a = p2a - s * I24;
+              const double a2 = s * ih2;
+              p2b = p2b - a2;
+              if (m >= M) p3 = p3 - bv[m - M] * a2;
+            });
   double dd1 = p2a + 1;
   double dd2 = pinned ? 1.0 : p2b;
   d1[i] = dd1;
   d2[i] = dd2;
-  f1[i] = 1.0 / 24.0;
+  f1[i] = I24;
   f2[i] = pinned ? 0.0 : ih2;
   dinv1[i] = (dd1 == 0.0) ? 1.0 : 1.0 / dd1;
   dinv2[i] = (dd2 == 0.0) ? 1.0 : 1.0 / dd2;
-  rhs1[i] = 1.0 * cnp[i] / h / h / h * c_dens;
+  rhs1[i] = 1.0 * cnp[i] / ch[i] / ch[i] / ch[i] * c_dens;
   pbnd[i] = pinned ? 0.0 : p3;
 }
+
+// Extra gather of the gradient fit: phi at interior nodes, the boundary value
+// at boundary nodes.
+struct PhiOrBv {
+  const double *phi, *bv;
+  int M;
+  __device__ double operator()(int m) const { return (m < M) ? phi[m] : bv[m - M]; }
+};
 
 __global__ void __launch_bounds__(256) k_gradient(
     int M, int lm, Geo G, double c_field, const double *__restrict__ cx,
@@ -104,8 +109,7 @@ __global__ void __launch_bounds__(256) k_gradient(
     const double *__restrict__ bv, double *__restrict__ grad, int *__restrict__ status) {
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= M) return;
-  const double xi = cx[i], yi = cy[i], zi = cz[i], h = ch[i];
-  const double maxdis = row_maxdis(dist, M, i, lm);
+  const FitRow f = fit_row(cx, cy, cz, ch, dist, M, i, lm);
   const double p4 = phi[i];
   // cf = (B^T B)^-1 B^T b (B: the weighted basis rows, b: the weighted
   // differences), by the semi-normal equations with R from the Cholesky
@@ -117,28 +121,26 @@ __global__ void __launch_bounds__(256) k_gradient(
   for (int q = 0; q < 45; ++q) R[q] = 0.0;
 #pragma unroll
   for (int q = 0; q < 9; ++q) Btb[q] = 0.0;
-  for (int j = 0; j < lm; ++j) {
-    int m = nbr[(size_t)j * M + i];
-    double w = weight_of(dist[(size_t)j * M + i], maxdis);
-    double r[9];
-    basis_row(pwrap_s(cx[m] - xi, G.len[0], G.bc) / h, pwrap_s(cy[m] - yi, G.len[1], G.bc) / h,
-              pwrap_s(cz[m] - zi, G.len[2], G.bc) / h, r);
+  for_slots(M, lm, i, nbr, dist, cx, cy, cz, PhiOrBv{phi, bv, M},
+            [&](int, int, double d, double x, double y, double z, double v) {
+              double r[9];
+              const double w = fit_slot(G, f, d, x, y, z, r);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) r[k] *= w;
-    double v = (m < M) ? phi[m] : bv[m - M];
-    double rb = w * (v - p4) * c_field;
-    gram_add(R, r);
+              for (int k = 0; k < 9; ++k) r[k] *= w;
+              const double rb = w * (v - p4) * c_field;
+              gram_add(R, r);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) Btb[k] += r[k] * rb;
-  }
+              for (int k = 0; k < 9; ++k) Btb[k] += r[k] * rb;
+            });
   if (chol_inplace(R)) { status[0] = APC_ERR_RANK; status[1] = i; return; }
   double cf[9];
   seminormal_solve(R, Btb, cf);
-  grad[i] = cf[0] / h;
-  grad[(size_t)M + i] = cf[1] / h;
-  grad[2 * (size_t)M + i] = cf[2] / h;
+  const double ih2 = f.ih * f.ih;
+  grad[i] = cf[0] * f.ih;
+  grad[(size_t)M + i] = cf[1] * f.ih;
+  grad[2 * (size_t)M + i] = cf[2] * f.ih;
 #pragma unroll
-  for (int q = 3; q < 9; ++q) grad[(size_t)q * M + i] = cf[q] / h / h;
+  for (int q = 3; q < 9; ++q) grad[(size_t)q * M + i] = cf[q] * ih2;
 }
 
 // Boundary values at the boundary nodes (bv_kind 1: uniform sphere).

@@ -173,8 +173,9 @@ def test_stencils_and_system(kind, n, bc, c, lm):
         s.set_constants(1.0 / n, 1.5 / 0.02, -1.0)
     gs, os_ = g.build_system(), o.build_system()
     pm = p[:M]
-    # the least-squares weights: QR (oracle, dgels-like) vs streamed Givens
-    # QR + semi-normal solve (GPU); agreement to 1e-9 of the row scale
+    # the least-squares weights: QR (oracle, dgels-like) vs the Cholesky
+    # factor of the Gram matrix + semi-normal solve (GPU); agreement to 1e-9
+    # of the row scale
     for f in ("a1", "a2"):
         scale = np.abs(os_[f]).max(axis=1, keepdims=True)[pm] + 1e-300
         err = np.abs(gs[f] - os_[f][pm]) / scale
