@@ -18,9 +18,8 @@ __global__ void __launch_bounds__(256) k_interp_kick(
     double dt, unsigned long long *__restrict__ maxbits, int wrapdx, double lx, double ly,
     double lz) {
   __shared__ double ws[4];
-  int s = blockIdx.x * 256 + threadIdx.x;
   double change = 0.0;
-  if (s < N) {
+  for (int s = blockIdx.x * 256 + threadIdx.x; s < N; s += gridDim.x * 256) {
     int i = owner[s];
     double dx = x[s] - cx[i], dy = y[s] - cy[i], dz = z[s] - cz[i];
     if (wrapdx) {  // fixed topology on a periodic box: minimum image to the leaf
@@ -42,7 +41,7 @@ __global__ void __launch_bounds__(256) k_interp_kick(
     vx[s] = vx[s] + a0 * Cv * dt;
     vy[s] = vy[s] + a1 * Cv * dt;
     vz[s] = vz[s] + a2 * Cv * dt;
-    change = (a0 * a0 + a1 * a1 + a2 * a2) * Cv * Cv * dt * dt;
+    change = fmax(change, (a0 * a0 + a1 * a1 + a2 * a2) * Cv * Cv * dt * dt);
   }
   change = block_max256(change, ws);
   if (threadIdx.x == 0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(change));
@@ -64,14 +63,13 @@ __global__ void __launch_bounds__(256) k_drift(int N, double *__restrict__ x, do
                                                double z0, double z1, int bc,
                                                unsigned long long *__restrict__ maxbits) {
   __shared__ double ws[4];
-  int s = blockIdx.x * 256 + threadIdx.x;
   double change = 0.0;
-  if (s < N) {
+  for (int s = blockIdx.x * 256 + threadIdx.x; s < N; s += gridDim.x * 256) {
     double ux = vx[s], uy = vy[s], uz = vz[s];
     x[s] = mwrap(x[s] + ux * Cx * dt, x0, x1, bc);
     y[s] = mwrap(y[s] + uy * Cx * dt, y0, y1, bc);
     z[s] = mwrap(z[s] + uz * Cx * dt, z0, z1, bc);
-    change = (ux * ux + uy * uy + uz * uz) * Cx * Cx * dt * dt;
+    change = fmax(change, (ux * ux + uy * uy + uz * uz) * Cx * Cx * dt * dt);
   }
   change = block_max256(change, ws);
   if (threadIdx.x == 0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(change));
@@ -83,6 +81,12 @@ __global__ void k_density(int S, const int *__restrict__ np, const double *__res
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i < S) out[i] = np[i] / h[i] / h[i] / h[i];
 }
+
+// The kick and the drift end in one atomic maximum per block on a single
+// word, and those atomics serialise: one block per particle tile (65,536 at
+// 16.8M particles) cost more than the particle traffic itself.  A grid of
+// at most 2048 blocks (eight per CU) strides over the particles instead.
+static unsigned mover_blocks(int N) { return std::max(1u, std::min(2048u, nblocks(N, 256))); }
 
 static int read_max(apc_ctx *c, double *out) {
   unsigned long long bits = 0;
@@ -108,7 +112,7 @@ int interp_kick(apc_ctx *c, double Cv, double dt, double *maxdv) {
     const double *b = c->prm.bbox;
     const int wrapdx = (c->frozen && c->prm.bc == 1) ? 1 : 0;
     KScope ks(c, "interp_kick", 100.0 * N + 96.0 * M);
-    hipLaunchKernelGGL(k_interp_kick, dim3(nblocks(N, 256)), dim3(256), 0, c->s, N, M, c->owner.p,
+    hipLaunchKernelGGL(k_interp_kick, dim3(mover_blocks(N)), dim3(256), 0, c->s, N, M, c->owner.p,
                        c->cx.p, c->cy.p, c->cz.p, c->grad.p, c->x.p, c->y.p, c->z.p, c->vx.p,
                        c->vy.p, c->vz.p, c->ax.p, c->ay.p, c->az.p, Cv, dt, c->red_u64.p, wrapdx,
                        b[1] - b[0], b[3] - b[2], b[5] - b[4]);
@@ -125,7 +129,7 @@ int drift(apc_ctx *c, double Cx, double dt, double *maxdx) {
   APC_HIP(c, hipMemsetAsync(c->red_u64.p, 0, sizeof(unsigned long long), c->s));
   {
     KScope ks(c, "drift", 72.0 * N);
-    hipLaunchKernelGGL(k_drift, dim3(nblocks(N, 256)), dim3(256), 0, c->s, N, c->x.p, c->y.p,
+    hipLaunchKernelGGL(k_drift, dim3(mover_blocks(N)), dim3(256), 0, c->s, N, c->x.p, c->y.p,
                        c->z.p, c->vx.p, c->vy.p, c->vz.p, Cx, dt, b[0], b[1], b[2], b[3], b[4],
                        b[5], c->prm.bc, c->red_u64.p);
   }
