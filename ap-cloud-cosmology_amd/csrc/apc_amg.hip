@@ -229,13 +229,17 @@ __global__ void __launch_bounds__(256) k_amg_pick(V v, double theta, const doubl
 }
 
 // MIS round, part 2: an undecided row with a strong root neighbour is
-// covered; the rest are counted, one atomic per wave.
+// covered; the rest are counted.  The grid is bounded (eight blocks per CU,
+// striding over the rows) and each block adds its count with one atomic:
+// atomics on one word serialise, and one per wave of a 7.5M-row level (117k)
+// cost more than the round's loads.
 template <class V>
 __global__ void __launch_bounds__(256) k_amg_cover(V v, double theta, const double *__restrict__ smax,
                                                    int *__restrict__ b, int *__restrict__ undecided) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  bool und = false;
-  if (i < v.n() && b[i] == 0) {
+  __shared__ int ws[4];
+  int und = 0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < v.n(); i += gridDim.x * 256) {
+    if (b[i] != 0) continue;
     const double lim = theta * smax[i];
     bool cov = false;
     auto f = [&](int j, double x, bool ok) {
@@ -245,10 +249,16 @@ __global__ void __launch_bounds__(256) k_amg_cover(V v, double theta, const doub
     if (theta > 0) v.template offdiag_all<true>(i, f);
     else v.template offdiag_all<false>(i, f);  // every coupling is strong: pattern only
     if (cov) b[i] = 2;
-    else und = true;
+    else ++und;
   }
-  const uint64_t m = __ballot(und);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(undecided, __popcll(m));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) und += __shfl_xor(und, o, 64);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = und;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int t = ws[0] + ws[1] + ws[2] + ws[3];
+    if (t) atomicAdd(undecided, t);
+  }
 }
 
 __global__ void k_amg_rootflag(int n, const int *__restrict__ st, int *__restrict__ flag) {
@@ -934,7 +944,8 @@ static int aggregate_level(apc_ctx *c, AmgHier *h, const V &v, int n, AmgLevel *
     if (round > 200) return fail(c, APC_ERR_UNSUPPORTED, "amg: independent set did not settle");
     APC_HIP(c, hipMemsetAsync(h->cnt.p, 0, sizeof(int), c->s));
     hipLaunchKernelGGL((k_amg_pick<V>), dim3(g), dim3(256), 0, c->s, v, o.theta, h->smax.p, a, b);
-    hipLaunchKernelGGL((k_amg_cover<V>), dim3(g), dim3(256), 0, c->s, v, o.theta, h->smax.p, b, h->cnt.p);
+    hipLaunchKernelGGL((k_amg_cover<V>), dim3(std::min(g, 2048u)), dim3(256), 0, c->s, v, o.theta,
+                       h->smax.p, b, h->cnt.p);
     int und = 0;
     APC_HIP(c, hipMemcpyAsync(&und, h->cnt.p, sizeof(int), hipMemcpyDeviceToHost, c->s));
     APC_HIP(c, hipStreamSynchronize(c->s));
