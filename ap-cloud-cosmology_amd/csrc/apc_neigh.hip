@@ -378,10 +378,10 @@ __device__ inline int ht_claim(int *ht, int m) {
   }
 }
 
-// wrapped offset to node m: octant in the rotated frame and squared distance
-__device__ inline int place_w(const Frame &F, const double4 *__restrict__ xyz, double x0, double y0,
-                              double z0, int m, double *dd) {
-  const double4 pm = xyz[m];
+// wrapped offset to a node at pm: octant in the rotated frame and squared
+// distance
+__device__ inline int place_p(const Frame &F, const double4 pm, double x0, double y0, double z0,
+                              double *dd) {
   double ex = pwrap(pm.x - x0, F.len[0], F.bc);
   double ey = pwrap(pm.y - y0, F.len[1], F.bc);
   double ez = pwrap(pm.z - z0, F.len[2], F.bc);
@@ -390,6 +390,12 @@ __device__ inline int place_w(const Frame &F, const double4 *__restrict__ xyz, d
   double zt = F.A[6] * ex + F.A[7] * ey + F.A[8] * ez;
   *dd = ex * ex + ey * ey + ez * ez;
   return (xt > 0) + 2 * (yt > 0) + 4 * (zt > 0);
+}
+
+// the same for node m
+__device__ inline int place_w(const Frame &F, const double4 *__restrict__ xyz, double x0, double y0,
+                              double z0, int m, double *dd) {
+  return place_p(F, xyz[m], x0, y0, z0, dd);
 }
 
 // The reference's quicksort on one short range by one lane (apartition /
@@ -631,19 +637,38 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
       }
       if (lane == 0) coff[lsnap] = T;
       __syncthreads();
+      // candidate t's index (-1 past the round's end): Neigh24 slot t - coff[a]
+      // of the last snapshot entry a with coff[a] <= t
+      auto cand_of = [&](int t) {
+        if (t >= T) return -1;
+        int lo = 0, hi = lsnap;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (coff[mid] <= t) lo = mid;
+          else hi = mid - 1;
+        }
+        return n24[(size_t)L.snap[lo] * 24 + (t - coff[lo])];
+      };
+      auto pos_of = [&](int m) {
+        return (m >= 0 && m < S) ? xyz[m] : make_double4(0.0, 0.0, 0.0, 0.0);
+      };
+      // The chunk's loads run ahead: chunk c+2's index and chunk c+1's
+      // position are in flight while chunk c claims its slots, so a wave
+      // does not wait on the chain (offset search, Neigh24 gather, position
+      // gather) once per chunk.  Positions are read for every valid
+      // candidate, fresh or not.
+      int mC = cand_of(lane), mB = cand_of(64 + lane);
+      double4 pC = pos_of(mC);
       for (int tb = 0; tb < T; tb += 64) {
+        const int mA = cand_of(tb + 128 + lane);
+        const double4 pB = pos_of(mB);
         const int t = tb + lane;
         const bool live = t < T;
-        int m = -1;
-        if (live) {
-          int lo = 0, hi = lsnap;  // last entry a with coff[a] <= t
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (coff[mid] <= t) lo = mid;
-            else hi = mid - 1;
-          }
-          m = n24[(size_t)L.snap[lo] * 24 + (t - coff[lo])];
-        }
+        const int m = mC;
+        const double4 pm = pC;
+        mC = mB;
+        pC = pB;
+        mB = mA;
         const bool invalid = live && m >= S;
         // new to the list: the index is in no earlier chunk (nor the node's
         // Neigh24) and no earlier lane of this chunk holds it.  Every
@@ -661,7 +686,7 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
         ++ctag;
         int o = 8;
         double dd = 0.0;
-        if (fresh) o = place_w(F, xyz, x0, y0, z0, m, &dd);
+        if (fresh) o = place_p(F, pm, x0, y0, z0, &dd);
         const uint64_t lt = lanes_below();
         const uint64_t mf = __ballot(fresh);
         const int rank = __popcll(mf & lt);  // position among this chunk's new entries
