@@ -81,16 +81,20 @@ __device__ inline double row_apply(const Op<T> &A, const T *__restrict__ x, int 
 struct AmgHier;  // apc_amg.hip
 
 // Defaults from sweeps on the 256^3 Zel'dovich + Plummer system (7.5M rows):
-// one Jacobi sweep each side, damping 0.85, a 1.5x coarse correction, a
-// W-cycle on the three finest levels and V below, and a dense solve once a
-// level has <= 256 rows (6 levels: 7.5M, 611k, 58k, 6.7k, 838, ~100) took the
-// step from 287 to 248 ms (40 -> 33 GMRES steps to rtol 1e-8) against a
-// W-cycle everywhere with a dense level of <= 1024 rows.  A larger correction
-// scale is faster still up to 1.7 and diverges at 1.9, so 1.5 keeps a margin.
+// one Jacobi sweep each side, damping 0.85, a W-cycle on the three finest
+// levels and V below, and a dense solve once a level has <= 256 rows (6
+// levels: 7.5M, 611k, 58k, 6.7k, 838, ~100) took the step from 287 to 248 ms
+// (40 -> 33 GMRES steps to rtol 1e-8) against a W-cycle everywhere with a
+// dense level of <= 1024 rows.  The coarse correction is scaled by 1.6: per
+// step 180 ms at 1.5 (33-34 GMRES steps), 174 at 1.6 (31-32), 173 at 1.65,
+// 176 at 1.7 (30-35: a step needing more), and the cycle alone diverges at
+// 1.9, so 1.6 keeps a margin.  Measured alternatives that were slower: the
+// W-cycle from level 1 down (a V at level 0: 46 steps), a dense solve at the
+// 851-row level, 8 lanes per row on level 1, damping 0.8 or 0.9 (no change).
 struct AmgOptions {
   int nu = 1;             // Jacobi sweeps before and after the coarse correction
   double omega = 0.85;    // Jacobi damping
-  double alpha = 1.5;     // scaling of the coarse correction
+  double alpha = 1.6;     // scaling of the coarse correction
   int gamma = 2;          // 1: V-cycle, 2: W-cycle
   int wlevels = 3;        // gamma applies on the first wlevels levels, V below
   double theta = 0.0;     // strength threshold (fraction of the row's largest |a_ij|)
