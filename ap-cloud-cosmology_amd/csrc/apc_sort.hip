@@ -54,19 +54,45 @@ __global__ void __launch_bounds__(256) k_rx_hist(const uint64_t *__restrict__ k,
   hist[t * nblk + blockIdx.x] = h[t];
 }
 
-// Stable scatter: rank = block offset of the digit + keys of the same digit
-// earlier in this tile (earlier iterations, lower waves, lower lanes).
+// Stable scatter: a key's output position is the block offset of its digit
+// plus the keys of the same digit earlier in this tile (earlier iterations,
+// lower waves, lower lanes).  The tile is first placed in LDS in that order
+// (the digit's tile-local offset from this tile's histogram plus the same
+// rank) and then written out in LDS order, so consecutive threads store
+// consecutive positions of a digit's run instead of one scattered word each.
 __global__ void __launch_bounds__(256) k_rx_scatter(const uint64_t *__restrict__ kin,
                                                     const int *__restrict__ vin,
                                                     uint64_t *__restrict__ kout,
                                                     int *__restrict__ vout, int n, int sh,
-                                                    const int *__restrict__ offs, int nblk) {
-  __shared__ int base[256];
+                                                    const int *__restrict__ offs,
+                                                    const int *__restrict__ hist, int nblk) {
+  __shared__ uint64_t sk[RX_TILE];
+  __shared__ int sv[RX_TILE];
+  __shared__ int gbase[256];  // output offset of digit t for this tile
+  __shared__ int lpre[256];   // tile-local offset of digit t
+  __shared__ int run[256];    // keys of digit t placed by earlier iterations
   __shared__ int wcnt[4][256];
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  base[t] = offs[t * nblk + blockIdx.x];
-  const long tile = (long)blockIdx.x * RX_TILE;
   const unsigned long long lower = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  gbase[t] = offs[t * nblk + blockIdx.x];
+  {  // exclusive scan of this tile's digit counts over the block
+    const int c0 = hist[t * nblk + blockIdx.x];
+    int inc = c0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += u;
+    }
+    if (lane == 63) wcnt[0][w] = inc;
+    __syncthreads();
+    int before = 0;
+    for (int q = 0; q < w; ++q) before += wcnt[0][q];
+    lpre[t] = before + inc - c0;
+    run[t] = 0;
+    __syncthreads();
+  }
+  const long tile = (long)blockIdx.x * RX_TILE;
+  const int tn = (int)((n - tile) < RX_TILE ? (n - tile) : RX_TILE);
   for (int it = 0; it < 16; ++it) {
     long i = tile + it * 256 + t;
     bool valid = i < n;
@@ -87,14 +113,21 @@ __global__ void __launch_bounds__(256) k_rx_scatter(const uint64_t *__restrict__
     if (valid && rank == cnt - 1) wcnt[w][d] = cnt;
     __syncthreads();
     if (valid) {
-      int off = base[d] + rank;
-      for (int q = 0; q < w; ++q) off += wcnt[q][d];
-      kout[off] = kk;
-      vout[off] = vv;
+      int lp = lpre[d] + run[d] + rank;
+      for (int q = 0; q < w; ++q) lp += wcnt[q][d];
+      sk[lp] = kk;
+      sv[lp] = vv;
     }
     __syncthreads();
-    base[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+    run[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
     __syncthreads();
+  }
+  for (int e = t; e < tn; e += 256) {
+    const uint64_t kk = sk[e];
+    const int d = (int)((kk >> sh) & 255);
+    const long g = (long)gbase[d] + (e - lpre[d]);
+    kout[g] = kk;
+    vout[g] = sv[e];
   }
 }
 
@@ -119,7 +152,7 @@ int radix_sort_pairs(apc_ctx *c, uint64_t *k, int *v, uint64_t *k2, int *v2, int
     {
       KScope ks(c, "radix_scatter", 24.0 * n);
       hipLaunchKernelGGL(k_rx_scatter, dim3(nblk), dim3(256), 0, c->s, ka, va, kb, vb, n, sh,
-                         c->histx.p, nblk);
+                         c->histx.p, c->hist.p, nblk);
     }
     std::swap(ka, kb);
     std::swap(va, vb);
