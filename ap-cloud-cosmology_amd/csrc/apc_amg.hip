@@ -65,7 +65,10 @@ struct CycleGraph {
   const void *in = nullptr, *st = nullptr, *a0 = nullptr;
   void *out = nullptr;
   int f32 = 0, gen = -1;
-  void release() {
+  // s: the stream the graph was launched on; it is drained first, so that no
+  // launch of this exec is still queued or running when it is destroyed
+  void release(hipStream_t s) {
+    if (exec && s) (void)hipStreamSynchronize(s);
     if (exec) hipGraphExecDestroy(exec);
     if (graph) hipGraphDestroy(graph);
     exec = nullptr;
@@ -89,8 +92,9 @@ struct AmgHier {
   DBuf<double> ev, smax, diag;
   DBuf<int> gcol;     // Galerkin entries grouped by coarse row: coarse column
   DBuf<double> gval;  // and value
+  hipStream_t s = nullptr;  // the owning context's stream (graphs are launched on it)
   ~AmgHier() {
-    for (auto &g : graphs) g.release();
+    for (auto &g : graphs) g.release(s);
     for (auto *l : L) { l->release(); delete l; }
     gj.release(); cinv.release(); colk.release(); cinv32.release(); key.release();
     key2.release(); idx.release(); idx2.release(); sa.release(); sb.release(); flag.release();
@@ -796,6 +800,13 @@ __global__ void k_cy_restrict(int nc, const int *__restrict__ mp, const int *__r
 }
 
 template <class T>
+__global__ void k_cy_copy(int n, const T *__restrict__ a, T *__restrict__ b, const GState *st) {
+  if (st && halted(st)) return;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+template <class T>
 __global__ void k_cy_prolong(int n, const int *__restrict__ agg, const T *__restrict__ xc,
                              T *__restrict__ x, double alpha, const GState *st) {
   if (st && halted(st)) return;
@@ -1207,7 +1218,8 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
               const AmgOptions &o) {
   if (!*hp) *hp = new AmgHier();
   AmgHier *h = *hp;
-  for (auto &gr : h->graphs) gr.release();
+  h->s = c->s;
+  for (auto &gr : h->graphs) gr.release(c->s);
   h->graphs.clear();
   ++h->gen;
   h->o = o;
@@ -1454,7 +1466,10 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
     sweep(cur, alt);
     std::swap(cur, alt);
   }
-  if (cur != x) APC_HIP(c, hipMemcpyAsync(x, cur, sizeof(T) * n, hipMemcpyDeviceToDevice, c->s));
+  // a kernel, not a device-to-device memcpy: the captured cycle then holds
+  // only this library's kernel nodes
+  if (cur != x)
+    hipLaunchKernelGGL((k_cy_copy<T>), dim3(g), dim3(256), 0, c->s, n, (const T *)cur, x, st);
   return 0;
 }
 
@@ -1502,11 +1517,11 @@ int amg_apply(apc_ctx *c, AmgHier *h, const Op<T> &A, const T *in, T *out, const
   g.graph = graph;
   e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
   if (e != hipSuccess) {
-    g.release();
+    g.release(c->s);
     return hip_fail(c, e, "hipGraphInstantiate (multigrid cycle)");
   }
   if (h->graphs.size() >= 4) {
-    h->graphs.front().release();
+    h->graphs.front().release(c->s);
     h->graphs.erase(h->graphs.begin());
   }
   h->graphs.push_back(g);

@@ -125,7 +125,22 @@ int Controller::solve() {
     if (AdaptiveTimeStep) {  // :146-162
       double max_density = 0.0;
       apc_host::Binding *b = apc_host::find_by_particles(physical_particle);
-      if (b) apc_max_density(b->ctx, &max_density);
+      if (!b) {
+        fprintf(stdout, "Controller::solve: adaptive time step without selected particles\n");
+        errorflag = APC_ERR_ARG;
+        break;
+      }
+      int r = apc_max_density(b->ctx, &max_density);
+      if (r) {
+        apc_host::fatal(b->ctx, r, "Controller::solve (adaptive time step)");
+        errorflag = r;
+        break;
+      }
+      if (!(max_density > 0.0)) {  // every node empty: the reference would divide by zero
+        fprintf(stdout, "Controller::solve: maximum node density is %g\n", max_density);
+        errorflag = APC_ERR_ARG;
+        break;
+      }
       double min_interparticle_distance = 1.0 / pow(max_density, 1.0 / 2);
       delta_t = dynamic_time_step_size * min_interparticle_distance;
       if (delta_t > Delta_A) delta_t = Delta_A;

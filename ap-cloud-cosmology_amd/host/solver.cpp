@@ -6,7 +6,8 @@
 
 #include "binding.h"
 
-void apc_parse_ksp_options(int argc, char **argv, apc_ksp_opts *o) {
+int apc_parse_ksp_options(int argc, char **argv, apc_ksp_opts *o) {
+  int status = 0;
   // PETSc defaults (KSPCreate / KSPGMRES)
   o->rtol = 1e-5;
   o->atol = 1e-50;
@@ -31,10 +32,17 @@ void apc_parse_ksp_options(int argc, char **argv, apc_ksp_opts *o) {
       // (right preconditioning, true-residual convergence test)
       if (!strcmp(v, "jacobi")) o->pc_type = 0;
       else if (!strcmp(v, "gamg") || !strcmp(v, "amg")) o->pc_type = 1;
-      else fprintf(stderr, "apcloud: -pc_type %s not supported (jacobi, gamg)\n", v);
+      else {
+        // PCSetType fails on a type it does not know, and KSPSetFromOptions
+        // passes that error on (solver.cpp:202,274 CHKERRQ): an error, not a
+        // silent Jacobi solve
+        fprintf(stderr, "apcloud: -pc_type %s not supported (jacobi, gamg)\n", v);
+        status = APC_ERR_UNSUPPORTED;
+      }
       ++i;
     }
   }
+  return status;
 }
 
 Solver::Solver(Computational_particle *computational_particle1, int argc_o, char *args_o[])
@@ -49,13 +57,17 @@ Solver::Solver(Computational_particle *computational_particle1, int argc_o, char
   dirichlet_point = 1;
   x_min = y_min = z_min = 0.0;
   x_max = y_max = z_max = 1.0;
-  apc_parse_ksp_options(argc, args, &opts);
+  opts_status = apc_parse_ksp_options(argc, args, &opts);
   memset(&info, 0, sizeof(info));
 }
 
 Solver::~Solver() {}
 
 int Solver::solve() {
+  if (opts_status) {
+    fprintf(stdout, "Solver::solve: unsupported PETSc options (%d)\n", opts_status);
+    return opts_status;
+  }
   apc_host::Binding *b = apc_host::find_by_comp(cp);
   if (!b) {
     fprintf(stdout, "Solver::solve: no computational particles selected\n");

@@ -36,10 +36,12 @@ class Solver {
   int argc;
   char **args;
   apc_ksp_opts opts;
+  int opts_status = 0;  // non-zero: argv named an option value this build does not support
   apc_solve_info info;
 };
 
-// Parses the PETSc options the path honours from argv.
-void apc_parse_ksp_options(int argc, char **argv, apc_ksp_opts *o);
+// Parses the PETSc options the path honours from argv.  Returns 0, or
+// APC_ERR_UNSUPPORTED for a -pc_type other than jacobi / gamg / amg.
+int apc_parse_ksp_options(int argc, char **argv, apc_ksp_opts *o);
 
 #endif
