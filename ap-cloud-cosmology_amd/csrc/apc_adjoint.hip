@@ -374,23 +374,42 @@ extern "C" int apc_set_state(apc_ctx *c, const double *x, const double *y, const
   return 0;
 }
 
+// With the topology frozen the device order never changes, so (x, v) is the
+// whole state.  Otherwise every step re-sorts the particles, and the slot
+// also keeps their caller indices; a restore then drops the tree and
+// everything built on it, as a drift does.
 extern "C" int apc_checkpoint(apc_ctx *c, int slot, int restore) {
   if (!c || slot < 0 || slot >= 64) return APC_ERR_ARG;
-  if (int r = need_frozen(c, "checkpoint")) return r;
   const int N = c->N;
+  if (N < 1) return fail(c, APC_ERR_ARG, "checkpoint: no particles");
+  if (c->frozen && !c->have_comp) return fail(c, APC_ERR_ARG, "checkpoint: frozen without a topology");
   double *st[6] = {c->x.p, c->y.p, c->z.p, c->vx.p, c->vy.p, c->vz.p};
   if (!restore) {
     if ((int)c->ckpt.size() <= slot) c->ckpt.resize(slot + 1);
+    if ((int)c->ckpt_id.size() <= slot) c->ckpt_id.resize(slot + 1);
     APC_ALLOC(c, c->ckpt[slot], (size_t)6 * N + 6);
+    APC_ALLOC(c, c->ckpt_id[slot], (size_t)N + 1);
     for (int d = 0; d < 6; ++d)
       APC_HIP(c, hipMemcpyAsync(c->ckpt[slot].p + (size_t)d * N, st[d], sizeof(double) * N,
                                 hipMemcpyDeviceToDevice, c->s));
+    APC_HIP(c, hipMemcpyAsync(c->ckpt_id[slot].p, c->id.p, sizeof(int) * N,
+                              hipMemcpyDeviceToDevice, c->s));
+    c->ckpt_n.resize(c->ckpt.size(), -1);
+    c->ckpt_n[slot] = N;
   } else {
-    if ((int)c->ckpt.size() <= slot || !c->ckpt[slot].p)
+    if ((int)c->ckpt.size() <= slot || !c->ckpt[slot].p || c->ckpt_n[slot] != N)
       return fail(c, APC_ERR_ARG, "checkpoint %d was not saved", slot);
     for (int d = 0; d < 6; ++d)
       APC_HIP(c, hipMemcpyAsync(st[d], c->ckpt[slot].p + (size_t)d * N, sizeof(double) * N,
                                 hipMemcpyDeviceToDevice, c->s));
+    if (!c->frozen) {
+      APC_HIP(c, hipMemcpyAsync(c->id.p, c->ckpt_id[slot].p, sizeof(int) * N,
+                                hipMemcpyDeviceToDevice, c->s));
+      c->sorted = false;
+      c->have_comp = false;
+      c->have_nbr = c->have_sys = 0;
+    }
+    c->have_fields = 0;
   }
   return 0;
 }

@@ -99,13 +99,28 @@ __global__ void k_set_grad(int M, const double *__restrict__ g, const double *__
   for (int q = 0; q < 6; ++q) grad[(size_t)(3 + q) * M + i] = H[6 * (size_t)i + q];
 }
 
+static thread_local std::string tl_err;
+int fail_noctx(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  tl_err = buf;
+  return code;
+}
+
 }  // namespace apc
 
 using namespace apc;
 
 extern "C" {
 
-const char *apc_last_error(const apc_ctx *c) { return c ? c->err.c_str() : "null context"; }
+// NULL: the last error of a context-free call on this thread
+const char *apc_last_error(const apc_ctx *c) {
+  if (c) return c->err.c_str();
+  return apc::tl_err.empty() ? "null context" : apc::tl_err.c_str();
+}
 
 const char *apc_version(void) { return "apcloud-mi355x 0.1 (gfx950)"; }
 
@@ -159,7 +174,9 @@ int apc_destroy(apc_ctx *c) {
                           &c->cic_v2, &c->cic_rng, &c->vnode, &c->vnode2};
   for (auto b : is) b->release();
   for (auto &b : c->ckpt) b.release(c->s);
+  for (auto &b : c->ckpt_id) b.release(c->s);
   c->key.release(c->s); c->key2.release(c->s); c->nprefix.release(c->s); c->red_u64.release(c->s);
+  c->visits.release(c->s);
   c->tr_key.release(c->s); c->tr_key2.release(c->s); c->cic_key.release(c->s); c->cic_key2.release(c->s);
   c->vcode.release(c->s); c->vcode2.release(c->s);
   hipStreamDestroy(c->s);

@@ -120,11 +120,15 @@ struct apc_ctx {
   int have_cic_w = 0;
   double cic_wsum = 0.0;
   std::vector<apc::DBuf<double>> ckpt;
+  std::vector<apc::DBuf<int>> ckpt_id;     // caller indices at the save (normal form)
+  std::vector<int> ckpt_n;                 // particle count at the save, -1 if none
   int nsearch_long = 0;
   int search_slots = 0;        // resident threads of the neighbour search
   int search_waves = 0;        // resident waves of the wave-per-node search
   int search_pen = 0;          // its penalty table is on the device
   apc::DBuf<int> status_dev;
+  apc::DBuf<unsigned long long> visits;   // Neigh24 entries the last search visited
+  unsigned long long search_visits = 0;
   apc::SolveState *gm = nullptr;
   // reductions
   apc::DBuf<unsigned long long> red_u64;
@@ -178,11 +182,24 @@ struct KScope {
 // Scan / reduction helpers (apc_scan.hip).  Exclusive scan of n ints; the
 // total is written to *total_dev (device) if non-null.
 int scan_exclusive(apc_ctx *c, const int *in, int *out, int n, int *total_dev);
+// The same on stream s with a caller workspace `slab` of scan_slab_len(n)
+// ints (no allocation, no total).
+size_t scan_slab_len(long n);
+hipError_t scan_exclusive_ws(hipStream_t s, const int *in, int *out, int n, int *slab);
 // Max of n non-negative doubles (device pointer in), result to host.
 int reduce_max_nonneg(apc_ctx *c, const double *in, int n, double *out_host);
 // Stable LSD radix sort of (k, v) on the low `bits` bits of k; k2/v2 are
 // same-sized scratch (apc_sort.hip).
 int radix_sort_pairs(apc_ctx *c, uint64_t *k, int *v, uint64_t *k2, int *v2, int n, int bits);
+// Morton keys and the stable sort of (key, index) on stream s with caller
+// workspaces only (no allocation): keys/idx receive the sorted pairs; k2/v2
+// hold n entries, hist/histx 256 * ceil(n / 4096) ints each, slab
+// scan_slab_len(256 * ceil(n / 4096)) ints (apc_sort.hip).
+hipError_t morton_sort_ws(hipStream_t s, int n, const double *x, const double *y, const double *z,
+                          const double *bbox, int depth, uint64_t *keys, int *idx, uint64_t *k2,
+                          int *v2, int *hist, int *histx, int *slab);
+// Error text of the last failed context-free call on this thread.
+int fail_noctx(int code, const char *fmt, ...);
 
 // Stage entry points implemented across files.
 int sort_particles(apc_ctx *c);                  // apc_sort.hip

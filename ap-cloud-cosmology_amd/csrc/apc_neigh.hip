@@ -560,8 +560,13 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
                                                     const int *__restrict__ n24,
                                                     const int *__restrict__ n24cnt,
                                                     int *__restrict__ nbr, double *__restrict__ dist,
-                                                    int *__restrict__ status, int *__restrict__ ovf) {
+                                                    int *__restrict__ status, int *__restrict__ ovf,
+                                                    unsigned long long *__restrict__ visits) {
   __shared__ WaveLds L;
+  // Neigh24 entries this wave visited (the node's own and every expansion
+  // round's candidates): the search's algorithmic traffic is 28 bytes per
+  // visit (index + centre) plus its output
+  unsigned long long nvis = 0;
   const int lane = threadIdx.x;
   const int nq = 10 * lm;
   // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
@@ -592,6 +597,7 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
     int l;
     {
       const int cnt = n24cnt[i0];
+      nvis += (unsigned)cnt;
       int m = -1, o = 8;
       if (lane < cnt) {
         m = n24[(size_t)i0 * 24 + lane];
@@ -636,6 +642,7 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
         T += tot;
       }
       if (lane == 0) coff[lsnap] = T;
+      nvis += (unsigned)T;
       __syncthreads();
       // candidate t's index (-1 past the round's end): Neigh24 slot t - coff[a]
       // of the last snapshot entry a with coff[a] <= t
@@ -795,6 +802,7 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
     }
     __syncthreads();
   }
+  if (lane == 0 && nvis) atomicAdd(visits, nvis);  // one atomic per wave
 }
 
 int search_neighbors(apc_ctx *c) {
@@ -806,6 +814,8 @@ int search_neighbors(apc_ctx *c) {
   APC_ALLOC(c, c->dist, (size_t)lm * M + 1);
   APC_ALLOC(c, c->status_dev, 4);
   APC_HIP(c, hipMemsetAsync(c->status_dev.p, 0, 4 * sizeof(int), c->s));
+  APC_ALLOC(c, c->visits, 1);
+  APC_HIP(c, hipMemsetAsync(c->visits.p, 0, sizeof(unsigned long long), c->s));
   Frame F;
   {  // solver.cpp:464-475, computed on the host
     double alpha = 0.01, beta = 0.001, gamma = 0.0001;
@@ -867,6 +877,7 @@ int search_neighbors(apc_ctx *c) {
   double4 *xyz = (double4 *)c->nxyz.p;
   hipLaunchKernelGGL(k_pack_xyz, dim3(nblocks(S, 256)), dim3(256), 0, c->s, S, c->cx.p, c->cy.p,
                      c->cz.p, xyz);
+  int search_scope = -1;  // timer of the wave search; its bytes are known after the launch
   if (wave) {
     if (!c->search_pen) {  // per context: the constant table lives on its device
       static const std::array<double, WCAP> pen = [] {
@@ -883,6 +894,7 @@ int search_neighbors(apc_ctx *c) {
       c->search_pen = 1;
     }
     KScope ks(c, "neighbor_search");
+    search_scope = ks.idx;
     int grid = std::min(M, c->search_waves);
     if (grid >= 8) grid -= grid % 8;  // a multiple of 8: node chunks dealt to the XCDs
     grid = std::max(1, grid);
@@ -894,7 +906,7 @@ int search_neighbors(apc_ctx *c) {
     }();
     hipLaunchKernelGGL(k_search_wave, dim3(grid), dim3(64), 0, c->s, M, S, lm,
                        std::min(fast_cap, WCAP), xchunk, F, (const double4 *)xyz, c->n24.p, c->n24cnt.p,
-                       c->nbr.p, c->dist.p, c->status_dev.p, c->novf.p);
+                       c->nbr.p, c->dist.p, c->status_dev.p, c->novf.p, c->visits.p);
   } else {
     KScope ks(c, "neighbor_search");
     hipLaunchKernelGGL(k_search, dim3(nblocks(slots, 256)), dim3(256), 0, c->s, M, S, M,
@@ -902,8 +914,16 @@ int search_neighbors(apc_ctx *c) {
                        c->status_dev.p, c->novf.p);
   }
   int st[3] = {0, 0, 0};
+  unsigned long long nvis = 0;
   APC_HIP(c, hipMemcpyAsync(st, c->status_dev.p, sizeof(st), hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipMemcpyAsync(&nvis, c->visits.p, sizeof(nvis), hipMemcpyDeviceToHost, c->s));
   APC_HIP(c, hipStreamSynchronize(c->s));
+  c->search_visits = nvis;
+  // algorithmic bytes: 28 per Neigh24 entry visited (its index and the
+  // centre of the node it names), the node's own centre and count, and the
+  // lm indices and distances written
+  if (search_scope >= 0)
+    ktime_bytes(c, search_scope, 28.0 * (double)nvis + (28.0 + 12.0 * lm) * (double)M);
   // nodes whose expansion outgrew the short list, with the full bound
   if (st[0] == 0 && st[2] > 0) {
     const int rows = st[2];

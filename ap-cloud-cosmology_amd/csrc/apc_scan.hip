@@ -52,18 +52,30 @@ __global__ void k_scan_total(const int *in, const int *out, int n, int *total) {
 
 // Recursive three-phase scan; `slab` holds this level's partial sums and
 // their scan, the next level's slab follows.
-static int scan_rec(apc_ctx *c, const int *in, int *out, int n, int *slab) {
-  if (n <= 0) return 0;
+static void scan_rec(hipStream_t s, const int *in, int *out, int n, int *slab) {
+  if (n <= 0) return;
   int nb = (n + 2047) / 2048;
   int *part = slab;
   int *poff = slab + nb + 8;
-  hipLaunchKernelGGL(k_scan_tile, dim3(nb), dim3(256), 0, c->s, in, out, n, nb > 1 ? part : nullptr);
+  hipLaunchKernelGGL(k_scan_tile, dim3(nb), dim3(256), 0, s, in, out, n, nb > 1 ? part : nullptr);
   if (nb > 1) {
-    int r = scan_rec(c, part, poff, nb, slab + 2 * (nb + 16));
-    if (r) return r;
-    hipLaunchKernelGGL(k_scan_add, dim3(nblocks(n, 256)), dim3(256), 0, c->s, out, n, poff);
+    scan_rec(s, part, poff, nb, slab + 2 * (nb + 16));
+    hipLaunchKernelGGL(k_scan_add, dim3(nblocks(n, 256)), dim3(256), 0, s, out, n, poff);
   }
-  return 0;
+}
+
+size_t scan_slab_len(long n) {
+  size_t need = 64;
+  for (long m = n; m > 1;) {
+    m = (m + 2047) / 2048;
+    need += 2 * (m + 16);
+  }
+  return need;
+}
+
+hipError_t scan_exclusive_ws(hipStream_t s, const int *in, int *out, int n, int *slab) {
+  scan_rec(s, in, out, n, slab);
+  return hipGetLastError();
 }
 
 // `in` and `out` must not alias.
@@ -72,14 +84,8 @@ int scan_exclusive(apc_ctx *c, const int *in, int *out, int n, int *total_dev) {
     if (total_dev) APC_HIP(c, hipMemsetAsync(total_dev, 0, sizeof(int), c->s));
     return 0;
   }
-  size_t need = 64;
-  for (long m = n; m > 1;) {
-    m = (m + 2047) / 2048;
-    need += 2 * (m + 16);
-  }
-  APC_ALLOC(c, c->scratch_i, need);
-  int r = scan_rec(c, in, out, n, c->scratch_i.p);
-  if (r) return r;
+  APC_ALLOC(c, c->scratch_i, scan_slab_len(n));
+  APC_HIP(c, scan_exclusive_ws(c->s, in, out, n, c->scratch_i.p));
   if (total_dev) hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, c->s, in, out, n, total_dev);
   APC_HIP(c, hipGetLastError());
   return 0;

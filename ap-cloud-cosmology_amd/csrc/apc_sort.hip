@@ -165,6 +165,46 @@ int radix_sort_pairs(apc_ctx *c, uint64_t *k, int *v, uint64_t *k2, int *v2, int
   return 0;
 }
 
+__global__ void k_copy_pairs(int n, const uint64_t *__restrict__ ka, const int *__restrict__ va,
+                             uint64_t *__restrict__ kb, int *__restrict__ vb) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    kb[i] = ka[i];
+    vb[i] = va[i];
+  }
+}
+
+// The keys and sort of sort_particles with every buffer supplied: kernels
+// only, on stream s (a caller may capture it into a graph).  The passes
+// alternate between (keys, idx) and (k2, v2); an odd pass count ends with a
+// copy kernel back into (keys, idx).
+hipError_t morton_sort_ws(hipStream_t s, int n, const double *x, const double *y, const double *z,
+                          const double *bbox, int depth, uint64_t *keys, int *idx, uint64_t *k2,
+                          int *v2, int *hist, int *histx, int *slab) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_keys, dim3(nblocks(n, 256)), dim3(256), 0, s, n, x, y, z, bbox[0], bbox[1],
+                     bbox[2], bbox[3], bbox[4], bbox[5], depth, keys, idx);
+  if (n > 1) {
+    const int nblk = (n + RX_TILE - 1) / RX_TILE;
+    uint64_t *ka = keys, *kb = k2;
+    int *va = idx, *vb = v2;
+    const int passes = (3 * depth + 7) / 8;
+    for (int p = 0; p < passes; ++p) {
+      const int sh = 8 * p;
+      hipLaunchKernelGGL(k_rx_hist, dim3(nblk), dim3(256), 0, s, ka, n, sh, hist, nblk);
+      hipError_t e = scan_exclusive_ws(s, hist, histx, 256 * nblk, slab);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(k_rx_scatter, dim3(nblk), dim3(256), 0, s, ka, va, kb, vb, n, sh, histx,
+                         hist, nblk);
+      std::swap(ka, kb);
+      std::swap(va, vb);
+    }
+    if (ka != keys)
+      hipLaunchKernelGGL(k_copy_pairs, dim3(nblocks(n, 256)), dim3(256), 0, s, n, ka, va, keys, idx);
+  }
+  return hipGetLastError();
+}
+
 // Gather the particle state into sorted order.
 __global__ void __launch_bounds__(256) k_permute(int n, const int *__restrict__ p,
                                                  const double *__restrict__ a0,

@@ -22,16 +22,29 @@ density solve uses the reference's Jacobi preconditioner (it converges in
 right preconditioning; the reference's users select a multigrid with
 -pc_type through KSPSetFromOptions, solver.cpp:202,274).
 
+Schedule: the reference's time loop (controller.cpp:106-208) from
+a_start to a_end with step da: a half drift, then steps of da from
+a_start + da/2 while t < a_end, the last one shortened to end at a_end (10
+steps for zeldovich256: nine of 0.008 and one of 0.004).  Warmup and timed
+steps walk this schedule; when it is used up, the state after the initial
+half drift is restored from a device-side copy (apc_checkpoint, inside the
+timed region) and the schedule starts again, so every timed step is a step
+of the configured trajectory with its own dt > 0.  The dt of every timed
+step is recorded.
+
 Legs, all from the same state (the initial conditions after the initial half
-drift): (1) accuracy: one FP64 step and one step of the measured precision,
-relative RMS difference of the accelerations and largest position
-difference; (2) the reference configuration (FP64, Jacobi, the same GMRES
-settings) for one step: its time, iterations and its force difference from
-the converged FP64 step (skipped with --no-ref-config); (3) the FP64 step
-with the same solver settings timed over K steps (skipped with --no-fp64);
-(4) the measured precision timed over K steps -> "value".  The roofline entry
-is for the kernel with the largest device time in one extra step of leg (4)
-run with the library's kernel timers on (the timed steps run without them).
+drift): (1) accuracy: every step of the schedule run in FP64 and in the
+measured precision from the same states, relative RMS difference of the
+accelerations and largest position difference per step; (2) the reference
+configuration (FP64, Jacobi, the same GMRES settings) for one step after one
+untimed step of it (first-use allocations): its time, iterations and its
+force difference from the converged FP64 step (skipped with
+--no-ref-config); (3) the FP64 step with the same solver settings timed over
+K steps (skipped with --no-fp64); (4) the measured precision timed over K
+steps -> "value".  The roofline entry is for the kernel with the largest
+total device time (average launch time x launches) in one extra step of leg
+(4) run with the library's kernel timers on (the timed steps run without
+them); the next kernels by device time are listed beside it.
 """
 import argparse
 import json
@@ -62,6 +75,11 @@ def make_ic(w, n_side=None):
         x, y, z, vx, vy, vz = ic.plummer_clusters(x, y, z, vx, vy, vz, w["seed"], n_spheres=64,
                                                   scale=0.005, fraction=0.2)
     return x, y, z, vx, vy, vz
+
+
+def log(msg):
+    """Progress of the legs, on stderr (stdout carries only the JSON line)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
 
 
 def eds(a):
@@ -104,7 +122,9 @@ def main():
                     help="skip the reference-configuration (Jacobi) leg")
     ap.add_argument("--pc", default="amg", choices=["amg", "jacobi"],
                     help="Poisson preconditioner (the density solve always uses Jacobi)")
-    ap.add_argument("--cpu-side", type=int, default=96, help="lattice side of the CPU sample")
+    ap.add_argument("--cpu-side", type=int, default=80, help="lattice side of the CPU sample")
+    ap.add_argument("--accuracy-steps", type=int, default=None,
+                    help="schedule steps in the force-error leg (default: all)")
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--restart", type=int, default=200)
     ap.add_argument("--max-it", type=int, default=1000)
@@ -123,6 +143,10 @@ def main():
     nsteps_total = w["steps"]
     K = args.steps if args.steps is not None else min(3, nsteps_total)
     W = args.warmup
+    if K < 1 or W < 0:
+        print("bench.py: --steps must be >= 1 and --warmup >= 0", file=sys.stderr)
+        sys.exit(2)
+    log("generating %s" % args.workload)
     x, y, z, vx, vy, vz = make_ic(w)
     N = len(x)
     params = apcloud.make_params(bbox=(0, 1, 0, 1, 0, 1), depth=20, bc=1, c=0.0, lm=27,
@@ -135,7 +159,9 @@ def main():
     ctx.drift(cx0, 0.5 * da)  # initial half drift (controller.cpp:106-111)
     t_state0 = a_start + 0.5 * da
     state0 = ctx.particles()[:6]
+    ctx.checkpoint(0)  # device-side copy of the state after the half drift
     cd = 1.0 / N
+    sched = schedule(a_start, a_end, da)
 
     def opts(precision, pc):
         rho = apcloud.ksp(rtol=args.rtol, restart=args.restart, max_it=args.max_it,
@@ -144,12 +170,22 @@ def main():
                           precision=precision, pc=pc)
         return rho, phi
 
-    def one_step(t, o):
-        dt = min(da, a_end - t) if t + da > a_end else da
+    def one_step(j, o):
+        """Step j of the schedule (from its own state)."""
+        t, dt = sched[j]
         cpot, cf, _, cv = eds(t)
         _, _, cxh, _ = eds(t + dt / 2)
         _, _, info, timing = ctx.step(cd, cpot, cf, cv, cxh, dt, o[0], o[1])
-        return t + dt, info, timing
+        return info, timing
+
+    def walk(j, o):
+        """Schedule step j mod len(sched); restores the initial state on the
+        device when the schedule starts again.  Returns (info, timing, dt)."""
+        jj = j % len(sched)
+        if jj == 0 and j > 0:
+            ctx.checkpoint(0, restore=True)
+        info, timing = one_step(jj, o)
+        return info, timing, sched[jj][1]
 
     def force_err(got, ref):
         d = np.stack([got[6] - ref[6], got[7] - ref[7], got[8] - ref[8]])
@@ -158,92 +194,128 @@ def main():
         dx -= np.round(dx)  # periodic unit box
         return float(np.sqrt((d ** 2).sum() / (a ** 2).sum())), float(np.abs(dx).max())
 
-    # (1) accuracy: one step of each form from the same state
-    acc, ref = None, None
+    # (1) accuracy: every schedule step of each form from the same state (the
+    # FP64 trajectory's state before the step), errors per step
+    acc, ref0 = None, None
     if not args.no_accuracy:
-        ctx.set_particles(*state0)
-        _, info64, _ = one_step(t_state0, opts(0, args.pc))
-        ref = ctx.particles()
-        ctx.set_particles(*state0)
-        _, info1, _ = one_step(t_state0, opts(args.precision, args.pc))
-        e, p = force_err(ctx.particles(), ref)
-        acc = {"rel_rms_force_err": e, "max_abs_pos_err": p,
-               "vs": "FP64 GPU step, same state and solver settings",
-               "poisson_converged": [info64.reason_phi > 0, info1.reason_phi > 0],
+        log("accuracy leg")
+        ctx.checkpoint(0, restore=True)
+        per_step, conv = [], []
+        nacc = len(sched) if args.accuracy_steps is None else min(args.accuracy_steps, len(sched))
+        for j in range(nacc):
+            ctx.checkpoint(1)  # the state before step j
+            info64, _ = one_step(j, opts(0, args.pc))
+            ref = ctx.particles()
+            if j == 0:
+                ref0 = ref
+            ctx.checkpoint(2)  # the FP64 state after step j
+            ctx.checkpoint(1, restore=True)
+            info1, _ = one_step(j, opts(args.precision, args.pc))
+            e, p = force_err(ctx.particles(), ref)
+            per_step.append({"a": round(sched[j][0], 6), "dt": sched[j][1],
+                             "rel_rms_force_err": e, "max_abs_pos_err": p})
+            conv.append([info64.reason_phi > 0, info1.reason_phi > 0])
+            ctx.checkpoint(2, restore=True)  # continue along the FP64 trajectory
+        acc = {"rel_rms_force_err": max(r["rel_rms_force_err"] for r in per_step),
+               "max_abs_pos_err": max(r["max_abs_pos_err"] for r in per_step),
+               "per_step": per_step,
+               "vs": "FP64 GPU step from the same state (the FP64 trajectory), same solver settings",
+               "poisson_converged": conv,
                "amg_levels": ctx.amg_levels() if args.pc == "amg" else None}
 
-    # (2) the reference configuration: FP64, Jacobi, same GMRES limits
+    # (2) the reference configuration: FP64, Jacobi, same GMRES limits; one
+    # untimed step first so that its first-use allocations are not timed
     refcfg = None
     if not args.no_ref_config and args.pc != "jacobi":
-        ctx.set_particles(*state0)
+        log("reference-configuration leg")
+        ctx.checkpoint(0, restore=True)
+        one_step(0, opts(0, "jacobi"))
+        ctx.checkpoint(0, restore=True)
         ctx.sync()
         t0 = time.perf_counter()
-        _, infor, _ = one_step(t_state0, opts(0, "jacobi"))
+        infor, _ = one_step(0, opts(0, "jacobi"))
         ctx.sync()
         wall_r = time.perf_counter() - t0
+        converged = infor.reason_rho > 0 and infor.reason_phi > 0
         refcfg = {"value": round(N / wall_r, 1), "ms_per_step": round(1e3 * wall_r, 2),
                   "gmres_its": (infor.its_rho, infor.its_phi),
                   "poisson_converged": infor.reason_phi > 0,
-                  "note": "FP64, Jacobi GMRES(%d), max %d iterations; one step (includes its "
-                          "first-use allocations)" % (args.restart, args.max_it)}
-        if ref is not None:
-            e, p = force_err(ctx.particles(), ref)
+                  "note": "FP64, Jacobi GMRES(%d), max %d iterations; the first schedule step, "
+                          "timed after one untimed step%s"
+                          % (args.restart, args.max_it,
+                             "" if converged else "; NOT converged: this is the time to reach "
+                             "max_it, not a time to solution")}
+        if ref0 is not None:
+            e, p = force_err(ctx.particles(), ref0)
             refcfg["rel_rms_force_err_vs_converged_fp64"] = e
 
     def timed(precision, profile=False):
-        """K steps from state0 after W warmup steps, kernel timers off; with
-        profile, one more step with the library's kernel timers on (HIP
-        events on its stream, launches issued directly) for the roofline."""
-        ctx.set_particles(*state0)
+        """K schedule steps after W warmup steps, from the state after the
+        half drift, kernel timers off; with profile, one more step with the
+        library's kernel timers on (HIP events on its stream, launches issued
+        directly) for the roofline."""
+        log("timed leg, precision %d: %d warmup + %d steps" % (precision, W, K))
+        ctx.checkpoint(0, restore=True)
         o = opts(precision, args.pc)
-        t = t_state0
-        for _ in range(W):
-            t, _, _ = one_step(t, o)
+        for j in range(W):
+            walk(j, o)
         ctx.sync()
-        infos, phases = [], np.zeros(3)
+        infos, dts, phases = [], [], np.zeros(3)
         t0 = time.perf_counter()
-        for _ in range(K):
-            t, info, timing = one_step(t, o)
+        for j in range(W, W + K):
+            info, timing, dt = walk(j, o)
             infos.append((info.its_rho, info.its_phi))
+            dts.append(dt)
             phases += timing
         ctx.sync()
         wall = time.perf_counter() - t0
+        if not all(d > 0 for d in dts):
+            raise RuntimeError("bench: a timed step had dt <= 0: %r" % dts)
         kt = None
         if profile:
             ctx.set_timing(True)
             ctx.reset_kernel_times()
-            one_step(t, o)
+            walk(W + K, o)
             ctx.sync()
             kt = ctx.kernel_times()
             ctx.set_timing(False)
-        return wall, infos, phases, kt, info.nc + info.nv
+        return wall, infos, dts, phases, kt, info.nc + info.nv
 
     fp64 = None
     if not args.no_fp64 and args.precision != 0:
-        wall64, infos64, _, _, _ = timed(0)
+        wall64, infos64, dts64, _, _, _ = timed(0)
         fp64 = {"value": round(N * K / wall64, 1), "ms_per_step": round(1e3 * wall64 / K, 2),
                 "gmres_its": infos64, "poisson_pc": args.pc}
-    wall, infos, phases, kt, M = timed(args.precision, profile=True)
+    wall, infos, dts, phases, kt, M = timed(args.precision, profile=True)
 
-    # roofline of the dominant kernel: largest total device time among the
-    # kernels whose launches carry algorithmic byte counts
-    best = None
-    for name, (avg_ms, cnt, avg_bytes) in kt.items():
-        if cnt == 0 or avg_bytes <= 0 or name == "stream_copy":
-            continue
-        tot = avg_ms * cnt
-        if best is None or tot > best[3]:
-            best = (name, avg_bytes, avg_ms, tot, cnt)
-    roof = None
+    # roofline of the dominant kernel: the largest total device time (average
+    # launch time x launches) over every timed kernel; each carries its
+    # algorithmic bytes per launch.  The multigrid cycle and setup timers
+    # enclose other timed launches and are not kernels.
+    enclosing = {"amg_cycle", "amg_cycle32", "amg_setup", "stream_copy"}
+    ranked = sorted(((v[0] * v[1], k, v) for k, v in kt.items()
+                     if v[1] > 0 and k not in enclosing), reverse=True)
     copy_gbs = ctx.stream_copy(2 ** 30, reps=10)
-    if best:
-        name, b, avg_ms, _, cnt = best
+
+    def roof_entry(name, avg_ms, cnt, b):
+        if b <= 0:
+            return {"kernel": name, "bound": "latency", "avg_launch_ms": round(avg_ms, 4),
+                    "launches": cnt, "total_ms": round(avg_ms * cnt, 3)}
         ach = b / (avg_ms * 1e-3) / 1e9
-        roof = {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+        return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                 "traffic": traffic_from_profiles(name, args.workload, b),
                 "bytes_per_launch": int(b), "avg_launch_ms": round(avg_ms, 4), "launches": cnt,
-                "measured_copy_GBs": round(copy_gbs, 1)}
+                "total_ms": round(avg_ms * cnt, 3)}
+
+    roof, others = None, []
+    for q, (_, name, (avg_ms, cnt, b)) in enumerate(ranked[:6]):
+        e = roof_entry(name, avg_ms, cnt, b)
+        if q == 0:
+            roof = e
+            roof["measured_copy_GBs"] = round(copy_gbs, 1)
+        else:
+            others.append(e)
 
     value = N * K / wall
     out = {
@@ -267,6 +339,9 @@ def main():
                                            if args.pc == "amg" else
                                            "fp32 inner gmres, fp64 refinement")
                              if args.precision == 1 else "fp64"},
+                   "schedule": "a=%g..%g, %d steps (last dt shortened to a_end), restarted "
+                               "from the initial state when used up" % (a_start, a_end, len(sched)),
+                   "timed_dt": dts,
                    "gmres_its": infos, "phase_s": [round(v, 3) for v in phases]},
         "fp64_gpu": fp64,
         "speedup_vs_fp64": round(value / fp64["value"], 3) if fp64 else None,
@@ -274,6 +349,7 @@ def main():
         "speedup_vs_reference_config": round(value / refcfg["value"], 3) if refcfg else None,
         "accuracy": acc,
         "roofline": roof,
+        "roofline_next": others,
         "cpu_baseline": None,
     }
     if args.verbose:
@@ -283,39 +359,81 @@ def main():
     ctx.close()
 
     if not args.no_cpu:
+        log("CPU baseline leg")
         out["cpu_baseline"] = cpu_baseline(args, w)
     print(json.dumps(out))
+
+
+def schedule(a_start, a_end, da):
+    """(t, dt) of every step of the reference's loop after the initial half
+    drift (controller.cpp:106-208): t += dt accumulates, the last dt is cut
+    to end at a_end."""
+    out, t = [], a_start + 0.5 * da
+    while t < a_end:
+        dt = da
+        if t + dt > a_end:
+            dt = a_end - t
+        out.append((t, dt))
+        t += dt
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(args, w):
     """The oracle (the FP64 CPU restatement, OpenMP where the reference has
     it) timed on this host on a bounded sample: one FP64 step of the same
-    workload generator at --cpu-side^3 particles, same solver settings."""
+    workload generator at --cpu-side^3 particles, same solver settings, on
+    this process's share of the host's cores and on one thread."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as po
 
     n_side = args.cpu_side
     x, y, z, vx, vy, vz = make_ic(w, n_side)
     N = len(x)
-    cores = min(16, os.cpu_count() or 1)  # this process's share of a one-GPU host
-    p = po.make_params(bbox=(0, 1, 0, 1, 0, 1), depth=20, bc=1, c=0.0, lm=27, units=2, omega=1.0,
-                       mass_per_particle=1.0 / N, rtol=args.rtol, restart=args.restart,
-                       max_it=args.max_it, nthreads=cores, leaf_capacity=8)
-    o = po.Oracle(p)
-    o.set_particles(x, y, z, vx, vy, vz)
-    a = 0.02 + 0.004
-    t0 = time.perf_counter()
-    o.build_tree()
-    S = o.select()
-    o.set_constants(1.0 / N, 1.5 / a, -1.0)
-    o.solve()
-    o.move_velocity(1.0, 0.008)
-    o.move_position(1.0, 0.008)
-    wall = time.perf_counter() - t0
+    host = os.cpu_count() or 1
+    cores = min(16, host)  # a one-GPU job's share of the host
+
+    def run(threads):
+        p = po.make_params(bbox=(0, 1, 0, 1, 0, 1), depth=20, bc=1, c=0.0, lm=27, units=2,
+                           omega=1.0, mass_per_particle=1.0 / N, rtol=args.rtol,
+                           restart=args.restart, max_it=args.max_it, nthreads=threads,
+                           leaf_capacity=8)
+        o = po.Oracle(p)
+        o.set_particles(x, y, z, vx, vy, vz)
+        a = 0.02 + 0.004
+        t0 = time.perf_counter()
+        o.build_tree()
+        S = o.select()
+        t1 = time.perf_counter()
+        o.set_constants(1.0 / N, 1.5 / a, -1.0)
+        o.solve()
+        t2 = time.perf_counter()
+        o.move_velocity(1.0, 0.008)
+        o.move_position(1.0, 0.008)
+        t3 = time.perf_counter()
+        return S, t3 - t0, (t1 - t0, t2 - t1, t3 - t2)
+
+    S, wall, ph = run(cores)
+    _, wall1, ph1 = run(1)
     return {"value": round(N / wall, 1), "unit": "particle-steps/s", "cores": cores,
             "kind": "port",
-            "sample": "1 FP64 step (Jacobi GMRES, the reference's solver), %d^3 particles from "
-                      "the %s generator (S=%d nodes), %.1f s" % (n_side, args.workload, S, wall)}
+            "sample": "1 FP64 step (Jacobi GMRES(%d), max %d iterations: the reference's solver) "
+                      "of %d^3 particles from the %s generator (S=%d nodes): %.1f s on %d threads "
+                      "(select %.2f, solve %.2f, move %.2f s)"
+                      % (args.restart, args.max_it, n_side, args.workload, S, wall, cores, *ph),
+            "single_thread": {"value": round(N / wall1, 1), "cores": 1, "wall_s": round(wall1, 2),
+                              "phase_s": [round(v, 3) for v in ph1]},
+            "host_cpus": host, "cpu_model": cpu_model()}
 
 
 if __name__ == "__main__":

@@ -208,7 +208,10 @@ int apc_freeze_topology(apc_ctx *ctx, int on);
 /* Particle state in caller order into the frozen device order. */
 int apc_set_state(apc_ctx *ctx, const double *x, const double *y, const double *z,
                   const double *vx, const double *vy, const double *vz);
-/* Save (restore = 0) or restore device-side copies of (x, v), slot 0..63. */
+/* Save (restore = 0) or restore device-side copies of (x, v), slot 0..63.
+ * Outside the frozen form the slot also keeps the particle order (caller
+ * indices), and a restore drops the tree built since: the next apc_step or
+ * apc_octree_build starts from the saved state, as if it had been uploaded. */
 int apc_checkpoint(apc_ctx *ctx, int slot, int restore);
 /* Loss = sum over a periodic n^3 grid of (CIC density / mean - 1)^2 of the
  * current positions; seeds the adjoint state with lambda_x = dL/dx,
