@@ -36,6 +36,7 @@ EXPORTS = [
     "apc_adjoint_kick", "apc_adjoint_fields", "apc_gradient_adjoint", "apc_solve_adjoint",
     "apc_batch_create", "apc_batch_destroy", "apc_batch_size", "apc_batch_box",
     "apc_batch_last_error", "apc_batch_set_particles", "apc_batch_run",
+    "apc_morton_sort_noalloc",
 ]
 
 
@@ -116,6 +117,8 @@ def lib():
         L.apc_batch_last_error.argtypes = [C.c_void_p]
         L.apc_batch_last_error.restype = C.c_char_p
         L.apc_batch_set_particles.argtypes = [C.c_void_p, C.c_int, C.c_int] + [_D] * 6
+        L.apc_morton_sort_noalloc.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, _D,
+                                              C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.apc_batch_run.argtypes = [C.c_void_p, C.c_int, _D, _D, C.POINTER(KspOpts),
                                     C.POINTER(KspOpts), C.POINTER(SolveInfo), _I]
         L.apc_stream_copy.argtypes = [C.c_void_p, C.c_double, C.c_int, _D]

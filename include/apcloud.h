@@ -243,6 +243,25 @@ int apc_gradient_adjoint(apc_ctx *ctx, const double *dg, const double *dH, doubl
 int apc_solve_adjoint(apc_ctx *ctx, int which, const apc_ksp_opts *opts, const double *rhs,
                       double *out, int *its, int *reason);
 
+/* ---- No allocation ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the Morton keys and stable key sort of apc_octree_build
+ * (computeKey64 octree.cpp:676-852, the sort of buildOctree octree.cpp:111-
+ * 494), read back by apc_get_sorted, on caller device arrays: x, y, z (n
+ * doubles each) in, keys (n) and index (n, the caller position of each
+ * sorted entry) out, bbox (6 doubles, host memory) and depth (1..21) as in
+ * apc_params.  Launches on `stream` (a hipStream_t; NULL = the default
+ * stream) and returns without waiting.  Its workspace is a fixed device
+ * array of the library, about 25 MB per GPU, held until the process ends:
+ * n <= APC_NOALLOC_MAX_N, larger n is rejected with APC_ERR_ARG
+ * (apc_last_error(NULL) gives the text).  Every call on one GPU shares that
+ * array: calls that overlap on one GPU race. */
+#define APC_NOALLOC_MAX_N (1 << 21)
+int apc_morton_sort_noalloc(int n, const double *x, const double *y, const double *z,
+                            const double *bbox, int depth, uint64_t *keys, int *index,
+                            void *stream);
+
 /* Per-kernel device time of the timed calls (HIP events on the context's
  * stream): average ms and algorithmic bytes per launch, launch counts; names
  * are NUL-separated.  Returns the number of kernels. */
