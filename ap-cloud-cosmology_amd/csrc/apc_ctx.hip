@@ -176,7 +176,7 @@ int apc_destroy(apc_ctx *c) {
   for (auto &b : c->ckpt) b.release(c->s);
   for (auto &b : c->ckpt_id) b.release(c->s);
   c->key.release(c->s); c->key2.release(c->s); c->nprefix.release(c->s); c->red_u64.release(c->s);
-  c->visits.release(c->s);
+  c->visits.release(c->s); c->scodes.release(c->s); c->slen.release(c->s); c->sidx.release(c->s);
   c->tr_key.release(c->s); c->tr_key2.release(c->s); c->cic_key.release(c->s); c->cic_key2.release(c->s);
   c->vcode.release(c->s); c->vcode2.release(c->s);
   hipStreamDestroy(c->s);

@@ -127,7 +127,9 @@ struct apc_ctx {
   int search_waves = 0;        // resident waves of the wave-per-node search
   int search_pen = 0;          // its penalty table is on the device
   apc::DBuf<int> status_dev;
-  apc::DBuf<unsigned long long> visits;   // Neigh24 entries the last search visited
+  apc::DBuf<unsigned long long> visits;   // Neigh24 entries visited, list entries sorted
+  apc::DBuf<uint8_t> scodes, slen;        // per node: sort codes of its list, list length
+  apc::DBuf<int> sidx;                    // per node: its list (node indices)
   unsigned long long search_visits = 0;
   apc::SolveState *gm = nullptr;
   // reductions

@@ -327,6 +327,7 @@ __global__ void __launch_bounds__(256) k_search(int M, int S, int nrows, const i
 //  Nodes whose list would exceed WCAP entries are queued for the per-thread
 //  search with the full bound.
 constexpr int WCAP = 256;   // list entries held in LDS (< 65536: ranges pack in 16 bits)
+constexpr int SCAP = 128;   // lists up to this long are sorted by k_search_sort (codes < 256)
 constexpr int WHASH = 512;  // membership hash slots (power of two, >= 2 WCAP)
 
 struct WaveLds {
@@ -561,12 +562,16 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
                                                     const int *__restrict__ n24cnt,
                                                     int *__restrict__ nbr, double *__restrict__ dist,
                                                     int *__restrict__ status, int *__restrict__ ovf,
-                                                    unsigned long long *__restrict__ visits) {
+                                                    unsigned long long *__restrict__ visits,
+                                                    uint8_t *__restrict__ scodes,
+                                                    int *__restrict__ sidx,
+                                                    uint8_t *__restrict__ slen) {
   __shared__ WaveLds L;
   // Neigh24 entries this wave visited (the node's own and every expansion
   // round's candidates): the search's algorithmic traffic is 28 bytes per
   // visit (index + centre) plus its output
   unsigned long long nvis = 0;
+  unsigned long long nlist = 0;  // list entries handed to k_search_sort
   const int lane = threadIdx.x;
   const int nq = 10 * lm;
   // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
@@ -739,7 +744,10 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
       return;
     }
     if (overflow) {
-      if (lane == 0) ovf[atomicAdd(&status[2], 1)] = i0;
+      if (lane == 0) {
+        ovf[atomicAdd(&status[2], 1)] = i0;
+        slen[i0] = 0;  // searched again by k_search
+      }
       __syncthreads();
       continue;
     }
@@ -792,17 +800,181 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
     if (nn[0] > 1) wave_qsort(L, 0, nn[0] - 1, nn[0]);
     for (int q = lane; q < nn[0]; q += 64) L.d2[q] = c_pen[q] * L.d2[q];
     __syncthreads();
-    wave_qsort(L, 0, l - 1, lm);  // positions 0..lm-1 are the output
-    for (int q = lane; q < lm; q += 64) {
-      const int m = L.mo[q];
-      double dd;
-      (void)place_w(F, xyz, x0, y0, z0, m, &dd);
-      nbr[(size_t)q * M + i0] = m;
-      dist[(size_t)q * M + i0] = sqrt(dd);
+    if (l <= SCAP) {
+      // The final quicksort goes to k_search_sort, one lane per node.  It
+      // compares only the order of the keys, so each key is replaced by the
+      // number of keys strictly below it: equal keys get equal codes, a
+      // larger key a larger code, and the quicksort's permutation of the
+      // codes is its permutation of the keys.
+      const double k0 = (lane < l) ? L.d2[lane] : 0.0;
+      const double k1 = (64 + lane < l) ? L.d2[64 + lane] : 0.0;
+      int c0 = 0, c1 = 0;
+      if (l <= 64) {
+        for (int j = 0; j < l; ++j) c0 += (L.d2[j] < k0) ? 1 : 0;
+      } else {
+        for (int j = 0; j < l; ++j) {
+          const double kj = L.d2[j];
+          c0 += (kj < k0) ? 1 : 0;
+          c1 += (kj < k1) ? 1 : 0;
+        }
+      }
+      uint8_t *oc = scodes + (size_t)i0 * SCAP;
+      int *oi = sidx + (size_t)i0 * SCAP;
+      if (lane < l) {
+        oc[lane] = (uint8_t)c0;
+        oi[lane] = L.mo[lane];
+      }
+      if (64 + lane < l) {
+        oc[64 + lane] = (uint8_t)c1;
+        oi[64 + lane] = L.mo[64 + lane];
+      }
+      if (lane == 0) slen[i0] = (uint8_t)l;
+      nlist += (unsigned)l;
+    } else {
+      wave_qsort(L, 0, l - 1, lm);  // positions 0..lm-1 are the output
+      for (int q = lane; q < lm; q += 64) {
+        const int m = L.mo[q];
+        double dd;
+        (void)place_w(F, xyz, x0, y0, z0, m, &dd);
+        nbr[(size_t)q * M + i0] = m;
+        dist[(size_t)q * M + i0] = sqrt(dd);
+      }
+      if (lane == 0) slen[i0] = 0;  // done here
     }
     __syncthreads();
   }
   if (lane == 0 && nvis) atomicAdd(visits, nvis);  // one atomic per wave
+  if (lane == 0 && nlist) atomicAdd(visits + 1, nlist);
+}
+
+// The final quicksort of k_search_wave's lists of at most SCAP entries, one
+// lane per node.  Each lane keeps its node's codes (order-equivalent keys,
+// all below 128) and the list positions they carry in LDS rows of its own,
+// and runs the reference's partition scans (apartition, solver.cpp:600-632)
+// four codes per 32-bit read: for codes b, p < 128, b + (127 - p) sets bit 7
+// of its byte exactly when b > p and never carries into the next byte.
+// Ranges lying wholly at or above lm are left unsorted (nothing below them
+// changes).  The output rows are written slot-major by consecutive lanes,
+// i.e. coalesced.
+constexpr int SSTR = SCAP + 8;  // bytes per lane row (8-byte aligned; spreads rows over banks)
+constexpr int SSTK = 12;        // > log2(SCAP) + 1 pending ranges per lane
+
+__device__ inline uint32_t row_u32(const uint8_t *row, int d) {
+  return *reinterpret_cast<const uint32_t *>(row + 4 * d);
+}
+
+// first x in [x0, r] with a[x] > p, or r + 1 (the reference's i-scan)
+__device__ inline int scan_gt(const uint8_t *a, int x0, int r, int p) {
+  const uint32_t add = 0x01010101u * (uint32_t)(127 - p);
+  int d = x0 >> 2;
+  uint32_t m = ((row_u32(a, d) + add) & 0x80808080u) & (0xffffffffu << (8 * (x0 & 3)));
+  while (m == 0) {
+    ++d;
+    if (4 * d > r) return r + 1;
+    m = (row_u32(a, d) + add) & 0x80808080u;
+  }
+  const int x = 4 * d + (__builtin_ctz(m) >> 3);
+  return x <= r ? x : r + 1;
+}
+
+// last y <= y0 with a[y] <= p; a[lo] == p stops it at lo (the j-scan)
+__device__ inline int scan_le(const uint8_t *a, int y0, int p) {
+  const uint32_t add = 0x01010101u * (uint32_t)(127 - p);
+  int d = y0 >> 2;
+  uint32_t m = (~(row_u32(a, d) + add) & 0x80808080u) & (0xffffffffu >> (8 * (3 - (y0 & 3))));
+  while (m == 0) {
+    --d;
+    m = ~(row_u32(a, d) + add) & 0x80808080u;
+  }
+  return 4 * d + ((31 - __builtin_clz(m)) >> 3);
+}
+
+__device__ inline int code_part(uint8_t *a, uint8_t *b, int lo, int hi) {
+  const int p = a[lo];
+  int i = lo, j = hi + 1;
+  for (;;) {
+    i = scan_gt(a, i + 1, hi, p);
+    j = scan_le(a, j - 1, p);
+    if (i >= j) break;
+    const uint8_t ta = a[i]; a[i] = a[j]; a[j] = ta;
+    const uint8_t tb = b[i]; b[i] = b[j]; b[j] = tb;
+  }
+  const uint8_t ta = a[lo]; a[lo] = a[j]; a[j] = ta;
+  const uint8_t tb = b[lo]; b[lo] = b[j]; b[j] = tb;
+  return j;
+}
+
+__global__ void __launch_bounds__(64) k_search_sort(int M, int lm, Frame F,
+                                                    const double4 *__restrict__ xyz,
+                                                    const uint8_t *__restrict__ scodes,
+                                                    const int *__restrict__ sidx,
+                                                    const uint8_t *__restrict__ slen,
+                                                    int *__restrict__ nbr,
+                                                    double *__restrict__ dist) {
+  __shared__ __align__(16) uint8_t sa[64 * SSTR];
+  __shared__ __align__(16) uint8_t sb[64 * SSTR];
+  __shared__ uint16_t stk[SSTK * 64];
+  const int t = threadIdx.x;
+  const int i0 = blockIdx.x * 64 + t;
+  if (i0 >= M) return;
+  const int l = slen[i0];
+  if (l == 0) return;  // sorted by k_search_wave or by the long-list pass
+  uint8_t *a = sa + t * SSTR;
+  uint8_t *b = sb + t * SSTR;
+  // the codes, bytes at and past l zeroed (the scans may read them); the
+  // positions 0..l-1 as payload
+  const uint2 *src = reinterpret_cast<const uint2 *>(scodes + (size_t)i0 * SCAP);
+  for (int w = 0; 8 * w < l; ++w) {
+    uint2 v = src[w];
+    const int left = l - 8 * w;  // codes in this 8-byte word
+    if (left < 8) {
+      const uint64_t keep = (left <= 0) ? 0ull : (~0ull >> (64 - 8 * left));
+      uint64_t u = ((uint64_t)v.y << 32) | v.x;
+      u &= keep;
+      v = make_uint2((uint32_t)u, (uint32_t)(u >> 32));
+    }
+    *reinterpret_cast<uint2 *>(a + 8 * w) = v;
+    const uint32_t base = 0x03020100u + 0x08080808u * (uint32_t)w;
+    *reinterpret_cast<uint2 *>(b + 8 * w) = make_uint2(base, base + 0x04040404u);
+  }
+  int top = 0;
+  stk[t] = (uint16_t)(0 | ((l - 1) << 8));
+  top = 1;
+  while (top > 0) {
+    const int pk = stk[(--top) * 64 + t];
+    int lo = pk & 0xff, hi = pk >> 8;
+    while (lo < hi && lo < lm) {
+      const int j = code_part(a, b, lo, hi);
+      if (j - lo < hi - j) {
+        stk[(top++) * 64 + t] = (uint16_t)((j + 1) | (hi << 8));
+        hi = j - 1;
+      } else {
+        stk[(top++) * 64 + t] = (uint16_t)(lo | ((j - 1) << 8));
+        lo = j + 1;
+      }
+    }
+  }
+  // output, eight slots at a time: their indices, then their centres, are
+  // loaded together so the gathers overlap
+  const double4 p0 = xyz[i0];
+  const int *li = sidx + (size_t)i0 * SCAP;
+  for (int q0 = 0; q0 < lm; q0 += 8) {
+    int mm[8];
+    double4 pp[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) mm[k] = (q0 + k < lm) ? li[b[q0 + k]] : i0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) pp[k] = xyz[mm[k]];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (q0 + k < lm) {
+        double dd;
+        (void)place_p(F, pp[k], p0.x, p0.y, p0.z, &dd);
+        nbr[(size_t)(q0 + k) * M + i0] = mm[k];
+        dist[(size_t)(q0 + k) * M + i0] = sqrt(dd);
+      }
+    }
+  }
 }
 
 int search_neighbors(apc_ctx *c) {
@@ -814,8 +986,8 @@ int search_neighbors(apc_ctx *c) {
   APC_ALLOC(c, c->dist, (size_t)lm * M + 1);
   APC_ALLOC(c, c->status_dev, 4);
   APC_HIP(c, hipMemsetAsync(c->status_dev.p, 0, 4 * sizeof(int), c->s));
-  APC_ALLOC(c, c->visits, 1);
-  APC_HIP(c, hipMemsetAsync(c->visits.p, 0, sizeof(unsigned long long), c->s));
+  APC_ALLOC(c, c->visits, 2);
+  APC_HIP(c, hipMemsetAsync(c->visits.p, 0, 2 * sizeof(unsigned long long), c->s));
   Frame F;
   {  // solver.cpp:464-475, computed on the host
     double alpha = 0.01, beta = 0.001, gamma = 0.0001;
@@ -879,6 +1051,9 @@ int search_neighbors(apc_ctx *c) {
                      c->cz.p, xyz);
   int search_scope = -1;  // timer of the wave search; its bytes are known after the launch
   if (wave) {
+    APC_ALLOC(c, c->scodes, (size_t)M * SCAP + 16);
+    APC_ALLOC(c, c->sidx, (size_t)M * SCAP + 4);
+    APC_ALLOC(c, c->slen, (size_t)M + 16);
     if (!c->search_pen) {  // per context: the constant table lives on its device
       static const std::array<double, WCAP> pen = [] {
         std::array<double, WCAP> t{};
@@ -906,7 +1081,8 @@ int search_neighbors(apc_ctx *c) {
     }();
     hipLaunchKernelGGL(k_search_wave, dim3(grid), dim3(64), 0, c->s, M, S, lm,
                        std::min(fast_cap, WCAP), xchunk, F, (const double4 *)xyz, c->n24.p, c->n24cnt.p,
-                       c->nbr.p, c->dist.p, c->status_dev.p, c->novf.p, c->visits.p);
+                       c->nbr.p, c->dist.p, c->status_dev.p, c->novf.p, c->visits.p,
+                       c->scodes.p, c->sidx.p, c->slen.p);
   } else {
     KScope ks(c, "neighbor_search");
     hipLaunchKernelGGL(k_search, dim3(nblocks(slots, 256)), dim3(256), 0, c->s, M, S, M,
@@ -914,16 +1090,28 @@ int search_neighbors(apc_ctx *c) {
                        c->status_dev.p, c->novf.p);
   }
   int st[3] = {0, 0, 0};
-  unsigned long long nvis = 0;
+  unsigned long long cnt2[2] = {0, 0};
   APC_HIP(c, hipMemcpyAsync(st, c->status_dev.p, sizeof(st), hipMemcpyDeviceToHost, c->s));
-  APC_HIP(c, hipMemcpyAsync(&nvis, c->visits.p, sizeof(nvis), hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipMemcpyAsync(cnt2, c->visits.p, sizeof(cnt2), hipMemcpyDeviceToHost, c->s));
   APC_HIP(c, hipStreamSynchronize(c->s));
-  c->search_visits = nvis;
-  // algorithmic bytes: 28 per Neigh24 entry visited (its index and the
-  // centre of the node it names), the node's own centre and count, and the
-  // lm indices and distances written
-  if (search_scope >= 0)
-    ktime_bytes(c, search_scope, 28.0 * (double)nvis + (28.0 + 12.0 * lm) * (double)M);
+  const double nvis = (double)cnt2[0], nlist = (double)cnt2[1];
+  c->search_visits = cnt2[0];
+  // algorithmic bytes of the wave search: 28 per Neigh24 entry visited (its
+  // index and the centre of the node it names), the node's own centre and
+  // count, and per list entry handed on its code (1) and index (4)
+  if (search_scope >= 0) ktime_bytes(c, search_scope, 28.0 * nvis + 28.0 * M + 5.0 * nlist);
+  // the final sorts (every list in the wave search's LDS was complete: no
+  // error was raised, so every node's length entry is this search's)
+  if (wave && st[0] == 0) {
+    // per node: length 1, codes (read whole), per output slot the index
+    // (4), the neighbour's centre (24), index + distance written (12), own
+    // centre 24
+    KScope ks(c, "neighbor_sort", nlist + (25.0 + 40.0 * lm) * M);
+    hipLaunchKernelGGL(k_search_sort, dim3(nblocks(M, 64)), dim3(64), 0, c->s, M, lm, F,
+                       (const double4 *)xyz, c->scodes.p, c->sidx.p, c->slen.p, c->nbr.p,
+                       c->dist.p);
+    APC_HIP(c, hipGetLastError());
+  }
   // nodes whose expansion outgrew the short list, with the full bound
   if (st[0] == 0 && st[2] > 0) {
     const int rows = st[2];

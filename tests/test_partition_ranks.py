@@ -159,3 +159,42 @@ def test_a_plausible_slip_is_caught():
         if K > 0 and j != int(lpos[K]):
             bad += 1
     assert bad > 0
+
+
+def codes_of(wd):
+    """The sort kernel's keys: for each entry, the number of entries strictly
+    below it."""
+    return np.array([(wd < v).sum() for v in wd], dtype=np.int64)
+
+
+def test_quicksort_on_codes_equals_quicksort_on_keys():
+    """The final quicksort runs on the codes instead of the penalised keys
+    (one lane per node, one byte per key): the permutation of the payload must
+    be the one the reference's quicksort gives on the keys themselves, ties
+    and the stop at the output positions included."""
+    for rng, wd in _arrays(5, 3000):
+        n = len(wd)
+        need = int(rng.integers(1, n + 1))
+        a, b = wd.copy(), np.arange(n)
+        qsort_scans(a, b, 0, n - 1)
+        cd = codes_of(wd)
+        assert cd.max() < 256
+        c, d = cd.astype(float), np.arange(n)
+        qsort_ranks(c, d, 0, n - 1, need=need, small=n, rng=rng)  # all by the scans, stopped
+        assert np.array_equal(b[:need], d[:need])
+
+
+def test_codes_that_merge_distinct_keys_are_caught():
+    """A lossy code (here: keys rounded before they are ranked, which merges
+    distinct keys) changes the permutation on these inputs, so the identity
+    above is not vacuous."""
+    bad = 0
+    for rng, wd in _arrays(6, 2000):
+        n = len(wd)
+        a, b = wd.copy(), np.arange(n)
+        qsort_scans(a, b, 0, n - 1)
+        merged = np.round(wd / 4.0) * 4.0  # distinct keys merged
+        c, d = merged.copy(), np.arange(n)
+        qsort_scans(c, d, 0, n - 1)
+        bad += int(not np.array_equal(b, d))
+    assert bad > 0
