@@ -36,7 +36,7 @@ EXPORTS = [
     "apc_adjoint_kick", "apc_adjoint_fields", "apc_gradient_adjoint", "apc_solve_adjoint",
     "apc_batch_create", "apc_batch_destroy", "apc_batch_size", "apc_batch_box",
     "apc_batch_last_error", "apc_batch_set_particles", "apc_batch_run",
-    "apc_morton_sort_noalloc",
+    "apc_morton_sort_noalloc", "apc_set_comp",
 ]
 
 
@@ -95,6 +95,8 @@ def lib():
         L.apc_get_comp_sizes.argtypes = [C.c_void_p, _I, _I, _I, _I]
         L.apc_get_comp.argtypes = [C.c_void_p, C.POINTER(CompView)]
         L.apc_neighbors.argtypes = [C.c_void_p, _I, _D]
+        L.apc_set_comp.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [_D] * 4 + \
+            [_I] * 5 + [C.c_int]
         L.apc_solver_set_neighbors.argtypes = [C.c_void_p, C.c_int]
         L.apc_solver_set_constants.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
         L.apc_build_system.argtypes = [C.c_void_p, _I] + [_D] * 6
@@ -325,6 +327,20 @@ class Context:
                    fneigh=i["first_neigh"], nneigh=i["number_neigh"], neigh24=n24[:nn.value])
         out.update(f)
         return out
+
+    def set_comp(self, nc, nv, nb, x, y, z, h, np_, neigh):
+        """Caller-filled computational particles (apc_set_comp); neigh: one
+        list of Neigh24 entries per row (nc + nv rows; boundary rows none)."""
+        S = nc + nv + nb
+        d = [np.ascontiguousarray(a, np.float64) for a in (x, y, z, h)]
+        npp = np.ascontiguousarray(np_, np.int32)
+        typ = np.array([0] * nc + [-1] * nv + [1] * nb, np.int32)
+        cnt = np.array([len(neigh[i]) if i < nc + nv else 0 for i in range(S)], np.int32)
+        first = np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(np.int32)
+        flat = np.array([m for i in range(nc + nv) for m in neigh[i]] or [0], np.int32)
+        self._chk(self.L.apc_set_comp(self.h, nc, nv, nb, *[_p(a) for a in d], _p(npp, _I),
+                                      _p(typ, _I), _p(first, _I), _p(cnt, _I), _p(flat, _I),
+                                      int(cnt.sum())), "apc_set_comp")
 
     def neighbors(self):
         nc, nv, nb, nn = C.c_int(), C.c_int(), C.c_int(), C.c_int()

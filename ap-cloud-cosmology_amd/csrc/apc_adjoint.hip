@@ -345,7 +345,7 @@ static int need_frozen(apc_ctx *c, const char *what) {
 
 extern "C" int apc_freeze_topology(apc_ctx *c, int on) {
   if (!c) return APC_ERR_ARG;
-  if (on && (!c->have_comp || !c->sorted))
+  if (on && (!c->have_comp || !c->sorted || c->comp_external))
     return fail(c, APC_ERR_ARG, "freeze_topology: build the octree and select first");
   c->frozen = on ? 1 : 0;
   if (!on) c->have_tr = 0;

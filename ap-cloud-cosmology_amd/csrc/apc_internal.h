@@ -87,6 +87,7 @@ struct apc_ctx {
   // computational particles: S = nc+nv+nb, M = nc+nv
   int nc = 0, nv = 0, nb = 0, nneigh = 0;
   bool have_comp = false;
+  int comp_external = 0;               // set by apc_set_comp: no particles belong to the nodes
   apc::DBuf<int> leaf_node;            // M: node id of each output leaf
   apc::DBuf<double> cx, cy, cz, ch;    // S
   apc::DBuf<int> cnp, cfirst, ctype;   // S

@@ -100,6 +100,8 @@ static int read_max(apc_ctx *c, double *out) {
 
 int interp_kick(apc_ctx *c, double Cv, double dt, double *maxdv) {
   if (!c->have_fields) return fail(c, APC_ERR_ARG, "move_velocity: no node fields (solve first)");
+  if (c->comp_external)
+    return fail(c, APC_ERR_ARG, "move_velocity: the nodes were set by apc_set_comp, not selected");
   const int N = c->N, M = c->nc + c->nv;
   APC_ALLOC(c, c->red_u64, 4);
   APC_HIP(c, hipMemsetAsync(c->red_u64.p, 0, sizeof(unsigned long long), c->s));

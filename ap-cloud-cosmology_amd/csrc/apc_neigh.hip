@@ -167,6 +167,7 @@ __device__ int search_node(int i0, const Cand &c, int M, int S, int lm, int cap,
     const int cnt = n24cnt[i0];
     for (int q = 0; q < cnt; ++q) {
       int m = n24[(size_t)i0 * 24 + q];
+      if ((unsigned)m >= (unsigned)S) { status[0] = APC_ERR_NEIGH_INVALID; return -1; }
       double dd;
       int o = place(m, &dd);
       c.MO(l) = (uint32_t)m | ((uint32_t)o << 29);
@@ -198,7 +199,7 @@ __device__ int search_node(int i0, const Cand &c, int M, int S, int lm, int cap,
       int cnt = (src < M) ? n24cnt[src] : 0;
       for (int q = 0; q < cnt; ++q) {
         int m = n24[(size_t)src * 24 + q];
-        if (m >= S) { status[0] = APC_ERR_NEIGH_INVALID; return -1; }
+        if (m < 0 || m >= S) { status[0] = APC_ERR_NEIGH_INVALID; return -1; }
         if (m == i0 || listed(c, l, (uint32_t)m)) continue;
         double dd;
         int o = place(m, &dd);
@@ -604,6 +605,13 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
       const int cnt = n24cnt[i0];
       nvis += (unsigned)cnt;
       int m = -1, o = 8;
+      // an index outside the nodes (possible only in a caller-filled set) is
+      // an error here; the reference would read past its arrays
+      const bool bad = lane < cnt && (unsigned)n24[(size_t)i0 * 24 + lane] >= (unsigned)S;
+      if (__ballot(bad)) {
+        if (lane == 0) status[0] = APC_ERR_NEIGH_INVALID;
+        return;
+      }
       if (lane < cnt) {
         m = n24[(size_t)i0 * 24 + lane];
         double dd;
@@ -681,7 +689,7 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
         mC = mB;
         pC = pB;
         mB = mA;
-        const bool invalid = live && m >= S;
+        const bool invalid = live && (m < 0 || m >= S);
         // new to the list: the index is in no earlier chunk (nor the node's
         // Neigh24) and no earlier lane of this chunk holds it.  Every
         // candidate claims its index's slot and offers its tag to the slot's

@@ -606,6 +606,7 @@ int select_tree(apc_ctx *c, double cc, int order, int mindepth, int maxdepth) {
   c->nv = nv;
   c->nb = nb;
   c->have_comp = true;
+  c->comp_external = 0;
   c->have_tr = 0;
   c->have_nbr = c->have_sys = c->have_fields = 0;
   return 0;
@@ -698,5 +699,71 @@ extern "C" int apc_get_comp(apc_ctx *c, const apc_comp_view *v) {
     for (int q = 0; q < 9; ++q) APC_HIP(c, cpd(dst[q], c->grad.p + (size_t)q * M, M));
   }
   APC_HIP(c, hipStreamSynchronize(c->s));
+  return 0;
+}
+
+// Solver::Solver takes a Computational_particle that its caller filled
+// (solver.h:53, particle.h:23-53); the reference fills it by the octree
+// selection, but the Solver does not depend on that.  This loads such a
+// caller-filled set: rows nc computational (Type 0) then nv vacuum (Type -1),
+// then nb boundary nodes (Type 1); Neigh24 of row i is
+// neigh24[first_neigh[i] .. first_neigh[i] + number_neigh[i]).  The entries
+// themselves are checked where the neighbour search reads them, in the
+// reference's order (solver.cpp:511-518).  The set has no physical
+// particles: the mover entry points refuse it until the next selection.
+extern "C" int apc_set_comp(apc_ctx *c, int nc, int nv, int nb, const double *x, const double *y,
+                            const double *z, const double *cell_length, const int *np,
+                            const int *type, const int *first_neigh, const int *number_neigh,
+                            const int *neigh24, int nneigh) {
+  if (!c) return APC_ERR_ARG;
+  if (c->frozen) return fail(c, APC_ERR_ARG, "set_comp: the topology is frozen");
+  if (nc < 0 || nv < 0 || nb < 0 || nc + nv < 1 || nneigh < 0)
+    return fail(c, APC_ERR_ARG, "set_comp: bad sizes nc=%d nv=%d nb=%d", nc, nv, nb);
+  if (!x || !y || !z || !cell_length || !np || !type || !first_neigh || !number_neigh ||
+      (nneigh > 0 && !neigh24))
+    return fail(c, APC_ERR_ARG, "set_comp: null array");
+  const int M = nc + nv, S = M + nb;
+  for (int i = 0; i < S; ++i) {
+    const int want = i < nc ? 0 : (i < M ? -1 : 1);
+    if (type[i] != want)
+      return fail(c, APC_ERR_ARG, "set_comp: node %d has Type %d, expected %d", i, type[i], want);
+    if (!(cell_length[i] > 0.0))
+      return fail(c, APC_ERR_ARG, "set_comp: node %d has CellLength %g", i, cell_length[i]);
+  }
+  std::vector<int> cnt(M), pad((size_t)M * 24, 0);
+  for (int i = 0; i < M; ++i) {
+    const int f = first_neigh[i], k = number_neigh[i];
+    if (k < 0 || f < 0 || (long)f + k > nneigh)
+      return fail(c, APC_ERR_ARG, "set_comp: node %d Neigh24 range [%d, %d+%d) outside 0..%d", i,
+                  f, f, k, nneigh);
+    if (k > 24)
+      return fail(c, APC_ERR_UNSUPPORTED, "set_comp: node %d has %d Neigh24 entries (at most 24)",
+                  i, k);
+    cnt[i] = k;
+    for (int q = 0; q < k; ++q) pad[(size_t)i * 24 + q] = neigh24[f + q];
+  }
+  std::vector<int> zero(S, 0);
+  APC_ALLOC(c, c->cx, S); APC_ALLOC(c, c->cy, S); APC_ALLOC(c, c->cz, S); APC_ALLOC(c, c->ch, S);
+  APC_ALLOC(c, c->cnp, S); APC_ALLOC(c, c->cfirst, S); APC_ALLOC(c, c->ctype, S);
+  APC_ALLOC(c, c->n24, (size_t)M * 24);
+  APC_ALLOC(c, c->n24cnt, M);
+  APC_ALLOC(c, c->n24first, (size_t)M + 1);
+  APC_HIP(c, hipMemcpyAsync(c->cx.p, x, 8ull * S, hipMemcpyHostToDevice, c->s));
+  APC_HIP(c, hipMemcpyAsync(c->cy.p, y, 8ull * S, hipMemcpyHostToDevice, c->s));
+  APC_HIP(c, hipMemcpyAsync(c->cz.p, z, 8ull * S, hipMemcpyHostToDevice, c->s));
+  APC_HIP(c, hipMemcpyAsync(c->ch.p, cell_length, 8ull * S, hipMemcpyHostToDevice, c->s));
+  APC_HIP(c, hipMemcpyAsync(c->cnp.p, np, 4ull * S, hipMemcpyHostToDevice, c->s));
+  APC_HIP(c, hipMemcpyAsync(c->cfirst.p, zero.data(), 4ull * S, hipMemcpyHostToDevice, c->s));
+  APC_HIP(c, hipMemcpyAsync(c->ctype.p, type, 4ull * S, hipMemcpyHostToDevice, c->s));
+  APC_HIP(c, hipMemcpyAsync(c->n24.p, pad.data(), 4ull * M * 24, hipMemcpyHostToDevice, c->s));
+  APC_HIP(c, hipMemcpyAsync(c->n24cnt.p, cnt.data(), 4ull * M, hipMemcpyHostToDevice, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));  // the host vectors go out of scope
+  c->nc = nc;
+  c->nv = nv;
+  c->nb = nb;
+  c->have_comp = true;
+  c->comp_external = 1;
+  c->have_tr = 0;
+  c->have_nbr = c->have_sys = c->have_fields = 0;
   return 0;
 }

@@ -122,6 +122,19 @@ int apc_select(apc_ctx *ctx, double c, int order, int mindepth, int maxdepth, in
 int apc_get_comp_sizes(apc_ctx *ctx, int *nc, int *nv, int *nb, int *nneigh);
 int apc_get_comp(apc_ctx *ctx, const apc_comp_view *view);
 
+/* Solver::Solver over a caller-filled Computational_particle (solver.h:53,
+ * particle.h:23-53): loads nc computational (Type 0), nv vacuum (Type -1) and
+ * nb boundary (Type 1) nodes with their centres, CellLength,
+ * NumberOfPhysicalParticles and Neigh24 (row i: neigh24[first_neigh[i] ..
+ * + number_neigh[i]), at most 24 entries, nneigh entries in all) in place
+ * of a selection; the solver entry points below then work on them.  Neigh24
+ * entries are checked where the neighbour search reads them
+ * (APC_ERR_NEIGH_INVALID, solver.cpp:511-518).  There are no physical
+ * particles: apc_interp_kick refuses until the next apc_select. */
+int apc_set_comp(apc_ctx *ctx, int nc, int nv, int nb, const double *x, const double *y,
+                 const double *z, const double *cell_length, const int *np, const int *type,
+                 const int *first_neigh, const int *number_neigh, const int *neigh24, int nneigh);
+
 /* Solver::searchneighboroctant (solver.cpp:456-570) for every computational
  * and vacuum node: lm neighbour indices and distances, row-major (M x lm). */
 int apc_neighbors(apc_ctx *ctx, int *nbr, double *dist);

@@ -554,6 +554,9 @@ int search_octant(const Comp &cp, const Frame &F, int lm, int i0, std::vector<in
   const int total = cp.size();
   for (int q = cp.fneigh[i0]; q < cp.fneigh[i0] + cp.nneigh[i0]; ++q) {
     int m = cp.neigh24[q];
+    /* the reference does not check the node's own entries (:481-492) and
+     * would read past its arrays; a caller-filled set can hold such entries */
+    if (m < 0 || m >= total) return -4;
     int oct = octant(m);
     if (nn[oct] >= nq) return -2;
     tn[oct * nq + nn[oct]] = m;
@@ -572,7 +575,8 @@ int search_octant(const Comp &cp, const Frame &F, int lm, int i0, std::vector<in
       int src = snap[a];
       for (int q = cp.fneigh[src]; q < cp.fneigh[src] + cp.nneigh[src]; ++q) {
         int m = cp.neigh24[q];
-        if (m > total) return -4;
+        if (m > total || m < 0) return -4; /* :511-518 (m == total: unchecked there, UB) */
+        if (m == total) return -4;
         int oct = octant(m);
         int flag = 0;
         for (int k = 0; k < nn[oct]; ++k)
@@ -1176,6 +1180,26 @@ int orc_select(OrcSim *s, double c, int order, int mindepth, int maxdepth) {
   int S = sel.run(s->cp, c, order, mindepth, maxdepth, s->p.boundary_at_faces != 0);
   s->have_comp = true;
   return S;
+}
+
+int orc_set_comp(OrcSim *s, int nc, int nv, int nb, const double *x, const double *y,
+                 const double *z, const double *h, const int *np, const int *fneigh,
+                 const int *nneigh, const int *neigh24, int total_neigh) {
+  Comp &c = s->cp;
+  c.clear();
+  c.nc = nc; c.nv = nv; c.nb = nb;
+  const int S = nc + nv + nb;
+  c.x.assign(x, x + S); c.y.assign(y, y + S); c.z.assign(z, z + S); c.h.assign(h, h + S);
+  c.np.assign(np, np + S);
+  c.first.assign(S, 0);
+  c.type.resize(S);
+  for (int i = 0; i < S; ++i) c.type[i] = i < nc ? 0 : (i < nc + nv ? -1 : 1);
+  c.fneigh.assign(fneigh, fneigh + S);
+  c.nneigh.assign(nneigh, nneigh + S);
+  c.neigh24.assign(neigh24, neigh24 + total_neigh);
+  s->F = make_frame(s->p.bbox, s->p.bc);
+  s->have_comp = true;
+  return 0;
 }
 
 int orc_comp_sizes(OrcSim *s, int *nc, int *nv, int *nb, int *nneigh) {

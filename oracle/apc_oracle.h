@@ -67,6 +67,10 @@ int orc_comp_sizes(OrcSim *s, int *nc, int *nv, int *nb, int *nneigh);
 int orc_comp_get(OrcSim *s, double *x, double *y, double *z, double *h, int *np, int *first,
                  int *type, int *fneigh, int *nneigh, int *neigh24);
 
+/* A caller-filled computational-particle set (Types 0, -1, 1 by position). */
+int orc_set_comp(OrcSim *s, int nc, int nv, int nb, const double *x, const double *y,
+                 const double *z, const double *h, const int *np, const int *fneigh,
+                 const int *nneigh, const int *neigh24, int total_neigh);
 int orc_neighbors(OrcSim *s, int *nbr, double *dist);
 int orc_build_system(OrcSim *s, int *cols, double *a1, double *a2, double *d1, double *d2,
                      double *rhs1, double *pbnd);

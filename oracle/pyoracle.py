@@ -65,6 +65,8 @@ def lib():
         L.orc_comp_sizes.argtypes = [C.c_void_p, _I, _I, _I, _I]
         L.orc_comp_get.argtypes = [C.c_void_p] + [_D] * 4 + [_I] * 6
         L.orc_neighbors.argtypes = [C.c_void_p, _I, _D]
+        L.orc_set_comp.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [_D] * 4 + \
+            [_I] * 4 + [C.c_int]
         L.orc_build_system.argtypes = [C.c_void_p, _I] + [_D] * 6
         L.orc_set_constants.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
         L.orc_solve.argtypes = [C.c_void_p, C.POINTER(OrcSolveInfo)]
@@ -187,6 +189,31 @@ class Oracle:
                     np=i[0], first=i[1], type=i[2], fneigh=i[3], nneigh=i[4],
                     neigh24=n24[:nn.value])
 
+    def set_comp(self, nc, nv, nb, x, y, z, h, np_, neigh):
+        """A caller-filled computational-particle set; neigh: one Neigh24 list
+        per row (nc + nv rows)."""
+        S = nc + nv + nb
+        d = [np.ascontiguousarray(a, np.float64) for a in (x, y, z, h)]
+        npp = np.ascontiguousarray(np_, np.int32)
+        cnt = np.array([len(neigh[i]) if i < nc + nv else 0 for i in range(S)], np.int32)
+        first = np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(np.int32)
+        flat = np.array([m for i in range(nc + nv) for m in neigh[i]] or [0], np.int32)
+        assert self.L.orc_set_comp(self.h, nc, nv, nb, *[_p(a) for a in d], _p(npp, _I),
+                                   _p(first, _I), _p(cnt, _I), _p(flat, _I), int(cnt.sum())) == 0
+
+    SEARCH_ERRORS = {-2: "too many neighbours in an octant", -3: "not enough neighbours",
+                     -4: "invalid neighbour"}
+
+    def neighbors_status(self):
+        """(status, nbr, dist): status 0 or the search's error code."""
+        c = self.comp()
+        M = c["nc"] + c["nv"]
+        lm = self.p.lm
+        nbr = np.zeros(M * lm, np.int32)
+        dist = np.zeros(M * lm)
+        r = self.L.orc_neighbors(self.h, _p(nbr, _I), _p(dist))
+        return r, nbr.reshape(M, lm), dist.reshape(M, lm)
+
     def neighbors(self):
         c = self.comp()
         M = c["nc"] + c["nv"]
@@ -228,6 +255,14 @@ class Oracle:
         rho, phi, g, H = np.zeros(M), np.zeros(M), np.zeros(3 * M), np.zeros(6 * M)
         assert self.L.orc_comp_fields(self.h, _p(rho), _p(phi), _p(g), _p(H)) == 0
         return dict(rho=rho, phi=phi, g=g.reshape(M, 3), H=H.reshape(M, 6))
+
+    def differentiate(self, phi):
+        """Differentiate (solver.cpp:295-362) on a given potential."""
+        phi = np.ascontiguousarray(phi, np.float64)
+        assert self.L.orc_set_potential(self.h, _p(phi)) == 0
+        r = self.L.orc_differentiate(self.h)
+        if r:
+            raise RuntimeError("oracle differentiate failed: %d" % r)
 
     def set_gradients(self, g, H):
         g = np.ascontiguousarray(g, np.float64).ravel()
@@ -312,7 +347,26 @@ def ref():
         R.ref_move.argtypes = [C.c_void_p, _D, _D, C.c_double, C.c_double, C.c_double, _D, _D]
         R.ref_particles.argtypes = [C.c_void_p] + [_D] * 9
         R.ref_free.argtypes = [C.c_void_p]
+        R.ref_view.argtypes = [C.c_char_p, C.c_double, C.c_double, _D, C.c_int] + [_D] * 9 + \
+            [C.c_int] + [_D] * 3 + [_I] * 3 + [_D] * 5
     return _ref
+
+
+def ref_view(out_dir, t0, t1, bbox, pp, cp):
+    """The reference's Viewer (viewer.cpp) on the given arrays: pp = 9 arrays
+    (x, y, z, v, a), cp = dict(x, y, z, np, type, first, density, potential,
+    p_x, p_y, p_z)."""
+    R = ref()
+    pp = [np.ascontiguousarray(a, np.float64) for a in pp]
+    S = len(cp["x"])
+    d = {k: np.ascontiguousarray(cp[k], np.float64) for k in
+         ("x", "y", "z", "density", "potential", "p_x", "p_y", "p_z")}
+    i = {k: np.ascontiguousarray(cp[k], np.int32) for k in ("np", "type", "first")}
+    bb = np.ascontiguousarray(bbox, np.float64)
+    return R.ref_view(out_dir.encode(), t0, t1, _p(bb), len(pp[0]), *[_p(a) for a in pp], S,
+                      _p(d["x"]), _p(d["y"]), _p(d["z"]), _p(i["np"], _I), _p(i["type"], _I),
+                      _p(i["first"], _I), _p(d["density"]), _p(d["potential"]), _p(d["p_x"]),
+                      _p(d["p_y"]), _p(d["p_z"]))
 
 
 def ref_tree(x, y, z, bbox, depth, cap=2_000_000):

@@ -3,7 +3,8 @@
  * unmodified octree.cpp / mover.cpp / particle.cpp / parameter.cpp, compiled
  * from /root/reference by oracle/Makefile into oracle/_ref/libref.so.  Used
  * by tests/ to pin the oracle's restatement (keys, octree arrays, selection,
- * Neigh24, interpolation, kick, drift) on small inputs.
+ * Neigh24, interpolation, kick, drift) on small inputs, and the build's
+ * Viewer against the reference's viewer.cpp.
  *
  * The selection keeps its per-call arrays on the stack (octree.cpp:1112-1113,
  * ~104 B per temporary node), so every reference call runs on a worker thread
@@ -36,6 +37,7 @@
 #define private public
 #include "octree.h"
 #include "mover.h"
+#include "viewer.h"
 #undef private
 
 namespace {
@@ -192,6 +194,41 @@ int ref_particles(void *h, double *x, double *y, double *z, double *vx, double *
   memcpy(x, r->x.data(), b); memcpy(y, r->y.data(), b); memcpy(z, r->z.data(), b);
   memcpy(vx, r->vx.data(), b); memcpy(vy, r->vy.data(), b); memcpy(vz, r->vz.data(), b);
   memcpy(ax, r->ax.data(), b); memcpy(ay, r->ay.data(), b); memcpy(az, r->az.data(), b);
+  return 0;
+}
+
+/* The reference's Viewer (viewer.cpp:3-168) on the given particles: writes
+ * <dir>/physical_<t0>.vtk by View_Physical_Particles_Limit with the box,
+ * then <dir>/physical_<t1>.vtk and <dir>/computational_<t1>.vtk. */
+int ref_view(const char *dir, double t0, double t1, const double *bb, int n, const double *x,
+             const double *y, const double *z, const double *vx, const double *vy,
+             const double *vz, const double *ax, const double *ay, const double *az, int S,
+             const double *cx, const double *cy, const double *cz, const int *np,
+             const int *type, const int *first, const double *dens, const double *pot,
+             const double *px, const double *py, const double *pz) {
+  std::vector<double> a[9];
+  const double *src[9] = {x, y, z, vx, vy, vz, ax, ay, az};
+  for (int k = 0; k < 9; ++k) a[k].assign(src[k], src[k] + n);
+  Physical_particle pp;
+  memset(&pp, 0, sizeof(pp));
+  pp.size = n;
+  pp.x = a[0].data(); pp.y = a[1].data(); pp.z = a[2].data();
+  pp.v_x = a[3].data(); pp.v_y = a[4].data(); pp.v_z = a[5].data();
+  pp.a_x = a[6].data(); pp.a_y = a[7].data(); pp.a_z = a[8].data();
+  Computational_particle cp;
+  cp.size = S;
+  cp.nc = S; cp.nv = 0; cp.nb = 0;
+  cp.x.assign(cx, cx + S); cp.y.assign(cy, cy + S); cp.z.assign(cz, cz + S);
+  cp.NumberOfPhysicalParticles.assign(np, np + S);
+  cp.Type.assign(type, type + S);
+  cp.IndexOfFirstPhysicalParticle.assign(first, first + S);
+  cp.Density.assign(dens, dens + S); cp.Potential.assign(pot, pot + S);
+  cp.p_x.assign(px, px + S); cp.p_y.assign(py, py + S); cp.p_z.assign(pz, pz + S);
+  snprintf(OutputFile, sizeof(OutputFile), "%s", dir);
+  Viewer v(&pp, &cp);
+  v.View_Physical_Particles_Limit(t0, bb[0], bb[2], bb[4], bb[1], bb[3], bb[5]);
+  v.View_Physical_Particles(t1);
+  v.View_Computational_Particles(t1);
   return 0;
 }
 

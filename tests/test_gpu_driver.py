@@ -68,3 +68,81 @@ def test_driver_rejects_missing_tag(tmp_path):
     p = subprocess.run([EXE, path], capture_output=True, text=True, timeout=60)
     assert p.returncode != 0
     assert "I miss a value for tag 'OmegaLambda'" in p.stdout
+
+
+def test_driver_adaptive_time_step_on_a_lattice(tmp_path):
+    """The reference's default build has ADAPTIVE_TIME_STEP on
+    (apcloud_macro.h, controller.cpp:146-162).  On an 8^3 lattice at rest the
+    step is dt = 0.05 Delta_A / sqrt(512) every step (test_oracle_pins.py
+    derives it): 19 steps from a = 0.02 to 0.0254 with Delta_A = 0.01, the
+    lattice unmoved, as the oracle's controller."""
+    from test_oracle_pins import adaptive_lattice_case
+    x, y, z, n, da, start, end, dt, steps = adaptive_lattice_case()
+    zero = np.zeros(n)
+    ic.write_vtk(os.path.join(tmp_path, "ic.vtk"), x, y, z, zero, zero, zero)
+    final = os.path.join(tmp_path, "final.bin")
+    end_z = 1.0 / end - 1.0
+    param = _params(str(tmp_path), n, final, end_z,
+                    extra=dict(AdaptiveTimeStep=1, DeltaA=da, Redshift=1.0 / start - 1.0))
+    print("case: driver adaptive n=%d" % n, flush=True)
+    p = subprocess.run([EXE, param, "-ksp_rtol", "1e-10", "-ksp_gmres_restart", "60",
+                        "-ksp_max_it", "500"], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    assert p.stdout.count("Number of computational particles") == steps
+    got = np.fromfile(final, dtype=np.float64).reshape(9, n)
+    assert max(np.abs(got[k] - p0).max() for k, p0 in enumerate((x, y, z))) < 1e-12
+    o = po.Oracle(po.make_params(depth=20, bc=1, c=0.0, lm=27, leaf_capacity=8, units=2,
+                                 omega=1.0, mass_per_particle=1.0 / n, rtol=1e-10, restart=60,
+                                 max_it=500, adaptive_dt=1))
+    o.set_particles(x, y, z, zero, zero, zero)
+    _, done = o.run(1.0 / (1.0 + (1.0 / start - 1.0)), 1.0 / (1.0 + end_z), da)
+    assert done == steps
+
+
+def test_driver_writes_the_reference_snapshots(tmp_path):
+    """WriteInterval 2 over three steps: the limit snapshot at a = 0.02, then
+    both snapshots at steps 0 and 2 and at the end (controller.cpp:95-96,
+    121-130, 222-223); the final physical snapshot byte for byte equals the
+    reference's viewer.cpp on the final particle state."""
+    x, y, z, vx, vy, vz = ic.zeldovich(16, seed=3, rms=0.01)
+    n = len(x)
+    ic.write_vtk(os.path.join(tmp_path, "ic.vtk"), x, y, z, vx, vy, vz)
+    out = os.path.join(tmp_path, "out")
+    os.makedirs(out)
+    final = os.path.join(tmp_path, "final.bin")
+    end_z = 1.0 / 0.032 - 1.0
+    param = _params(str(tmp_path), n, final, end_z, extra=dict(WriteInterval=2, OutputFile=out))
+    print("case: driver snapshots n=%d" % n, flush=True)
+    p = subprocess.run([EXE, param, "-ksp_rtol", "1e-10", "-ksp_gmres_restart", "60",
+                        "-ksp_max_it", "3000"], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    want = ["physical_%f.vtk" % t for t in (0.02, 0.022, 0.030, 0.032)] + \
+        ["computational_%f.vtk" % t for t in (0.022, 0.030, 0.032)]
+    assert sorted(os.listdir(out)) == sorted(want)
+    for nm in want[1:4]:
+        lines = open(os.path.join(out, nm)).read().split("\n")
+        # positions, velocity vectors and three scalars, acceleration vectors
+        # and three scalars: 9 n data lines plus the block headers
+        assert lines[4] == "POINTS %d double" % n and 9 * n < len(lines) < 9 * n + 40
+    if not po.ref_available():
+        pytest.skip("oracle/_ref not built: the byte comparison needs the reference viewer")
+    got = np.fromfile(final, dtype=np.float64).reshape(9, n)
+    ref_dir = os.path.join(tmp_path, "ref")
+    os.makedirs(ref_dir)
+    one = dict(x=[0.0], y=[0.0], z=[0.0], np=[0], type=[0], first=[0], density=[0.0],
+               potential=[0.0], p_x=[0.0], p_y=[0.0], p_z=[0.0])
+    assert po.ref_view(ref_dir, 0.5, 0.032, (0, 1, 0, 1, 0, 1), list(got), one) == 0
+    a = open(os.path.join(out, "physical_0.032000.vtk"), "rb").read()
+    b = open(os.path.join(ref_dir, "physical_0.032000.vtk"), "rb").read()
+    assert a == b
+
+
+def test_driver_rejects_an_unsupported_preconditioner(tmp_path):
+    x, y, z, vx, vy, vz = ic.zeldovich(8, seed=3, rms=0.01)
+    n = len(x)
+    ic.write_vtk(os.path.join(tmp_path, "ic.vtk"), x, y, z, vx, vy, vz)
+    param = _params(str(tmp_path), n, os.path.join(tmp_path, "final.bin"), 1.0 / 0.024 - 1.0)
+    p = subprocess.run([EXE, param, "-pc_type", "ilu"], capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode != 0
+    assert "-pc_type ilu not supported" in p.stderr

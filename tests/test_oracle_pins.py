@@ -246,3 +246,35 @@ def test_eds_coefficients():
         np.testing.assert_allclose([cpot, cx, cv], [1.5 / a, np.sqrt(a), a ** -1.5], rtol=1e-15)
     p0 = po.make_params(units=0)
     assert po.coefficients(p0, 0.3) == [1.0, 1000.0, -1.0, 1.0, 1.0]
+
+
+def adaptive_lattice_case():
+    """An 8^3 lattice at rest, leaf capacity 8: every leaf holds 8 particles
+    in a cell of side 1/4, so max n_p/h^3 = 512 on every step (the lattice
+    does not move: the density is uniform, so the force is zero).  The
+    controller's step (controller.cpp:146-162) is then dt = 0.05 Delta_A /
+    512^(1/2) — the reference takes a square root, not a cube root — below
+    Delta_A.  From a = 0.02 (t starts at a + Delta_A / 2) to 0.0254 with
+    Delta_A = 0.01 that is 18 full steps and a shortened 19th."""
+    x, y, z = ic.lattice(8)
+    n = len(x)
+    da, start, end = 0.01, 0.02, 0.0254
+    dt = 0.05 * da / np.sqrt(512.0)
+    steps = int(np.ceil((end - (start + 0.5 * da)) / dt))
+    return x, y, z, n, da, start, end, dt, steps
+
+
+def test_adaptive_time_step_closed_form_on_a_lattice():
+    x, y, z, n, da, start, end, dt, steps = adaptive_lattice_case()
+    assert steps == 19
+    # a cube root would give 7 steps, no adaptive step 1: the count pins it
+    assert int(np.ceil((end - start - 0.5 * da) / (0.05 * da / 8.0))) == 7
+    zero = np.zeros(n)
+    o = po.Oracle(po.make_params(depth=20, bc=1, c=0.0, lm=27, leaf_capacity=8, units=2,
+                                 omega=1.0, mass_per_particle=1.0 / n, rtol=1e-10, restart=60,
+                                 max_it=500, adaptive_dt=1))
+    o.set_particles(x, y, z, zero, zero, zero)
+    _, done = o.run(start, end, da)
+    assert done == steps
+    P = o.particles()
+    assert max(np.abs(P[k] - p0).max() for k, p0 in enumerate((x, y, z))) < 1e-12
