@@ -36,7 +36,8 @@ EXPORTS = [
     "apc_adjoint_kick", "apc_adjoint_fields", "apc_gradient_adjoint", "apc_solve_adjoint",
     "apc_batch_create", "apc_batch_destroy", "apc_batch_size", "apc_batch_box",
     "apc_batch_last_error", "apc_batch_set_particles", "apc_batch_run",
-    "apc_morton_sort_noalloc", "apc_set_comp",
+    "apc_morton_sort_noalloc", "apc_set_comp", "apc_solver_set_fit_threshold",
+    "apc_fit_conditions",
 ]
 
 
@@ -60,7 +61,8 @@ class SolveInfo(C.Structure):
     _fields_ = [("its_rho", C.c_int), ("reason_rho", C.c_int), ("rnorm_rho", C.c_double),
                 ("its_phi", C.c_int), ("reason_phi", C.c_int), ("rnorm_phi", C.c_double),
                 ("refine_rho", C.c_int), ("refine_phi", C.c_int), ("nc", C.c_int),
-                ("nv", C.c_int), ("nb", C.c_int)]
+                ("nv", C.c_int), ("nb", C.c_int), ("fit_qr_rows", C.c_int),
+                ("fit_max_cond", C.c_double)]
 
 
 class CompView(C.Structure):
@@ -95,6 +97,8 @@ def lib():
         L.apc_get_comp_sizes.argtypes = [C.c_void_p, _I, _I, _I, _I]
         L.apc_get_comp.argtypes = [C.c_void_p, C.POINTER(CompView)]
         L.apc_neighbors.argtypes = [C.c_void_p, _I, _D]
+        L.apc_solver_set_fit_threshold.argtypes = [C.c_void_p, C.c_double]
+        L.apc_fit_conditions.argtypes = [C.c_void_p, _D, _I, _D]
         L.apc_set_comp.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int] + [_D] * 4 + \
             [_I] * 5 + [C.c_int]
         L.apc_solver_set_neighbors.argtypes = [C.c_void_p, C.c_int]
@@ -341,6 +345,19 @@ class Context:
         self._chk(self.L.apc_set_comp(self.h, nc, nv, nb, *[_p(a) for a in d], _p(npp, _I),
                                       _p(typ, _I), _p(first, _I), _p(cnt, _I), _p(flat, _I),
                                       int(cnt.sum())), "apc_set_comp")
+
+    def set_fit_threshold(self, cond_qr):
+        self._chk(self.L.apc_solver_set_fit_threshold(self.h, float(cond_qr)),
+                  "apc_solver_set_fit_threshold")
+
+    def fit_conditions(self):
+        """(per-row condition estimates, rows fitted by QR, largest estimate
+        of the others) of the last system build."""
+        cond = np.zeros(self.comp_rows())
+        q, mx = C.c_int(), C.c_double()
+        self._chk(self.L.apc_fit_conditions(self.h, _p(cond), C.byref(q), C.byref(mx)),
+                  "apc_fit_conditions")
+        return cond, q.value, mx.value
 
     def neighbors(self):
         nc, nv, nb, nn = C.c_int(), C.c_int(), C.c_int(), C.c_int()

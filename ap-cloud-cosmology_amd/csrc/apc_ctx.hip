@@ -176,7 +176,8 @@ int apc_destroy(apc_ctx *c) {
   for (auto &b : c->ckpt) b.release(c->s);
   for (auto &b : c->ckpt_id) b.release(c->s);
   c->key.release(c->s); c->key2.release(c->s); c->nprefix.release(c->s); c->red_u64.release(c->s);
-  c->visits.release(c->s); c->scodes.release(c->s); c->slen.release(c->s); c->sidx.release(c->s);
+  c->visits.release(c->s); c->fitq_rows.release(c->s); c->fitq_stat.release(c->s);
+  c->fit_cond.release(c->s); c->scodes.release(c->s); c->slen.release(c->s); c->sidx.release(c->s);
   c->tr_key.release(c->s); c->tr_key2.release(c->s); c->cic_key.release(c->s); c->cic_key2.release(c->s);
   c->vcode.release(c->s); c->vcode2.release(c->s);
   hipStreamDestroy(c->s);
@@ -260,7 +261,13 @@ int apc_solver_solve(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_opts *o2,
   if (r) return r;
   r = solve_density_potential(c, o1, o2, info);
   if (r) return r;
-  return differentiate(c);
+  r = differentiate(c);
+  if (r) return r;
+  if (info) {
+    info->fit_qr_rows = c->stencil_qr_rows;
+    info->fit_max_cond = c->stencil_max_cond;
+  }
+  return 0;
 }
 
 int apc_solver_amg_levels(apc_ctx *c, int *rows, long long *nnz, int cap, int *nlev) {

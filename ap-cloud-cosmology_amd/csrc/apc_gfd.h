@@ -70,6 +70,36 @@ __device__ inline bool chol_inplace(double *R) {
   return deficient;
 }
 
+// ||R||_F ||R^-1||_F for the packed upper triangular R (R^T R = B^T B): an
+// upper bound on cond_2(R) = cond_2(B) that exceeds it by at most a factor
+// of 9.  Column c of R^-1 is R^-1 e_c by back substitution (zero below c).
+__device__ inline double cond_frob(const double *R) {
+  double nr = 0.0;
+#pragma unroll
+  for (int q = 0; q < 45; ++q) nr += R[q] * R[q];
+  double id[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) id[k] = 1.0 / R[RI(k, k)];
+  double ni = 0.0;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    double v[9];
+#pragma unroll
+    for (int k = 8; k >= 0; --k) {
+      if (k > c) {
+        v[k] = 0.0;
+        continue;
+      }
+      double t = (k == c) ? 1.0 : 0.0;
+#pragma unroll
+      for (int q = k + 1; q <= c; ++q) t -= R[RI(k, q)] * v[q];
+      v[k] = t * id[k];
+      ni += v[k] * v[k];
+    }
+  }
+  return sqrt(nr * ni);
+}
+
 // z = R^-1 R^-T u (the semi-normal solve (B^T B) z = u)
 __device__ inline void seminormal_solve(const double *R, const double *u, double *z) {
   double y[9];

@@ -103,6 +103,14 @@ struct apc_ctx {
   apc::DBuf<double> sco;   // lm x M Laplacian weights s_ij
   apc::DBuf<double> d1, d2, f1, f2, dinv1, dinv2, rhs1, pbnd, bv, rho, phi, rhs2;
   apc::DBuf<double> grad;  // 9 x M: gx gy gz hxx hyy hzz hxy hxz hyz
+  // least-squares conditioning: rows whose estimate exceeds fit_cond_qr are
+  // fitted by Householder QR
+  double fit_cond_qr = 1e5;
+  apc::DBuf<int> fitq_rows;
+  apc::DBuf<unsigned long long> fitq_stat;  // count, largest estimate (bits)
+  apc::DBuf<double> fit_cond;               // per row: the last fit's estimate
+  int fit_qr_rows = 0, stencil_qr_rows = 0;
+  double fit_max_cond = 0.0, stencil_max_cond = 0.0;
   apc::DBuf<double> nscratch;  // neighbour search scratch
   apc::DBuf<int> nscratch_i;
   apc::DBuf<int> nscratch_f;   // candidate fingerprints

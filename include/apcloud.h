@@ -71,6 +71,9 @@ typedef struct {
   int its_phi, reason_phi; double rnorm_phi;
   int refine_rho, refine_phi;   /* outer refinement sweeps (precision form) */
   int nc, nv, nb;               /* sizes of the system that was solved */
+  int fit_qr_rows;              /* least-squares rows fitted by Householder QR (condition
+                                   estimate above the threshold, see apc_solver_set_fit_threshold) */
+  double fit_max_cond;          /* largest condition estimate among the Cholesky-fitted rows */
 } apc_solve_info;
 
 /* Host view of the computational particles (particle.h:23-53), filled by
@@ -154,6 +157,18 @@ int apc_build_system(apc_ctx *ctx, int *cols, double *a1, double *a2, double *d1
  * Solve_Potential, Differentiate. */
 int apc_solver_solve(apc_ctx *ctx, const apc_ksp_opts *rho, const apc_ksp_opts *phi,
                      apc_solve_info *info);
+/* The least-squares fits of Build_Linear_System and Differentiate
+ * (LAPACKE_dgels, solver.cpp:106,342).  Each row is fitted through the
+ * Cholesky factor R of its Gram matrix (BᵀB = RᵀR, B the weighted basis) when
+ * its condition estimate ||R||_F ||R^-1||_F (at most 9 cond_2(B)) is at most
+ * cond_qr, and by Householder QR of B in FP64 (dgels' method: error growing
+ * with cond(B), not its square) otherwise or when a Cholesky pivot fails.
+ * Default cond_qr = 1e5; 0 sends every row to QR. */
+int apc_solver_set_fit_threshold(apc_ctx *ctx, double cond_qr);
+/* After a system build: per row the condition estimate (M values, infinity
+ * where the Cholesky factorisation failed), the number of rows fitted by QR
+ * and the largest estimate among the others.  Any pointer may be NULL. */
+int apc_fit_conditions(apc_ctx *ctx, double *cond, int *qr_rows, double *max_cond);
 /* Multigrid hierarchy of the last solve run with pc_type 1: *nlev levels, the
  * first min(cap, *nlev) row counts and stored nonzeros (level 0 = the GFD
  * matrix).  *nlev = 0 if no multigrid was built.  No reference counterpart

@@ -24,6 +24,7 @@ import numpy as np
 import pytest
 
 import apcloud
+import pyoracle as po
 from apcloud import ic
 
 pytestmark = pytest.mark.gpu
@@ -51,6 +52,7 @@ def full():
     g.select()
     g.set_constants(1.0 / len(x), 1.5 / 0.02, -1.0)
     sysm = g.build_system()
+    cond, qr_rows, max_cond = g.fit_conditions()
     nbr, dist = g.neighbors()
     comp = g.comp()
     M = comp["nc"] + comp["nv"]
@@ -60,7 +62,11 @@ def full():
     fields = g.fields()
     print("its", info.its_rho, info.its_phi, flush=True)
     rows = np.random.default_rng(0).choice(M, size=min(SAMPLE, M), replace=False)
+    worst = np.argsort(np.where(np.isfinite(cond), cond, np.inf))[-1000:]
+    keep = np.unique(np.concatenate([rows[:2000], worst]))
     out = dict(M=M, rows=rows, nbr=nbr[rows], dist=dist[rows], sys=sysm, info=info,
+               cond=cond, qr_rows=qr_rows, max_cond=max_cond, keep=keep,
+               k_cols=sysm["cols"][keep], k_a1=sysm["a1"][keep], k_dist=dist[keep],
                rho=fields["rho"], phi=fields["phi"], x=comp["x"], y=comp["y"], z=comp["z"],
                h=comp["h"])
     out["cols"], out["a1"], out["a2"] = sysm["cols"][rows], sysm["a1"][rows], sysm["a2"][rows]
@@ -133,3 +139,30 @@ def test_full_size_solves_satisfy_their_systems(full):
     r2 = rhs2 - _apply(S["d2"], S["a2"], cols, full["phi"])
     rel2 = np.linalg.norm(r2) / np.linalg.norm(rhs2)
     assert rel2 < 2 * RTOL, rel2
+
+
+def test_full_size_fit_conditioning(full):
+    """The condition estimates at 256^3: how many rows the QR fallback takes
+    (printed), and the sampled rows plus the 1,000 worst-conditioned rows
+    against the oracle's Householder least squares (the dgels replacement)
+    on the same neighbours, to 1e-9 of each row's scale."""
+    cond, nq, mx = full["cond"], full["qr_rows"], full["max_cond"]
+    finite = cond[np.isfinite(cond)]
+    print("fit conditioning: %d rows by QR; estimates median %.3e, max %.3e (Cholesky rows: %.3e)"
+          % (nq, np.median(finite), finite.max(), mx), flush=True)
+    assert nq == int((~(cond <= 1e5)).sum())
+    x, y, z, h = full["x"], full["y"], full["z"], full["h"]
+    e4 = np.exp(-4.0)
+    bad = 0.0
+    for k, i in enumerate(full["keep"]):
+        cols, s, d = full["k_cols"][k], 24.0 * full["k_a1"][k], full["k_dist"][k]
+        md = 0.7 * d.max()
+        w = (np.exp(-d * d / md / md) - e4) / (1 - e4)
+        p = [_wrap(c[cols] - c[i]) / h[i] for c in (x, y, z)]
+        B = np.stack([p[0], p[1], p[2], p[0] ** 2 / 2, p[1] ** 2 / 2, p[2] ** 2 / 2, p[0] * p[1],
+                      p[0] * p[2], p[1] * p[2]], axis=1) * w[:, None]
+        info, X = po.wlsq(B, np.diag(w))
+        assert info == 0
+        ref = X[3] + X[4] + X[5]
+        bad = max(bad, np.abs(s - ref).max() / np.abs(ref).max())
+    assert bad < 1e-9, bad
