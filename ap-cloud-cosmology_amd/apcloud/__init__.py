@@ -37,7 +37,7 @@ EXPORTS = [
     "apc_batch_create", "apc_batch_destroy", "apc_batch_size", "apc_batch_box",
     "apc_batch_last_error", "apc_batch_set_particles", "apc_batch_run",
     "apc_morton_sort_noalloc", "apc_set_comp", "apc_solver_set_fit_threshold",
-    "apc_fit_conditions",
+    "apc_fit_conditions", "apc_create_packed", "apc_pack_set_masses", "apc_pack_rows",
 ]
 
 
@@ -88,6 +88,9 @@ def lib():
         L.apc_device_count.argtypes = [_I]
         L.apc_create.argtypes = [C.POINTER(Params), C.POINTER(C.c_void_p)]
         L.apc_destroy.argtypes = [C.c_void_p]
+        L.apc_create_packed.argtypes = [C.POINTER(Params), C.c_int, _I, C.POINTER(C.c_void_p)]
+        L.apc_pack_set_masses.argtypes = [C.c_void_p, _D]
+        L.apc_pack_rows.argtypes = [C.c_void_p, _I, _I]
         L.apc_set_particles.argtypes = [C.c_void_p, C.c_int] + [_D] * 6
         L.apc_get_particles.argtypes = [C.c_void_p] + [_D] * 9
         L.apc_octree_build.argtypes = [C.c_void_p]
@@ -560,3 +563,35 @@ class Context:
 
     def sync(self):
         self._chk(self.L.apc_sync(self.h), "apc_sync")
+
+
+class PackedContext(Context):
+    """Independent periodic boxes packed into one context
+    (apc_create_packed): counts[b] particles of box b, given to
+    set_particles box after box; every kernel of a step runs over all boxes
+    in one launch."""
+
+    def __init__(self, params, counts):
+        counts = np.ascontiguousarray(counts, np.int32)
+        L = lib()
+        h = C.c_void_p()
+        r = L.apc_create_packed(C.byref(params), len(counts), _p(counts, _I), C.byref(h))
+        if r != 0:
+            raise ApcError("apc_create_packed failed: %s" % ERRORS.get(r, r))
+        super().__init__(params, _handle=h)
+        self.owned = True
+        self.counts = counts
+        self.pstart = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+
+    def set_masses(self, mass):
+        m = np.ascontiguousarray(mass, np.float64)
+        assert m.shape == (len(self.counts),)
+        self._chk(self.L.apc_pack_set_masses(self.h, _p(m)), "apc_pack_set_masses")
+
+    def rows(self):
+        """(row_start[nbox + 1], comp_rows[nbox]) of the current selection."""
+        nbox = len(self.counts)
+        rs = np.zeros(nbox + 1, np.int32)
+        nc = np.zeros(nbox, np.int32)
+        self._chk(self.L.apc_pack_rows(self.h, _p(rs, _I), _p(nc, _I)), "apc_pack_rows")
+        return rs, nc

@@ -347,6 +347,8 @@ extern "C" int apc_freeze_topology(apc_ctx *c, int on) {
   if (!c) return APC_ERR_ARG;
   if (on && (!c->have_comp || !c->sorted || c->comp_external))
     return fail(c, APC_ERR_ARG, "freeze_topology: build the octree and select first");
+  if (on && c->nbox > 1)
+    return fail(c, APC_ERR_UNSUPPORTED, "freeze_topology: not on a packed multi-box context");
   c->frozen = on ? 1 : 0;
   if (!on) c->have_tr = 0;
   return 0;

@@ -47,9 +47,11 @@ struct AmgLevel {
   DBuf<float> sval32;
   DBuf<int> agg;      // coarse row of each row (all but the coarsest level)
   DBuf<int> mp, ml;   // rows of each coarse row, grouped (CSR over next level)
+  DBuf<int> box;      // packed boxes: the box of each row (levels below the finest)
   DBuf<double> x, b, r, t;
   DBuf<float> x32, b32, r32, t32;
   void release() {
+    box.release();
     soff.release(); swid.release(); scol.release(); sval.release(); sval32.release();
     rp.release(); col.release(); val.release(); dinv.release(); val32.release();
     dinv32.release(); agg.release(); mp.release(); ml.release(); x.release(); b.release();
@@ -83,7 +85,7 @@ struct AmgHier {
   int nlev = 0;
   int nc = 0;
   bool dense = false;
-  DBuf<double> gj, cinv, colk;
+  DBuf<double> gj, cinv, colk, boxw;
   DBuf<float> cinv32;
   AmgOptions o;
   bool fp32 = false;
@@ -96,7 +98,7 @@ struct AmgHier {
   ~AmgHier() {
     for (auto &g : graphs) g.release(s);
     for (auto *l : L) { l->release(); delete l; }
-    gj.release(); cinv.release(); colk.release(); cinv32.release(); key.release();
+    gj.release(); cinv.release(); colk.release(); boxw.release(); cinv32.release(); key.release();
     key2.release(); idx.release(); idx2.release(); sa.release(); sb.release(); flag.release();
     pos.release(); cnt.release(); ev.release(); smax.release(); diag.release();
     gcol.release(); gval.release();
@@ -660,12 +662,20 @@ __global__ void k_amg_convert(size_t n, const S *__restrict__ a, D *__restrict__
     b[i] = (D)a[i];
 }
 
-// [A | I] for Gauss-Jordan, A from CSR plus shift/n everywhere.
-__global__ void k_gj_fill(Csr<double> A, double shift, double *__restrict__ g) {
+// [A | I] for Gauss-Jordan, A from CSR plus shift/n everywhere; with packed
+// boxes (box non-null) shift/n_b in the block of each box b instead (the
+// constant mode of every box), bw[i] = 1/n_b of row i's box.
+__global__ void k_gj_fill(Csr<double> A, double shift, const int *__restrict__ box,
+                          const double *__restrict__ bw, double *__restrict__ g) {
   const int n = A.n;
   int i = blockIdx.x;
-  for (int j = threadIdx.x; j < 2 * n; j += blockDim.x)
-    g[(size_t)i * 2 * n + j] = (j < n) ? shift / n : (j - n == i ? 1.0 : 0.0);
+  for (int j = threadIdx.x; j < 2 * n; j += blockDim.x) {
+    double e;
+    if (j >= n) e = (j - n == i) ? 1.0 : 0.0;
+    else if (box) e = (box[i] == box[j]) ? shift * bw[i] : 0.0;
+    else e = shift / n;
+    g[(size_t)i * 2 * n + j] = e;
+  }
   __syncthreads();
   if (threadIdx.x == 0)
     for (int e = A.rp[i]; e < A.rp[i + 1]; ++e) g[(size_t)i * 2 * n + A.col[e]] += A.val[e];
@@ -1176,10 +1186,35 @@ static int build_sell(apc_ctx *c, AmgHier *h, AmgLevel *L) {
   return 0;
 }
 
-static int dense_inverse(apc_ctx *c, AmgHier *h, AmgLevel *L, bool periodic) {
+// Box of every coarse row: the box of any of its rows (aggregates never
+// join rows of two boxes: the boxes' blocks share no entry).
+__global__ void k_box_down(int n, const int *__restrict__ agg, const int *__restrict__ bf,
+                           int *__restrict__ bc) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) bc[agg[i]] = bf[i];
+}
+
+static int dense_inverse(apc_ctx *c, AmgHier *h, AmgLevel *L, bool periodic, const int *box) {
   const int n = L->n;
   Csr<double> A{L->rp.p, L->col.p, L->val.p, n};
   double shift = 0.0;
+  if (periodic && box) {
+    std::vector<int> hb(n);
+    APC_HIP(c, hipMemcpyAsync(hb.data(), box, sizeof(int) * n, hipMemcpyDeviceToHost, c->s));
+    APC_HIP(c, hipStreamSynchronize(c->s));
+    std::vector<int> cnt(c->nbox, 0);
+    for (int b : hb) {
+      if (b < 0 || b >= c->nbox) return fail(c, APC_ERR_ARG, "amg: coarse row of box %d", b);
+      ++cnt[b];
+    }
+    std::vector<double> w(n);
+    for (int i = 0; i < n; ++i) w[i] = 1.0 / cnt[hb[i]];
+    APC_ALLOC(c, h->boxw, n + 1);
+    APC_HIP(c, hipMemcpyAsync(h->boxw.p, w.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->s));
+    APC_HIP(c, hipStreamSynchronize(c->s));
+  } else {
+    box = nullptr;
+  }
   if (periodic) {  // mean |diagonal|, from the host copy of the level's dinv
     std::vector<double> dv(n);
     APC_HIP(c, hipMemcpyAsync(dv.data(), L->dinv.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
@@ -1195,7 +1230,8 @@ static int dense_inverse(apc_ctx *c, AmgHier *h, AmgLevel *L, bool periodic) {
   APC_ALLOC(c, h->cinv, (size_t)n * n + 1);
   APC_ALLOC(c, h->cnt, 4);
   APC_HIP(c, hipMemsetAsync(h->cnt.p, 0, sizeof(int), c->s));
-  hipLaunchKernelGGL(k_gj_fill, dim3(n), dim3(256), 0, c->s, A, shift, h->gj.p);
+  hipLaunchKernelGGL(k_gj_fill, dim3(n), dim3(256), 0, c->s, A, shift, box,
+                     (const double *)h->boxw.p, h->gj.p);
   const unsigned ge = std::max(1u, std::min(1024u, nblocks((long)2 * n * n, 256)));
   for (int k = 0; k < n; ++k) {
     hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, c->s, n, k, h->gj.p, h->colk.p, h->cnt.p);
@@ -1231,6 +1267,7 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
     return h->L[l];
   };
   KScope ks(c, "amg_setup");
+  const int *box0 = (periodic && c->nbox > 1) ? (const int *)c->rowbox.p : nullptr;
   AmgLevel *L0 = level(0);
   L0->n = A.M;
   L0->nnz = A.M * (A.lm + 1);
@@ -1254,6 +1291,11 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
     }
     if (r) return r;
     if (nc < 1 || nc >= n) break;  // no coarsening: this level is the coarsest
+    if (box0) {
+      APC_ALLOC(c, level(l + 1)->box, nc + 1);
+      hipLaunchKernelGGL(k_box_down, dim3(nblocks(n, 256)), dim3(256), 0, c->s, n, Lf->agg.p,
+                         l == 0 ? box0 : (const int *)Lf->box.p, level(l + 1)->box.p);
+    }
     {
       KScope ks1(c, "amg_setup_group");
       r = group_members(c, h, Lf, n, nc);
@@ -1294,7 +1336,7 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
   h->nc = Lc->n;
   if (k > 0 && Lc->n <= 1024) {  // else (coarsening stalled) smoothing only
     KScope ks1(c, "amg_setup_dense");
-    r = dense_inverse(c, h, Lc, periodic);
+    r = dense_inverse(c, h, Lc, periodic, box0 ? (const int *)Lc->box.p : nullptr);
     if (r) return r;
     h->dense = true;
   }

@@ -1,6 +1,8 @@
 // Context lifetime, particle transfer, solver entry points, per-kernel
 // event timers and the one-call step.
+#include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstdarg>
 
 #include "apc_internal.h"
@@ -141,12 +143,76 @@ int apc_create(const apc_params *p, apc_ctx **out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return APC_ERR_NO_DEVICE;
   apc_ctx *c = new apc_ctx();
   c->prm = *p;
+  c->depth_req = p->depth;
+  c->maxdepth_req = p->maxdepth;
   hipGetDevice(&c->dev);
   if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return APC_ERR_HIP;
   }
   *out = c;
+  return APC_OK;
+}
+
+// nbox boxes of the same parameters in one context: the box id sits above
+// the 3*depth Morton bits of the sort key, so the key budget of 64 bits may
+// lower the depth (48 boxes: 6 bits, depth at most 19); the selection
+// refuses a tree that the lowered depth would cut.
+int apc_create_packed(const apc_params *p, int nbox, const int *counts, apc_ctx **out) {
+  if (!p || !out || !counts || nbox < 1 || nbox > 64) return APC_ERR_ARG;
+  *out = nullptr;
+  if (p->bc != 1) return APC_ERR_UNSUPPORTED;  // periodic boxes only
+  long total = 0;
+  for (int b = 0; b < nbox; ++b) {
+    if (counts[b] < 1) return APC_ERR_ARG;
+    total += counts[b];
+  }
+  if (total > INT_MAX) return APC_ERR_ARG;
+  int bits = 0;
+  while ((1 << bits) < nbox) ++bits;
+  apc_params q = *p;
+  const int dmax = (64 - bits) / 3;
+  q.depth = std::min(p->depth, dmax);
+  q.maxdepth = std::min(p->maxdepth, q.depth);
+  int r = apc_create(&q, out);
+  if (r) return r;
+  apc_ctx *c = *out;
+  c->nbox = nbox;
+  c->boxbits = bits;
+  c->depth_req = p->depth;
+  c->maxdepth_req = p->maxdepth;
+  c->pstart_h.assign(nbox + 1, 0);
+  for (int b = 0; b < nbox; ++b) c->pstart_h[b + 1] = c->pstart_h[b] + counts[b];
+  std::vector<double> ones(nbox, 1.0);
+  if (c->pstart.ensure(nbox + 1, c->s) != hipSuccess || c->boxmass.ensure(nbox, c->s) != hipSuccess ||
+      hipMemcpy(c->pstart.p, c->pstart_h.data(), sizeof(int) * (nbox + 1), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(c->boxmass.p, ones.data(), sizeof(double) * nbox, hipMemcpyHostToDevice) != hipSuccess) {
+    apc_destroy(c);
+    *out = nullptr;
+    return APC_ERR_NOMEM;
+  }
+  return APC_OK;
+}
+
+int apc_pack_set_masses(apc_ctx *c, const double *m) {
+  if (!c || !m) return APC_ERR_ARG;
+  if (c->nbox < 2) return fail(c, APC_ERR_ARG, "pack_set_masses: not a packed context");
+  for (int b = 0; b < c->nbox; ++b)
+    if (!(m[b] > 0.0)) return fail(c, APC_ERR_ARG, "pack_set_masses: box %d mass %g", b, m[b]);
+  APC_HIP(c, hipMemcpyAsync(c->boxmass.p, m, sizeof(double) * c->nbox, hipMemcpyHostToDevice, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  c->have_sys = 0;  // rhs1 carries the masses
+  return APC_OK;
+}
+
+int apc_pack_rows(apc_ctx *c, int *row_start, int *comp_rows) {
+  if (!c) return APC_ERR_ARG;
+  if (!c->have_comp) return fail(c, APC_ERR_ARG, "pack_rows: no computational particles");
+  for (int b = 0; b <= c->nbox; ++b) {
+    if (row_start) row_start[b] = c->rstart_h[b];
+    if (comp_rows && b < c->nbox) comp_rows[b] = c->rnc_h[b];
+  }
   return APC_OK;
 }
 
@@ -176,7 +242,11 @@ int apc_destroy(apc_ctx *c) {
   for (auto &b : c->ckpt) b.release(c->s);
   for (auto &b : c->ckpt_id) b.release(c->s);
   c->key.release(c->s); c->key2.release(c->s); c->nprefix.release(c->s); c->red_u64.release(c->s);
-  c->visits.release(c->s); c->fitq_rows.release(c->s); c->fitq_stat.release(c->s);
+  c->visits.release(c->s); c->pstart.release(c->s); c->rstart.release(c->s);
+  c->rowbox.release(c->s); c->boxmass.release(c->s); c->pk_key.release(c->s);
+  c->pk_key2.release(c->s); c->pk_val.release(c->s); c->pk_val2.release(c->s);
+  c->pk_leaf.release(c->s); c->pk_type.release(c->s); c->pk_b.release(c->s);
+  c->pk_sig.release(c->s); c->fitq_rows.release(c->s); c->fitq_stat.release(c->s);
   c->fit_cond.release(c->s); c->scodes.release(c->s); c->slen.release(c->s); c->sidx.release(c->s);
   c->tr_key.release(c->s); c->tr_key2.release(c->s); c->cic_key.release(c->s); c->cic_key2.release(c->s);
   c->vcode.release(c->s); c->vcode2.release(c->s);
@@ -188,6 +258,9 @@ int apc_destroy(apc_ctx *c) {
 int apc_set_particles(apc_ctx *c, int n, const double *x, const double *y, const double *z,
                       const double *vx, const double *vy, const double *vz) {
   if (!c || n < 1 || !x || !y || !z) return fail(c, APC_ERR_ARG, "set_particles: bad arguments");
+  if (c->nbox > 1 && n != c->pstart_h[c->nbox])
+    return fail(c, APC_ERR_ARG, "set_particles: %d particles, the boxes hold %d", n,
+                c->pstart_h[c->nbox]);
   c->frozen = 0;  // new particles: a new topology
   c->have_tr = 0;
   c->have_cic_w = 0;
@@ -347,7 +420,7 @@ int apc_step(apc_ctx *c, double c_dens, double c_pot, double c_field, double Cv,
     r = apc_octree_build(c);
     if (r) return r;
     int S = 0;
-    r = apc_select(c, c->prm.c, c->prm.order, c->prm.mindepth, c->prm.maxdepth, &S);
+    r = apc_select(c, c->prm.c, c->prm.order, c->prm.mindepth, c->maxdepth_req, &S);
     if (r) return r;
   }
   APC_HIP(c, hipStreamSynchronize(c->s));

@@ -198,6 +198,28 @@ struct NoExtra {
   __device__ double operator()(int) const { return 0.0; }
 };
 
+// Packed boxes: the box of every row, the first row of every box and the
+// boxes' mass factors; all null for a single-box context.  The periodic
+// potential system pins the first row of every box (row 0 alone without
+// boxes), and the density right-hand side carries the box's mass factor.
+struct Boxes {
+  const int *rowbox, *rstart;
+  const double *mass;
+};
+
+__device__ inline bool pin_row(const Boxes &B, int pin, int i) {
+  return pin && (B.rowbox ? B.rstart[B.rowbox[i]] == i : i == 0);
+}
+
+__device__ inline double box_mass(const Boxes &B, int i) {
+  return B.rowbox ? B.mass[B.rowbox[i]] : 1.0;
+}
+
+inline Boxes boxes_of(apc_ctx *c) {
+  if (c->nbox > 1) return Boxes{c->rowbox.p, c->rstart.p, c->boxmass.p};
+  return Boxes{nullptr, nullptr, nullptr};
+}
+
 inline Geo geo_of(apc_ctx *c) {
   Geo G;
   const double *b = c->prm.bbox;

@@ -444,6 +444,51 @@ __global__ void __launch_bounds__(256) k_rhs2_periodic(int M, const double *__re
     rhs2[i] = (rho[i] - avg) * c_pot;
 }
 
+// Packed boxes (rows box-major, rstart[b] the first row of box b): the
+// periodic right-hand side of every box with its own mean, one block per box.
+__global__ void __launch_bounds__(256) k_rhs2_boxes(const int *__restrict__ rstart,
+                                                    const double *__restrict__ rho,
+                                                    const double *__restrict__ h, double c_pot,
+                                                    double *__restrict__ rhs2) {
+  __shared__ double ws[4];
+  const int lo = rstart[blockIdx.x], hi = rstart[blockIdx.x + 1];
+  double a = 0.0, b = 0.0;
+  for (int i = lo + threadIdx.x; i < hi; i += 256) {
+    const double h3 = h[i] * h[i] * h[i];
+    a += rho[i] * h3;
+    b += h3;
+  }
+  a = block_sum256(a, ws);
+  b = block_sum256(b, ws);
+  const double avg = a / b;
+  for (int i = lo + threadIdx.x; i < hi; i += 256) rhs2[i] = (rho[i] - avg) * c_pot;
+}
+
+// sig[b] = || w b ||_2 over box b's rows (w = 1 if null); 1 for a zero box.
+__global__ void __launch_bounds__(256) k_box_norms(const int *__restrict__ rstart,
+                                                   const double *__restrict__ v,
+                                                   const double *__restrict__ w,
+                                                   double *__restrict__ sig) {
+  __shared__ double ws[4];
+  const int lo = rstart[blockIdx.x], hi = rstart[blockIdx.x + 1];
+  double a = 0.0;
+  for (int i = lo + threadIdx.x; i < hi; i += 256) {
+    const double e = w ? w[i] * v[i] : v[i];
+    a += e * e;
+  }
+  a = block_sum256(a, ws);
+  if (threadIdx.x == 0) sig[blockIdx.x] = (a > 0.0) ? sqrt(a) : 1.0;
+}
+
+// out = in / sig[box] (inverse = 0) or in * sig[box] (inverse = 1)
+__global__ void k_box_scale(int M, const int *__restrict__ rowbox, const double *__restrict__ sig,
+                            const double *__restrict__ in, double *__restrict__ out, int inverse) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= M) return;
+  const double q = sig[rowbox[i]];
+  out[i] = inverse ? in[i] * q : in[i] / q;
+}
+
 __global__ void k_rhs2_bounded(int M, const double *__restrict__ rho, const double *__restrict__ pbnd,
                                double c_pot, double *__restrict__ rhs2) {
   int i = blockIdx.x * 256 + threadIdx.x;
@@ -761,6 +806,35 @@ static int gmres(apc_ctx *c, const Op<double> &A, const double *b, double *x,
                            reason, rnorm, sweeps);
 }
 
+// gmres over the rows of packed boxes.  The boxes' blocks are decoupled, so
+// scaling box b's right-hand side by 1/sig_b (zero initial guess) scales its
+// solution by the same factor.  With sig_b the norm of what the convergence
+// test measures (the left-Jacobi preconditioned right-hand side, or the
+// right-hand side itself under the right-preconditioned multigrid), every
+// box enters with norm 1, ||b|| = sqrt(nbox), and the global test ||r|| <=
+// (rtol / sqrt(nbox)) ||b|| bounds every box's own relative residual
+// ||r_b|| / ||b_b|| by rtol.  The solution is scaled back at the end.
+static int gmres_boxes(apc_ctx *c, const Op<double> &A, const double *b, double *x,
+                       const apc_ksp_opts &o, bool pinned, int *its, int *reason, double *rnorm,
+                       int *sweeps) {
+  if (c->nbox < 2) return gmres(c, A, b, x, o, pinned, its, reason, rnorm, sweeps);
+  const int M = A.M, nbox = c->nbox;
+  APC_ALLOC(c, c->pk_b, M + 1);
+  APC_ALLOC(c, c->pk_sig, nbox + 1);
+  hipLaunchKernelGGL(k_box_norms, dim3(nbox), dim3(256), 0, c->s, c->rstart.p, b,
+                     o.pc_type == 1 ? (const double *)nullptr : A.dinv, c->pk_sig.p);
+  hipLaunchKernelGGL(k_box_scale, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, c->rowbox.p,
+                     c->pk_sig.p, b, c->pk_b.p, 0);
+  apc_ksp_opts ob = o;
+  ob.rtol = o.rtol / sqrt((double)nbox);
+  int r = gmres(c, A, c->pk_b.p, x, ob, pinned, its, reason, rnorm, sweeps);
+  if (r) return r;
+  hipLaunchKernelGGL(k_box_scale, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, c->rowbox.p,
+                     c->pk_sig.p, x, x, 1);
+  APC_HIP(c, hipGetLastError());
+  return 0;
+}
+
 int solve_density_potential(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_opts *o2,
                             apc_solve_info *info) {
   const int M = c->nc + c->nv;
@@ -776,11 +850,14 @@ int solve_density_potential(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_op
   Op<double> A2{c->nbr.p, c->sco.p, c->d2.p, c->f2.p, c->dinv2.p, M, c->prm.lm};
   apc_solve_info inf;
   memset(&inf, 0, sizeof(inf));
-  int r = gmres(c, A1, c->rhs1.p, c->rho.p, *o1, false, &inf.its_rho, &inf.reason_rho,
-                &inf.rnorm_rho, &inf.refine_rho);
+  int r = gmres_boxes(c, A1, c->rhs1.p, c->rho.p, *o1, false, &inf.its_rho, &inf.reason_rho,
+                      &inf.rnorm_rho, &inf.refine_rho);
   if (r) return r;
   // right-hand side of the Poisson solve (solver.cpp:233-261)
-  if (c->prm.bc == 1) {
+  if (c->prm.bc == 1 && c->nbox > 1) {
+    hipLaunchKernelGGL(k_rhs2_boxes, dim3(c->nbox), dim3(256), 0, c->s, c->rstart.p, c->rho.p,
+                       c->ch.p, c->c_pot, c->rhs2.p);
+  } else if (c->prm.bc == 1) {
     const int nb = (int)std::min<long>(RED_BLOCKS, std::max<long>(1, nblocks(M, 256)));
     SolveState *S = c->gm;
     double *part = S->part.p;
@@ -791,8 +868,8 @@ int solve_density_potential(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_op
     hipLaunchKernelGGL(k_rhs2_bounded, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, c->rho.p,
                        c->pbnd.p, c->c_pot, c->rhs2.p);
   }
-  r = gmres(c, A2, c->rhs2.p, c->phi.p, *o2, c->prm.bc == 1, &inf.its_phi, &inf.reason_phi,
-            &inf.rnorm_phi, &inf.refine_phi);
+  r = gmres_boxes(c, A2, c->rhs2.p, c->phi.p, *o2, c->prm.bc == 1, &inf.its_phi, &inf.reason_phi,
+                  &inf.rnorm_phi, &inf.refine_phi);
   if (r) return r;
   inf.nc = c->nc;
   inf.nv = c->nv;

@@ -62,6 +62,21 @@ struct SolveState;  // apc_gmres.hip
 
 struct apc_ctx {
   apc_params prm;
+  // packed boxes (apc_create_packed): nbox independent boxes of the same
+  // parameters in one context.  Box b owns the particles [pstart[b],
+  // pstart[b+1]) in every device order (the box id is the top of the sort
+  // key) and the rows [rstart[b], rstart[b+1]) of the selection, its
+  // computational rows first.  A plain context is one box.
+  int nbox = 1, boxbits = 0;
+  // the caller's depth and maxdepth (a packed context's key budget may lower
+  // prm.depth and prm.maxdepth)
+  int depth_req = 0, maxdepth_req = 0;
+  std::vector<int> pstart_h, rstart_h, rnc_h;
+  apc::DBuf<int> pstart, rstart, rowbox;
+  apc::DBuf<double> boxmass;            // per box: factor of the density constant
+  apc::DBuf<uint64_t> pk_key, pk_key2;  // packed selection: rows sorted by box
+  apc::DBuf<int> pk_val, pk_val2, pk_leaf, pk_type;
+  apc::DBuf<double> pk_b, pk_sig;       // packed solves: scaled right-hand side, box scales
   hipStream_t s = nullptr;
   int dev = 0;
   std::string err;

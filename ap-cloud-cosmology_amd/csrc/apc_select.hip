@@ -145,18 +145,79 @@ __global__ void k_td_open(SelGeom g, Tree t, const uint64_t *__restrict__ key, i
   t.child[i] = base;
 }
 
-__global__ void k_root(SelGeom g, Tree t, int n) {
-  if (threadIdx.x != 0) return;
-  t.prefix[0] = 0;
-  t.level[0] = 0;
-  t.first[0] = 0;
-  t.np[0] = n;
-  t.status[0] = ST_LEAF_TD;
-  t.child[0] = -1;
-  t.parent[0] = -1;
-  t.cx[0] = (g.bb[1] + g.bb[0]) / 2;
-  t.cy[0] = (g.bb[3] + g.bb[2]) / 2;
-  t.cz[0] = (g.bb[5] + g.bb[4]) / 2;
+// The roots: one per box (nodes 0..nbox-1), each over its box's particles
+// (pstart; NULL for a plain context: one root over all n).
+__global__ void k_roots(SelGeom g, Tree t, int nbox, const int *__restrict__ pstart, int n) {
+  const int b = threadIdx.x;
+  if (b >= nbox) return;
+  t.prefix[b] = 0;
+  t.level[b] = 0;
+  t.first[b] = pstart ? pstart[b] : 0;
+  t.np[b] = pstart ? pstart[b + 1] - pstart[b] : n;
+  t.status[b] = ST_LEAF_TD;
+  t.child[b] = -1;
+  t.parent[b] = -1;
+  t.cx[b] = (g.bb[1] + g.bb[0]) / 2;
+  t.cy[b] = (g.bb[3] + g.bb[2]) / 2;
+  t.cz[b] = (g.bb[5] + g.bb[4]) / 2;
+}
+
+// Box of a node: the root above it (roots are nodes 0..nbox-1).
+__device__ inline int node_box(const Tree &t, int i) {
+  while (t.parent[i] >= 0) i = t.parent[i];
+  return i;
+}
+
+// Packed boxes: sort keys (box) and values (current output position) of the
+// rows [comp (creation order) | vacuum (level, Morton)]; a stable sort by
+// box then gives every box its computational rows, then its vacuum rows, in
+// the order a context of that box alone would give them.  Also flags a leaf
+// the lowered depth keeps from opening (cut != 0).
+__global__ void k_pack_keys(SelGeom g, Tree t, int M, const int *__restrict__ leaf_node,
+                            int maxdepth_req, uint64_t *__restrict__ key, int *__restrict__ val,
+                            int *__restrict__ cut) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= M) return;
+  const int i = leaf_node[r];
+  key[r] = (uint64_t)node_box(t, i);
+  val[r] = r;
+  const int lv = t.level[i];
+  if (lv == g.maxdepth && g.maxdepth < maxdepth_req) {
+    bool crit;
+    if (g.leaf_cap > 0) {
+      crit = t.np[i] > g.leaf_cap;
+    } else {
+      double temp = 1.0 * t.np[i];
+      for (int q = 0; q < 2 * g.order - 2; ++q) temp = temp * g.hl[lv];
+      crit = temp > g.c;
+    }
+    if (crit) atomicOr(cut, 1);
+  }
+}
+
+// Rows in box-major order: leaf, output map, type, box of each row; the
+// first row of every box and of its vacuum rows (vstart, -1 if none).
+__global__ void k_pack_emit(int M, int nc, const uint64_t *__restrict__ key,
+                            const int *__restrict__ val, const int *__restrict__ leaf_in,
+                            int *__restrict__ leaf_out, int *__restrict__ out,
+                            int *__restrict__ rtype, int *__restrict__ rowbox,
+                            int *__restrict__ rstart, int *__restrict__ vstart) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= M) return;
+  const int b = (int)key[p], v = val[p];
+  const int i = leaf_in[v];
+  leaf_out[p] = i;
+  out[i] = p;
+  rtype[p] = (v < nc) ? 0 : -1;
+  rowbox[p] = b;
+  const bool first = (p == 0) || (int)key[p - 1] != b;
+  if (first) rstart[b] = p;
+  if (v >= nc && (first || val[p - 1] < nc)) vstart[b] = p;
+}
+
+__global__ void k_fill_int(int n, int v, int *__restrict__ a) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) a[i] = v;
 }
 
 // -------------------------------------------------------------- balance --
@@ -241,6 +302,7 @@ __global__ void k_vac_emit(int nv, int nc, const int *__restrict__ node, int *__
 }
 
 __global__ void k_leaf_data(SelGeom g, Tree t, int M, const int *__restrict__ leaf_node, int nc,
+                            const int *__restrict__ rtype,
                             double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
                             double *__restrict__ h, int *__restrict__ np, int *__restrict__ first,
                             int *__restrict__ type, int *__restrict__ bcount) {
@@ -254,7 +316,7 @@ __global__ void k_leaf_data(SelGeom g, Tree t, int M, const int *__restrict__ le
   h[r] = g.hl[lv];
   np[r] = t.np[i];
   first[r] = t.first[i];
-  type[r] = (r < nc) ? 0 : -1;
+  type[r] = rtype ? rtype[r] : ((r < nc) ? 0 : -1);
   int nbf = 0;
   if (g.bc != 1) {
     uint64_t dummy;
@@ -395,6 +457,12 @@ static int read_ints(apc_ctx *c, const int *dev, int *host, int n) {
 int select_tree(apc_ctx *c, double cc, int order, int mindepth, int maxdepth) {
   if (!c->sorted) return fail(c, APC_ERR_ARG, "select: octree not built");
   const apc_params &p = c->prm;
+  // a packed context's key budget may have lowered the depth below the
+  // caller's: select to the lowered depth and refuse (below) a tree that the
+  // caller's depth would have refined further
+  int maxdepth_req = maxdepth;
+  if (c->nbox > 1 && maxdepth > p.depth && maxdepth <= c->depth_req && mindepth <= p.depth)
+    maxdepth = p.depth;
   if (maxdepth > p.depth || maxdepth < 0 || mindepth < 0 || order < 1)
     return fail(c, APC_ERR_UNSUPPORTED, "select: need 0 <= mindepth, maxdepth <= depth (%d)", p.depth);
   SelGeom g = make_geom(c, cc, order, mindepth, maxdepth);
@@ -405,10 +473,12 @@ int select_tree(apc_ctx *c, double cc, int order, int mindepth, int maxdepth) {
   int r = ensure_nodes(c, cap);
   if (r) return r;
   Tree t = tree_of(c);
-  hipLaunchKernelGGL(k_root, dim3(1), dim3(64), 0, c->s, g, t, n);
+  const int nbox = c->nbox;
+  hipLaunchKernelGGL(k_roots, dim3(1), dim3(64), 0, c->s, g, t, nbox,
+                     nbox > 1 ? (const int *)c->pstart.p : nullptr, n);
   c->lvl_start.assign(1, 0);
-  c->lvl_start.push_back(1);
-  int T = 1;
+  c->lvl_start.push_back(nbox);
+  int T = nbox;
   int hc[4];
   // ---- top-down, one level per pass
   for (int lv = 0;; ++lv) {
@@ -564,6 +634,51 @@ int select_tree(apc_ctx *c, double cc, int order, int mindepth, int maxdepth) {
     }
   }
   const int M = nc + nv;
+  const int *rtype = nullptr;
+  if (nbox > 1) {
+    // box-major rows: one stable sort of [comp | vacuum] by box
+    APC_ALLOC(c, c->pk_key, M + 1); APC_ALLOC(c, c->pk_key2, M + 1);
+    APC_ALLOC(c, c->pk_val, M + 1); APC_ALLOC(c, c->pk_val2, M + 1);
+    APC_ALLOC(c, c->pk_leaf, M + 1); APC_ALLOC(c, c->pk_type, M + 1);
+    APC_ALLOC(c, c->rowbox, M + 1); APC_ALLOC(c, c->rstart, 2 * nbox + 2);
+    APC_HIP(c, hipMemsetAsync(c->dcount.p + 8, 0, sizeof(int), c->s));
+    hipLaunchKernelGGL(k_pack_keys, dim3(nblocks(M, 256)), dim3(256), 0, c->s, g, t, M,
+                       c->leaf_node.p, maxdepth_req, c->pk_key.p, c->pk_val.p, c->dcount.p + 8);
+    r = radix_sort_pairs(c, c->pk_key.p, c->pk_val.p, c->pk_key2.p, c->pk_val2.p, M, c->boxbits);
+    if (r) return r;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, c->s, 2 * nbox + 2, -1, c->rstart.p);
+    hipLaunchKernelGGL(k_pack_emit, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, nc,
+                       c->pk_key.p, c->pk_val.p, c->leaf_node.p, c->pk_leaf.p, c->nout.p,
+                       c->pk_type.p, c->rowbox.p, c->rstart.p, c->rstart.p + nbox + 1);
+    std::swap(c->leaf_node, c->pk_leaf);
+    rtype = c->pk_type.p;
+    std::vector<int> st(2 * nbox + 2);
+    int cut = 0;
+    APC_HIP(c, hipMemcpyAsync(st.data(), c->rstart.p, sizeof(int) * (2 * nbox + 2),
+                              hipMemcpyDeviceToHost, c->s));
+    APC_HIP(c, hipMemcpyAsync(&cut, c->dcount.p + 8, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    APC_HIP(c, hipStreamSynchronize(c->s));
+    if (cut)
+      return fail(c, APC_ERR_UNSUPPORTED, "select: packed boxes cut the tree at depth %d (the "
+                  "64-bit sort key leaves 3 bits per level below the box id); use fewer boxes",
+                  p.depth);
+    c->rstart_h.assign(nbox + 1, M);
+    c->rnc_h.assign(nbox, 0);
+    for (int b = 0; b < nbox; ++b) {
+      if (st[b] < 0) return fail(c, APC_ERR_ARG, "select: box %d has no rows", b);
+      c->rstart_h[b] = st[b];
+    }
+    for (int b = 0; b < nbox; ++b) {
+      const int vs = st[nbox + 1 + b] >= 0 ? st[nbox + 1 + b] : c->rstart_h[b + 1];
+      c->rnc_h[b] = vs - c->rstart_h[b];
+    }
+    st.assign(c->rstart_h.begin(), c->rstart_h.end());
+    APC_HIP(c, hipMemcpyAsync(c->rstart.p, st.data(), sizeof(int) * (nbox + 1),
+                              hipMemcpyHostToDevice, c->s));
+  } else {
+    c->rstart_h.assign({0, M});
+    c->rnc_h.assign(1, nc);
+  }
   // boundary faces per leaf -> boundary node ids
   APC_ALLOC(c, c->n24first, (size_t)M + 1);
   APC_ALLOC(c, c->n24cnt, (size_t)M + 1);
@@ -575,7 +690,7 @@ int select_tree(apc_ctx *c, double cc, int order, int mindepth, int maxdepth) {
   {
     KScope ks(c, "select_output");
     hipLaunchKernelGGL(k_leaf_data, dim3(nblocks(M, 256)), dim3(256), 0, c->s, g, t, M,
-                       c->leaf_node.p, nc, c->cx.p, c->cy.p, c->cz.p, c->ch.p, c->cnp.p,
+                       c->leaf_node.p, nc, rtype, c->cx.p, c->cy.p, c->cz.p, c->ch.p, c->cnp.p,
                        c->cfirst.p, c->ctype.p, c->n24cnt.p);
   }
   r = scan_exclusive(c, c->n24cnt.p, c->n24first.p, M, c->dcount.p);
@@ -717,6 +832,7 @@ extern "C" int apc_set_comp(apc_ctx *c, int nc, int nv, int nb, const double *x,
                             const int *neigh24, int nneigh) {
   if (!c) return APC_ERR_ARG;
   if (c->frozen) return fail(c, APC_ERR_ARG, "set_comp: the topology is frozen");
+  if (c->nbox > 1) return fail(c, APC_ERR_UNSUPPORTED, "set_comp: not on a packed multi-box context");
   if (nc < 0 || nv < 0 || nb < 0 || nc + nv < 1 || nneigh < 0)
     return fail(c, APC_ERR_ARG, "set_comp: bad sizes nc=%d nv=%d nb=%d", nc, nv, nb);
   if (!x || !y || !z || !cell_length || !np || !type || !first_neigh || !number_neigh ||

@@ -15,7 +15,8 @@ __global__ void __launch_bounds__(256) k_keys(int n, const double *__restrict__ 
                                               const double *__restrict__ z, double bx0,
                                               double bx1, double by0, double by1, double bz0,
                                               double bz1, int depth, uint64_t *__restrict__ key,
-                                              int *__restrict__ idx) {
+                                              int *__restrict__ idx, int nbox = 1,
+                                              const int *__restrict__ pstart = nullptr) {
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   double px = x[i], py = y[i], pz = z[i];
@@ -29,6 +30,17 @@ __global__ void __launch_bounds__(256) k_keys(int n, const double *__restrict__ 
     if (bxb) lx = mx; else rx = mx;
     if (byb) ly = my; else ry = my;
     if (bzb) lz = mz; else rz = mz;
+  }
+  if (nbox > 1) {
+    // packed boxes: the box owning position i (pstart ascending) above the
+    // Morton bits, so the sort groups the boxes and keeps each box's order
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (pstart[mid] <= i) lo = mid;
+      else hi = mid - 1;
+    }
+    k |= (uint64_t)lo << (3 * depth);
   }
   key[i] = k;
   idx[i] = i;
@@ -183,7 +195,7 @@ hipError_t morton_sort_ws(hipStream_t s, int n, const double *x, const double *y
                           int *v2, int *hist, int *histx, int *slab) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_keys, dim3(nblocks(n, 256)), dim3(256), 0, s, n, x, y, z, bbox[0], bbox[1],
-                     bbox[2], bbox[3], bbox[4], bbox[5], depth, keys, idx);
+                     bbox[2], bbox[3], bbox[4], bbox[5], depth, keys, idx, 1, (const int *)nullptr);
   if (n > 1) {
     const int nblk = (n + RX_TILE - 1) / RX_TILE;
     uint64_t *ka = keys, *kb = k2;
@@ -237,9 +249,10 @@ int sort_particles(apc_ctx *c) {
     KScope ks(c, "keys", 36.0 * n);
     hipLaunchKernelGGL(k_keys, dim3(nblocks(n, 256)), dim3(256), 0, c->s, n, c->x.p, c->y.p,
                        c->z.p, p.bbox[0], p.bbox[1], p.bbox[2], p.bbox[3], p.bbox[4], p.bbox[5],
-                       p.depth, c->key.p, c->perm.p);
+                       p.depth, c->key.p, c->perm.p, c->nbox, c->pstart.p);
   }
-  int r = radix_sort_pairs(c, c->key.p, c->perm.p, c->key2.p, c->perm2.p, n, 3 * p.depth);
+  int r = radix_sort_pairs(c, c->key.p, c->perm.p, c->key2.p, c->perm2.p, n,
+                           3 * p.depth + c->boxbits);
   if (r) return r;
   APC_ALLOC(c, c->t0, n); APC_ALLOC(c, c->t1, n); APC_ALLOC(c, c->t2, n);
   APC_ALLOC(c, c->t3, n); APC_ALLOC(c, c->t4, n); APC_ALLOC(c, c->t5, n);

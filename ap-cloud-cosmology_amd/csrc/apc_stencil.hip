@@ -52,7 +52,7 @@ __device__ inline void fit_max(const FitQueue &Q, double est, double *ws) {
 }
 
 __global__ void __launch_bounds__(256) k_stencil(
-    int M, int lm, Geo G, int pin_row0, double c_dens, const double *__restrict__ cx,
+    int M, int lm, Geo G, int pin_row0, Boxes B, double c_dens, const double *__restrict__ cx,
     const double *__restrict__ cy, const double *__restrict__ cz, const double *__restrict__ ch,
     const int *__restrict__ cnp, const int *__restrict__ nbr, const double *__restrict__ dist,
     const double *__restrict__ bv, double *__restrict__ sco, double *__restrict__ d1,
@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(256) k_stencil(
                 gram_add(R, r);
               });
     const bool deficient = chol_inplace(R);
-    rhs1[i] = 1.0 * cnp[i] / ch[i] / ch[i] / ch[i] * c_dens;
+    rhs1[i] = 1.0 * cnp[i] / ch[i] / ch[i] / ch[i] * (c_dens * box_mass(B, i));
     if (!fit_needs_qr(Q, i, R, deficient, &est)) {
       // y = R^-1 R^-T u, u = e3+e4+e5
       double z[9];
@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(256) k_stencil(
       constexpr double I24 = 1.0 / 24.0;
       double p2a = 0.0, p2b = 0.0, p3 = 0.0;
       const double ih2 = f.ih * f.ih;
-      const bool pinned = pin_row0 && i == 0;
+      const bool pinned = pin_row(B, pin_row0, i);
       for_slots(M, lm, i, nbr, dist, cx, cy, cz, NoExtra{},
                 [&](int j, int m, double d, double x, double yy, double zz, double) {
                   double r[9];
@@ -202,7 +202,8 @@ __global__ void __launch_bounds__(256) k_gradient(
 // A zero diagonal of R (dgels' info > 0) is APC_ERR_RANK.
 template <bool GRAD>
 __global__ void __launch_bounds__(64) k_fit_qr(
-    int nq, const int *__restrict__ qrows, int M, int lm, Geo G, int pin_row0, double c_field,
+    int nq, const int *__restrict__ qrows, int M, int lm, Geo G, int pin_row0, Boxes B,
+    double c_field,
     const double *__restrict__ cx, const double *__restrict__ cy, const double *__restrict__ cz,
     const double *__restrict__ ch, const int *__restrict__ nbr, const double *__restrict__ dist,
     const double *__restrict__ phi, const double *__restrict__ bv, double *__restrict__ sco,
@@ -286,7 +287,7 @@ __global__ void __launch_bounds__(64) k_fit_qr(
         if (mj >= M) p3 = p3 - bv[mj - M] * a2;
       }
       if (lane == 0) {
-        const bool pinned = pin_row0 && i == 0;
+        const bool pinned = pin_row(B, pin_row0, i);
         const double dd1 = p2a + 1, dd2 = pinned ? 1.0 : p2b;
         d1[i] = dd1;
         d2[i] = dd2;
@@ -375,12 +376,14 @@ static int fit_finish(apc_ctx *c, bool grad, const char *what) {
               (double)nq * ((grad ? 44.0 : 44.0) * lm + 100.0));
     if (grad)
       hipLaunchKernelGGL((k_fit_qr<true>), dim3(g), dim3(64), 0, c->s, nq, c->fitq_rows.p, M, lm,
-                         geo_of(c), pin, c->c_field, c->cx.p, c->cy.p, c->cz.p, c->ch.p, c->nbr.p,
+                         geo_of(c), pin, boxes_of(c), c->c_field, c->cx.p, c->cy.p, c->cz.p,
+                         c->ch.p, c->nbr.p,
                          c->dist.p, c->phi.p, c->bv.p, nullptr, nullptr, nullptr, nullptr,
                          nullptr, nullptr, nullptr, nullptr, c->grad.p, c->status_dev.p);
     else
       hipLaunchKernelGGL((k_fit_qr<false>), dim3(g), dim3(64), 0, c->s, nq, c->fitq_rows.p, M,
-                         lm, geo_of(c), pin, c->c_field, c->cx.p, c->cy.p, c->cz.p, c->ch.p,
+                         lm, geo_of(c), pin, boxes_of(c), c->c_field, c->cx.p, c->cy.p, c->cz.p,
+                         c->ch.p,
                          c->nbr.p, c->dist.p, (const double *)nullptr, c->bv.p, c->sco.p, c->d1.p,
                          c->d2.p, c->f1.p, c->f2.p, c->dinv1.p, c->dinv2.p, c->pbnd.p, nullptr,
                          c->status_dev.p);
@@ -418,7 +421,7 @@ int build_stencils(apc_ctx *c) {
     // per row: own xyz, h, n_p 36 read, eight row arrays and the estimate 72 written
     KScope ks(c, "stencil", (double)M * (44.0 * lm + 108.0));
     hipLaunchKernelGGL(k_stencil, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, lm, geo_of(c), pin,
-                       c->c_dens, c->cx.p, c->cy.p, c->cz.p, c->ch.p, c->cnp.p, c->nbr.p,
+                       boxes_of(c), c->c_dens, c->cx.p, c->cy.p, c->cz.p, c->ch.p, c->cnp.p, c->nbr.p,
                        c->dist.p, c->bv.p, c->sco.p, c->d1.p, c->d2.p, c->f1.p, c->f2.p,
                        c->dinv1.p, c->dinv2.p, c->rhs1.p, c->pbnd.p, Q);
   }

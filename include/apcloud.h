@@ -97,6 +97,29 @@ int apc_device_count(int *n);
 int apc_create(const apc_params *p, apc_ctx **out);
 int apc_destroy(apc_ctx *ctx);
 
+/* Packed boxes: nbox (<= 64) independent periodic boxes of the same
+ * parameters in one context.  No counterpart in the reference, which runs one
+ * box per Controller (controller.cpp:16-219); a packed context runs every box
+ * through each kernel of the path in one launch.  counts[b] particles of box
+ * b are the rows pstart[b] .. pstart[b+1]-1 of apc_set_particles (pstart the
+ * running sum of counts).  The box id sits above the Morton bits of the
+ * 64-bit sort key, which lowers the depth to (64 - ceil(log2 nbox)) / 3 when
+ * p->depth is deeper; apc_select then refuses (APC_ERR_UNSUPPORTED) a tree
+ * that p->depth would refine beyond the lowered depth.  Computational rows
+ * are box-major: box b's rows are row_start[b] .. row_start[b+1]-1 (see
+ * apc_pack_rows), its computational nodes first, each box in the order a
+ * context of that box alone gives.  Every box's potential system pins its
+ * first row; the density right-hand side of box b carries mass[b]
+ * (apc_pack_set_masses, default 1) times c_dens.  Each GMRES solve scales
+ * every box to unit right-hand side and divides rtol by sqrt(nbox), so that
+ * every box meets rtol on its own.  bc must be 1 (APC_ERR_UNSUPPORTED
+ * otherwise); apc_set_comp and apc_freeze_topology refuse a packed context. */
+int apc_create_packed(const apc_params *p, int nbox, const int *counts, apc_ctx **out);
+int apc_pack_set_masses(apc_ctx *ctx, const double *mass);
+/* row_start: nbox + 1 entries, comp_rows: nbox entries (either may be NULL),
+ * valid after apc_select. */
+int apc_pack_rows(apc_ctx *ctx, int *row_start, int *comp_rows);
+
 /* initialization.cpp:41-91 hands the path a Physical_particle; this copies
  * the caller's SoA arrays (particle.h:5-21) to HBM. */
 int apc_set_particles(apc_ctx *ctx, int n, const double *x, const double *y, const double *z,
