@@ -38,6 +38,7 @@ EXPORTS = [
     "apc_batch_last_error", "apc_batch_set_particles", "apc_batch_run",
     "apc_morton_sort_noalloc", "apc_set_comp", "apc_solver_set_fit_threshold",
     "apc_fit_conditions", "apc_create_packed", "apc_pack_set_masses", "apc_pack_rows",
+    "apc_pack_solve_info",
 ]
 
 
@@ -91,6 +92,7 @@ def lib():
         L.apc_create_packed.argtypes = [C.POINTER(Params), C.c_int, _I, C.POINTER(C.c_void_p)]
         L.apc_pack_set_masses.argtypes = [C.c_void_p, _D]
         L.apc_pack_rows.argtypes = [C.c_void_p, _I, _I]
+        L.apc_pack_solve_info.argtypes = [C.c_void_p, _I, _I, _I, _I, _D, _D]
         L.apc_set_particles.argtypes = [C.c_void_p, C.c_int] + [_D] * 6
         L.apc_get_particles.argtypes = [C.c_void_p] + [_D] * 9
         L.apc_octree_build.argtypes = [C.c_void_p]
@@ -595,3 +597,13 @@ class PackedContext(Context):
         nc = np.zeros(nbox, np.int32)
         self._chk(self.L.apc_pack_rows(self.h, _p(rs, _I), _p(nc, _I)), "apc_pack_rows")
         return rs, nc
+
+    def solve_info(self):
+        """Per box: (its_rho, its_phi, reason_rho, reason_phi, res_rho,
+        res_phi) of the last solve."""
+        nbox = len(self.counts)
+        ints = [np.zeros(nbox, np.int32) for _ in range(4)]
+        res = [np.zeros(nbox) for _ in range(2)]
+        self._chk(self.L.apc_pack_solve_info(self.h, *[_p(a, _I) for a in ints],
+                                             *[_p(a) for a in res]), "apc_pack_solve_info")
+        return tuple(ints) + tuple(res)

@@ -48,10 +48,14 @@ struct AmgLevel {
   DBuf<int> agg;      // coarse row of each row (all but the coarsest level)
   DBuf<int> mp, ml;   // rows of each coarse row, grouped (CSR over next level)
   DBuf<int> box;      // packed boxes: the box of each row (levels below the finest)
+  // packed boxes: 1 on the rows of a box whose coarsening has stopped at or
+  // above this level (no smoothing here: dinv 0; the coarse correction taken
+  // once, unscaled: the box's coarsest solve passes through unchanged)
+  DBuf<unsigned char> pass;
   DBuf<double> x, b, r, t;
   DBuf<float> x32, b32, r32, t32;
   void release() {
-    box.release();
+    box.release(); pass.release();
     soff.release(); swid.release(); scol.release(); sval.release(); sval32.release();
     rp.release(); col.release(); val.release(); dinv.release(); val32.release();
     dinv32.release(); agg.release(); mp.release(); ml.release(); x.release(); b.release();
@@ -85,7 +89,14 @@ struct AmgHier {
   int nlev = 0;
   int nc = 0;
   bool dense = false;
-  DBuf<double> gj, cinv, colk, boxw;
+  DBuf<double> gj, cinv, colk;
+  // packed boxes: per-box coarsening stops (frozen), the coarsest level's box
+  // blocks (first rows, Gauss-Jordan and inverse offsets, shifts)
+  int nbox = 1;
+  bool blocks = false;
+  DBuf<int> frozen, bstart;
+  DBuf<long long> goff, ioff;
+  DBuf<double> bshift;
   DBuf<float> cinv32;
   AmgOptions o;
   bool fp32 = false;
@@ -98,7 +109,8 @@ struct AmgHier {
   ~AmgHier() {
     for (auto &g : graphs) g.release(s);
     for (auto *l : L) { l->release(); delete l; }
-    gj.release(); cinv.release(); colk.release(); boxw.release(); cinv32.release(); key.release();
+    gj.release(); cinv.release(); colk.release(); cinv32.release(); key.release();
+    frozen.release(); bstart.release(); goff.release(); ioff.release(); bshift.release();
     key2.release(); idx.release(); idx2.release(); sa.release(); sb.release(); flag.release();
     pos.release(); cnt.release(); ev.release(); smax.release(); diag.release();
     gcol.release(); gval.release();
@@ -662,20 +674,12 @@ __global__ void k_amg_convert(size_t n, const S *__restrict__ a, D *__restrict__
     b[i] = (D)a[i];
 }
 
-// [A | I] for Gauss-Jordan, A from CSR plus shift/n everywhere; with packed
-// boxes (box non-null) shift/n_b in the block of each box b instead (the
-// constant mode of every box), bw[i] = 1/n_b of row i's box.
-__global__ void k_gj_fill(Csr<double> A, double shift, const int *__restrict__ box,
-                          const double *__restrict__ bw, double *__restrict__ g) {
+// [A | I] for Gauss-Jordan, A from CSR plus shift/n everywhere.
+__global__ void k_gj_fill(Csr<double> A, double shift, double *__restrict__ g) {
   const int n = A.n;
   int i = blockIdx.x;
-  for (int j = threadIdx.x; j < 2 * n; j += blockDim.x) {
-    double e;
-    if (j >= n) e = (j - n == i) ? 1.0 : 0.0;
-    else if (box) e = (box[i] == box[j]) ? shift * bw[i] : 0.0;
-    else e = shift / n;
-    g[(size_t)i * 2 * n + j] = e;
-  }
+  for (int j = threadIdx.x; j < 2 * n; j += blockDim.x)
+    g[(size_t)i * 2 * n + j] = (j < n) ? shift / n : (j - n == i ? 1.0 : 0.0);
   __syncthreads();
   if (threadIdx.x == 0)
     for (int e = A.rp[i]; e < A.rp[i + 1]; ++e) g[(size_t)i * 2 * n + A.col[e]] += A.val[e];
@@ -816,12 +820,35 @@ __global__ void k_cy_copy(int n, const T *__restrict__ a, T *__restrict__ b, con
   if (i < n) b[i] = a[i];
 }
 
+// x += alpha xc[agg]; rows flagged in pass (packed boxes) take alpha_pass
 template <class T>
 __global__ void k_cy_prolong(int n, const int *__restrict__ agg, const T *__restrict__ xc,
-                             T *__restrict__ x, double alpha, const GState *st) {
+                             T *__restrict__ x, double alpha, const unsigned char *__restrict__ pass,
+                             double alpha_pass, const GState *st) {
   if (st && halted(st)) return;
   int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) x[i] = (T)((double)x[i] + alpha * (double)xc[agg[i]]);
+  if (i >= n) return;
+  const double a = (pass && pass[i]) ? alpha_pass : alpha;
+  x[i] = (T)((double)x[i] + a * (double)xc[agg[i]]);
+}
+
+// packed boxes, coarsest level: x_i = (inverse of row i's box block) b, one
+// wave per row
+template <class T>
+__global__ void __launch_bounds__(64) k_cy_dense_blocks(const int *__restrict__ box,
+                                                        const int *__restrict__ bstart,
+                                                        const long long *__restrict__ ioff,
+                                                        const T *__restrict__ inv,
+                                                        const T *__restrict__ b,
+                                                        T *__restrict__ x, const GState *st) {
+  if (st && halted(st)) return;
+  const int i = blockIdx.x, lane = threadIdx.x;
+  const int bx = box[i], r0 = bstart[bx], n = bstart[bx + 1] - r0;
+  const T *row = inv + ioff[bx] + (size_t)(i - r0) * n;
+  double s = 0.0;
+  for (int j = lane; j < n; j += 64) s += (double)row[j] * (double)b[r0 + j];
+  s = wave_sum(s);
+  if (lane == 0) x[i] = (T)s;
 }
 
 template <class T>
@@ -947,8 +974,18 @@ static int ilog2_ceil(uint64_t v) {
   return std::max(b, 1);
 }
 
+// Packed boxes: the MIS of a level starts with every row of a stopped box a
+// root, so its rows become singleton aggregates and its block is carried
+// unchanged.
+__global__ void k_amg_freeze_roots(int n, const int *__restrict__ box,
+                                   const int *__restrict__ frozen, int *__restrict__ a) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) a[i] = frozen[box[i]] ? 1 : 0;
+}
+
 template <class V>
-static int aggregate_level(apc_ctx *c, AmgHier *h, const V &v, int n, AmgLevel *Lf, int *nc_out) {
+static int aggregate_level(apc_ctx *c, AmgHier *h, const V &v, int n, AmgLevel *Lf, int *nc_out,
+                           const int *box = nullptr) {
   const AmgOptions &o = h->o;
   const unsigned g = nblocks(n, 256);
   APC_ALLOC(c, h->smax, n + 1);
@@ -959,7 +996,11 @@ static int aggregate_level(apc_ctx *c, AmgHier *h, const V &v, int n, AmgLevel *
   APC_ALLOC(c, h->cnt, 4);
   if (o.theta > 0) hipLaunchKernelGGL((k_amg_smax<V>), dim3(g), dim3(256), 0, c->s, v, h->smax.p);
   else APC_HIP(c, hipMemsetAsync(h->smax.p, 0, sizeof(double) * n, c->s));
-  APC_HIP(c, hipMemsetAsync(h->sa.p, 0, sizeof(int) * n, c->s));
+  if (box)  // packed boxes: the rows of stopped boxes start as roots
+    hipLaunchKernelGGL(k_amg_freeze_roots, dim3(g), dim3(256), 0, c->s, n, box, h->frozen.p,
+                       h->sa.p);
+  else
+    APC_HIP(c, hipMemsetAsync(h->sa.p, 0, sizeof(int) * n, c->s));
   int *a = h->sa.p, *b = h->sb.p;
   for (int round = 0;; ++round) {
     if (round > 200) return fail(c, APC_ERR_UNSUPPORTED, "amg: independent set did not settle");
@@ -1194,27 +1235,170 @@ __global__ void k_box_down(int n, const int *__restrict__ agg, const int *__rest
   if (i < n) bc[agg[i]] = bf[i];
 }
 
-static int dense_inverse(apc_ctx *c, AmgHier *h, AmgLevel *L, bool periodic, const int *box) {
+// ---- packed boxes: per-box coarsening stops and the coarsest box blocks ----
+// first row of every box on a box-major level; start[nbox] = n
+__global__ void k_box_starts(int n, int nbox, const int *__restrict__ box, int *__restrict__ start) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  if (i == 0 || box[i - 1] != box[i]) start[box[i]] = i;
+  if (i == n - 1) start[nbox] = n;
+}
+
+// rows of stopped boxes: flagged, and no smoothing (dinv 0)
+__global__ void k_pass_rows(int n, const int *__restrict__ box, const int *__restrict__ frozen,
+                            unsigned char *__restrict__ pass, double *__restrict__ dinv) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int f = frozen[box[i]] ? 1 : 0;
+  pass[i] = (unsigned char)f;
+  if (f) dinv[i] = 0.0;
+}
+
+// shift of box b's block: the mean |a_ii| over its rows (periodic), else 0;
+// one block per box
+__global__ void __launch_bounds__(256) k_bgj_shift(Csr<double> A, const int *__restrict__ bstart,
+                                                   int periodic, double *__restrict__ shift) {
+  __shared__ double ws[4];
+  const int b = blockIdx.x, r0 = bstart[b], r1 = bstart[b + 1];
+  double a = 0.0;
+  for (int i = r0 + threadIdx.x; i < r1; i += 256)
+    for (int e = A.rp[i]; e < A.rp[i + 1]; ++e)
+      if (A.col[e] == i) a += fabs(A.val[e]);
+  a = block_sum256(a, ws);
+  if (threadIdx.x == 0) shift[b] = periodic ? a / (r1 - r0) : 0.0;
+}
+
+// [A_b + shift_b / n_b 11^T | I] of row i's box b at goff[b] (n_b rows of
+// 2 n_b); one block per row
+__global__ void k_bgj_fill(Csr<double> A, const int *__restrict__ box,
+                           const int *__restrict__ bstart, const long long *__restrict__ goff,
+                           const double *__restrict__ shift, double *__restrict__ G) {
+  const int i = blockIdx.x;
+  const int b = box[i], r0 = bstart[b], n = bstart[b + 1] - r0, li = i - r0;
+  double *g = G + goff[b] + (size_t)li * 2 * n;
+  const double sh = shift[b] / n;
+  for (int j = threadIdx.x; j < 2 * n; j += blockDim.x)
+    g[j] = (j < n) ? sh : (j - n == li ? 1.0 : 0.0);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int e = A.rp[i]; e < A.rp[i + 1]; ++e) g[A.col[e] - r0] += A.val[e];
+}
+
+// Gauss-Jordan with partial pivoting of every box block, one block per box
+// (the steps of k_gj_pivot / k_gj_elim / k_gj_extract within one launch);
+// the inverse to inv at ioff[b].  A zero pivot counts in bad.
+__global__ void __launch_bounds__(256) k_bgj(const int *__restrict__ bstart,
+                                             const long long *__restrict__ goff,
+                                             const long long *__restrict__ ioff,
+                                             double *__restrict__ G, double *__restrict__ inv,
+                                             int *__restrict__ bad) {
+  __shared__ double colk[1024];
+  __shared__ double bv[256];
+  __shared__ int bi[256];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int n = bstart[b + 1] - bstart[b], w = 2 * n;
+  double *g = G + goff[b];
+  for (int k = 0; k < n; ++k) {
+    double v = -1.0;
+    int vi = k;
+    for (int i = k + t; i < n; i += 256) {
+      const double a = fabs(g[(size_t)i * w + k]);
+      if (a > v) { v = a; vi = i; }
+    }
+    bv[t] = v;
+    bi[t] = vi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (t < s && (bv[t + s] > bv[t] || (bv[t + s] == bv[t] && bi[t + s] < bi[t]))) {
+        bv[t] = bv[t + s];
+        bi[t] = bi[t + s];
+      }
+      __syncthreads();
+    }
+    const int p = bi[0];
+    if (!(bv[0] > 0.0)) {  // uniform: every thread read the same bv[0]
+      if (t == 0) atomicAdd(bad, 1);
+      return;
+    }
+    if (p != k)
+      for (int j = t; j < w; j += 256) {
+        const double a = g[(size_t)k * w + j];
+        g[(size_t)k * w + j] = g[(size_t)p * w + j];
+        g[(size_t)p * w + j] = a;
+      }
+    __syncthreads();
+    const double ip = 1.0 / g[(size_t)k * w + k];
+    __syncthreads();
+    for (int j = t; j < w; j += 256) g[(size_t)k * w + j] *= ip;
+    __syncthreads();
+    for (int i = t; i < n; i += 256) colk[i] = g[(size_t)i * w + k];
+    __syncthreads();
+    for (int e = t; e < n * w; e += 256) {
+      const int i = e / w, j = e - i * w;
+      if (i != k) g[e] -= colk[i] * g[(size_t)k * w + j];
+    }
+    __syncthreads();
+  }
+  double *o = inv + ioff[b];
+  for (int e = t; e < n * n; e += 256) {
+    const int i = e / n, j = e - i * n;
+    o[e] = g[(size_t)i * w + n + j];
+  }
+}
+
+static int block_inverse(apc_ctx *c, AmgHier *h, AmgLevel *L, bool periodic,
+                         const std::vector<int> &start) {
+  const int n = L->n, nbox = h->nbox;
+  std::vector<long long> go(nbox), io(nbox);
+  long long gt = 0, it = 0;
+  for (int b = 0; b < nbox; ++b) {
+    const long long nb = start[b + 1] - start[b];
+    if (nb < 1 || nb > 1024)
+      return fail(c, APC_ERR_UNSUPPORTED, "amg: box %d keeps %lld rows on the coarsest level "
+                  "(its coarsening stalled above the dense size)", b, nb);
+    go[b] = gt;
+    io[b] = it;
+    gt += 2 * nb * nb;
+    it += nb * nb;
+  }
+  Csr<double> A{L->rp.p, L->col.p, L->val.p, n};
+  APC_ALLOC(c, h->bstart, nbox + 1);
+  APC_ALLOC(c, h->goff, nbox);
+  APC_ALLOC(c, h->ioff, nbox);
+  APC_ALLOC(c, h->bshift, nbox);
+  APC_ALLOC(c, h->gj, gt + 1);
+  APC_ALLOC(c, h->cinv, it + 1);
+  APC_ALLOC(c, h->cnt, 4);
+  APC_HIP(c, hipMemcpyAsync(h->bstart.p, start.data(), sizeof(int) * (nbox + 1),
+                            hipMemcpyHostToDevice, c->s));
+  APC_HIP(c, hipMemcpyAsync(h->goff.p, go.data(), sizeof(long long) * nbox, hipMemcpyHostToDevice,
+                            c->s));
+  APC_HIP(c, hipMemcpyAsync(h->ioff.p, io.data(), sizeof(long long) * nbox, hipMemcpyHostToDevice,
+                            c->s));
+  APC_HIP(c, hipMemsetAsync(h->cnt.p, 0, sizeof(int), c->s));
+  hipLaunchKernelGGL(k_bgj_shift, dim3(nbox), dim3(256), 0, c->s, A, h->bstart.p,
+                     (periodic && h->o.shift) ? 1 : 0, h->bshift.p);
+  hipLaunchKernelGGL(k_bgj_fill, dim3(n), dim3(256), 0, c->s, A, L->box.p, h->bstart.p, h->goff.p,
+                     h->bshift.p, h->gj.p);
+  hipLaunchKernelGGL(k_bgj, dim3(nbox), dim3(256), 0, c->s, h->bstart.p, h->goff.p, h->ioff.p,
+                     h->gj.p, h->cinv.p, h->cnt.p);
+  int bad = 0;
+  APC_HIP(c, hipMemcpyAsync(&bad, h->cnt.p, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));  // the offsets' host vectors go out of scope
+  if (bad) return fail(c, APC_ERR_UNSUPPORTED, "amg: %d singular coarsest box blocks", bad);
+  if (h->fp32) {
+    APC_ALLOC(c, h->cinv32, it + 1);
+    hipLaunchKernelGGL((k_amg_convert<double, float>), dim3(256), dim3(256), 0, c->s, (size_t)it,
+                       h->cinv.p, h->cinv32.p);
+  }
+  return 0;
+}
+
+static int dense_inverse(apc_ctx *c, AmgHier *h, AmgLevel *L, bool periodic) {
   const int n = L->n;
+  periodic = periodic && h->o.shift;
   Csr<double> A{L->rp.p, L->col.p, L->val.p, n};
   double shift = 0.0;
-  if (periodic && box) {
-    std::vector<int> hb(n);
-    APC_HIP(c, hipMemcpyAsync(hb.data(), box, sizeof(int) * n, hipMemcpyDeviceToHost, c->s));
-    APC_HIP(c, hipStreamSynchronize(c->s));
-    std::vector<int> cnt(c->nbox, 0);
-    for (int b : hb) {
-      if (b < 0 || b >= c->nbox) return fail(c, APC_ERR_ARG, "amg: coarse row of box %d", b);
-      ++cnt[b];
-    }
-    std::vector<double> w(n);
-    for (int i = 0; i < n; ++i) w[i] = 1.0 / cnt[hb[i]];
-    APC_ALLOC(c, h->boxw, n + 1);
-    APC_HIP(c, hipMemcpyAsync(h->boxw.p, w.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->s));
-    APC_HIP(c, hipStreamSynchronize(c->s));
-  } else {
-    box = nullptr;
-  }
   if (periodic) {  // mean |diagonal|, from the host copy of the level's dinv
     std::vector<double> dv(n);
     APC_HIP(c, hipMemcpyAsync(dv.data(), L->dinv.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
@@ -1230,8 +1414,7 @@ static int dense_inverse(apc_ctx *c, AmgHier *h, AmgLevel *L, bool periodic, con
   APC_ALLOC(c, h->cinv, (size_t)n * n + 1);
   APC_ALLOC(c, h->cnt, 4);
   APC_HIP(c, hipMemsetAsync(h->cnt.p, 0, sizeof(int), c->s));
-  hipLaunchKernelGGL(k_gj_fill, dim3(n), dim3(256), 0, c->s, A, shift, box,
-                     (const double *)h->boxw.p, h->gj.p);
+  hipLaunchKernelGGL(k_gj_fill, dim3(n), dim3(256), 0, c->s, A, shift, h->gj.p);
   const unsigned ge = std::max(1u, std::min(1024u, nblocks((long)2 * n * n, 256)));
   for (int k = 0; k < n; ++k) {
     hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, c->s, n, k, h->gj.p, h->colk.p, h->cnt.p);
@@ -1267,7 +1450,30 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
     return h->L[l];
   };
   KScope ks(c, "amg_setup");
-  const int *box0 = (periodic && c->nbox > 1) ? (const int *)c->rowbox.p : nullptr;
+  // Packed boxes: every box stops coarsening where a context of that box
+  // alone would (at most coarse_max rows, but level 0 coarsened once; or when
+  // its coarsening stalls); its rows then pass through the deeper levels as
+  // singleton aggregates, and the coarsest level solves every box's block
+  // with its own dense inverse.  Levels stay box-major: coarse rows are
+  // numbered by their roots' rows.
+  const bool packed = c->nbox > 1;
+  const int nbox = c->nbox;
+  h->nbox = nbox;
+  h->blocks = false;
+  std::vector<int> frozen(nbox, 0), cnt(nbox, 0), start;
+  if (packed) {
+    if ((int)c->rstart_h.size() != nbox + 1 || c->rstart_h[nbox] != A.M)
+      return fail(c, APC_ERR_ARG, "amg: the boxes' rows do not cover the system");
+    for (int b = 0; b < nbox; ++b) cnt[b] = c->rstart_h[b + 1] - c->rstart_h[b];
+    start = c->rstart_h;
+    APC_ALLOC(c, h->frozen, nbox);
+    APC_HIP(c, hipMemsetAsync(h->frozen.p, 0, sizeof(int) * nbox, c->s));
+  }
+  auto all_frozen = [&]() {
+    for (int f : frozen)
+      if (!f) return false;
+    return true;
+  };
   AmgLevel *L0 = level(0);
   L0->n = A.M;
   L0->nnz = A.M * (A.lm + 1);
@@ -1282,19 +1488,39 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
   for (int l = 0;; ++l) {
     AmgLevel *Lf = level(l);
     // stop at the dense size (but coarsen level 0 at least once) or the cap
-    if ((l > 0 && n <= o.coarse_max) || l + 1 >= o.max_levels) break;
+    if (packed ? (l > 0 && all_frozen()) : (l > 0 && n <= o.coarse_max)) break;
+    if (l + 1 >= o.max_levels) break;
+    const int *boxf = packed ? (l == 0 ? (const int *)c->rowbox.p : (const int *)Lf->box.p)
+                             : nullptr;
     int nc = 0;
     {
       KScope ks1(c, "amg_setup_aggregate");
-      if (l == 0) r = aggregate_level(c, h, EllView{A}, n, Lf, &nc);
-      else r = aggregate_level(c, h, CsrView{Csr<double>{Lf->rp.p, Lf->col.p, Lf->val.p, n}}, n, Lf, &nc);
+      if (l == 0) r = aggregate_level(c, h, EllView{A}, n, Lf, &nc, boxf);
+      else r = aggregate_level(c, h, CsrView{Csr<double>{Lf->rp.p, Lf->col.p, Lf->val.p, n}}, n, Lf, &nc, boxf);
     }
     if (r) return r;
     if (nc < 1 || nc >= n) break;  // no coarsening: this level is the coarsest
-    if (box0) {
-      APC_ALLOC(c, level(l + 1)->box, nc + 1);
-      hipLaunchKernelGGL(k_box_down, dim3(nblocks(n, 256)), dim3(256), 0, c->s, n, Lf->agg.p,
-                         l == 0 ? box0 : (const int *)Lf->box.p, level(l + 1)->box.p);
+    if (packed) {
+      AmgLevel *Ln = level(l + 1);
+      APC_ALLOC(c, Ln->box, nc + 1);
+      APC_ALLOC(c, h->bstart, nbox + 1);
+      hipLaunchKernelGGL(k_box_down, dim3(nblocks(n, 256)), dim3(256), 0, c->s, n, Lf->agg.p, boxf,
+                         Ln->box.p);
+      hipLaunchKernelGGL(k_box_starts, dim3(nblocks(nc, 256)), dim3(256), 0, c->s, nc, nbox,
+                         (const int *)Ln->box.p, h->bstart.p);
+      start.assign(nbox + 1, 0);
+      APC_HIP(c, hipMemcpyAsync(start.data(), h->bstart.p, sizeof(int) * (nbox + 1),
+                                hipMemcpyDeviceToHost, c->s));
+      APC_HIP(c, hipStreamSynchronize(c->s));
+      for (int b = 0; b < nbox; ++b) {
+        const int m = start[b + 1] - start[b];
+        if (m < 1) return fail(c, APC_ERR_ARG, "amg: box %d has no rows on level %d", b, l + 1);
+        if (!frozen[b] && (m <= o.coarse_max || m >= cnt[b])) frozen[b] = 1;
+        cnt[b] = m;
+      }
+      APC_HIP(c, hipMemcpyAsync(h->frozen.p, frozen.data(), sizeof(int) * nbox,
+                                hipMemcpyHostToDevice, c->s));
+      APC_HIP(c, hipStreamSynchronize(c->s));  // the flags' host vector changes on the next level
     }
     {
       KScope ks1(c, "amg_setup_group");
@@ -1319,6 +1545,11 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
       r = build_sell(c, h, Lc);
     }
     if (r) return r;
+    if (packed) {
+      APC_ALLOC(c, Lc->pass, nc + 1);
+      hipLaunchKernelGGL(k_pass_rows, dim3(nblocks(nc, 256)), dim3(256), 0, c->s, nc,
+                         (const int *)Lc->box.p, (const int *)h->frozen.p, Lc->pass.p, Lc->dinv.p);
+    }
     if (fp32) {
       KScope ks1(c, "amg_setup_convert");
       APC_ALLOC(c, Lc->sval32, Lc->sell_len + 1);
@@ -1334,9 +1565,16 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
   h->nlev = k + 1;
   AmgLevel *Lc = h->L[k];
   h->nc = Lc->n;
-  if (k > 0 && Lc->n <= 1024) {  // else (coarsening stalled) smoothing only
+  if (packed) {
+    if (k == 0) return fail(c, APC_ERR_UNSUPPORTED, "amg: the packed boxes did not coarsen");
     KScope ks1(c, "amg_setup_dense");
-    r = dense_inverse(c, h, Lc, periodic, box0 ? (const int *)Lc->box.p : nullptr);
+    r = block_inverse(c, h, Lc, periodic, start);
+    if (r) return r;
+    h->blocks = true;
+    h->dense = true;
+  } else if (k > 0 && Lc->n <= 1024) {  // else (coarsening stalled) smoothing only
+    KScope ks1(c, "amg_setup_dense");
+    r = dense_inverse(c, h, Lc, periodic);
     if (r) return r;
     h->dense = true;
   }
@@ -1431,7 +1669,11 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
     }
   };
   if (l == h->nlev - 1) {
-    if (h->dense) {
+    if (h->blocks) {
+      KScope ks(c, "amg_dense");
+      hipLaunchKernelGGL((k_cy_dense_blocks<T>), dim3(n), dim3(64), 0, c->s, L->box.p, h->bstart.p,
+                         h->ioff.p, cinv_of<T>(h), b, x, st);
+    } else if (h->dense) {
       KScope ks(c, "amg_dense");
       hipLaunchKernelGGL((k_cy_dense<T>), dim3(n), dim3(64), 0, c->s, n, cinv_of<T>(h), b, x, st);
     } else {  // no dense solve: smoothing only (odd sweep count ends in x)
@@ -1500,8 +1742,11 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
     {
       KScope ks(c, ell ? (f32 ? "amg_prolong0_32" : "amg_prolong0") : kProlongName[l],
                 ell ? (double)n * (4 + 3 * sT) : 0.0);
+      // packed boxes: a stopped box's rows take the first coarse correction
+      // unscaled and none after it (its coarsest solve, passed through once)
+      const unsigned char *pass = (h->nbox > 1 && l > 0) ? L->pass.p : nullptr;
       hipLaunchKernelGGL((k_cy_prolong<T>), dim3(g), dim3(256), 0, c->s, n, L->agg.p, Pc.x, cur,
-                         o.alpha, st);
+                         o.alpha, pass, q == 0 ? 1.0 : 0.0, st);
     }
   }
   for (int s = 0; s < o.nu; ++s) {

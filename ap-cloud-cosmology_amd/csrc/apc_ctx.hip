@@ -208,10 +208,28 @@ int apc_pack_set_masses(apc_ctx *c, const double *m) {
 
 int apc_pack_rows(apc_ctx *c, int *row_start, int *comp_rows) {
   if (!c) return APC_ERR_ARG;
-  if (!c->have_comp) return fail(c, APC_ERR_ARG, "pack_rows: no computational particles");
+  if ((int)c->rstart_h.size() != c->nbox + 1 || (int)c->rnc_h.size() != c->nbox)
+    return fail(c, APC_ERR_ARG, "pack_rows: no selection yet");
   for (int b = 0; b <= c->nbox; ++b) {
     if (row_start) row_start[b] = c->rstart_h[b];
     if (comp_rows && b < c->nbox) comp_rows[b] = c->rnc_h[b];
+  }
+  return APC_OK;
+}
+
+int apc_pack_solve_info(apc_ctx *c, int *its_rho, int *its_phi, int *reason_rho,
+                        int *reason_phi, double *res_rho, double *res_phi) {
+  if (!c) return APC_ERR_ARG;
+  if (c->nbox < 2) return fail(c, APC_ERR_ARG, "pack_solve_info: not a packed context");
+  if ((int)c->box_its[0].size() != c->nbox || (int)c->box_its[1].size() != c->nbox)
+    return fail(c, APC_ERR_ARG, "pack_solve_info: no solve yet");
+  for (int b = 0; b < c->nbox; ++b) {
+    if (its_rho) its_rho[b] = c->box_its[0][b];
+    if (its_phi) its_phi[b] = c->box_its[1][b];
+    if (reason_rho) reason_rho[b] = c->box_reason[0][b];
+    if (reason_phi) reason_phi[b] = c->box_reason[1][b];
+    if (res_rho) res_rho[b] = c->box_res[0][b];
+    if (res_phi) res_phi[b] = c->box_res[1][b];
   }
   return APC_OK;
 }
@@ -245,8 +263,7 @@ int apc_destroy(apc_ctx *c) {
   c->visits.release(c->s); c->pstart.release(c->s); c->rstart.release(c->s);
   c->rowbox.release(c->s); c->boxmass.release(c->s); c->pk_key.release(c->s);
   c->pk_key2.release(c->s); c->pk_val.release(c->s); c->pk_val2.release(c->s);
-  c->pk_leaf.release(c->s); c->pk_type.release(c->s); c->pk_b.release(c->s);
-  c->pk_sig.release(c->s); c->fitq_rows.release(c->s); c->fitq_stat.release(c->s);
+  c->pk_leaf.release(c->s); c->pk_type.release(c->s); c->fitq_rows.release(c->s); c->fitq_stat.release(c->s);
   c->fit_cond.release(c->s); c->scodes.release(c->s); c->slen.release(c->s); c->sidx.release(c->s);
   c->tr_key.release(c->s); c->tr_key2.release(c->s); c->cic_key.release(c->s); c->cic_key2.release(c->s);
   c->vcode.release(c->s); c->vcode2.release(c->s);

@@ -31,42 +31,6 @@
 
 namespace apc {
 
-enum { KSP_CONVERGED_RTOL = 2, KSP_CONVERGED_ATOL = 3, KSP_DIVERGED_NULL = -2,
-       KSP_DIVERGED_ITS = -3, KSP_DIVERGED_DTOL = -4, KSP_DIVERGED_BREAKDOWN = -5,
-       KSP_DIVERGED_NANORINF = -9 };
-
-struct GDev {
-  GState *st;
-  double *HH, *cc, *ss, *grs, *sig, *h, *coef, *part, *hist;
-};
-
-struct SolveState {
-  DBuf<double> V64, dscratch, r64, part, z64, u64;
-  DBuf<float> V32, s32, d32, f32, dinv32, z32, u32, Z32;
-  DBuf<double> hess;  // HH, cc, ss, grs, sig, h, coef, hist
-  DBuf<GState> st;
-  GState *hst = nullptr;  // pinned mirror
-  AmgHier *amg = nullptr;
-  int mcap = 0, histcap = 0;
-  ~SolveState() {  // the owner has drained its stream (apc_destroy)
-    V64.release(); dscratch.release(); r64.release(); part.release(); z64.release();
-    u64.release(); V32.release(); s32.release(); d32.release(); f32.release();
-    dinv32.release(); z32.release(); u32.release(); Z32.release(); hess.release(); st.release();
-    if (hst) hipHostFree(hst);
-    amg_free(amg);
-  }
-};
-
-__device__ inline int converged(GState *st, int nit, double rn) {
-  if (nit == 0) {
-    st->rnorm0 = rn;
-    st->ttol = fmax(st->rtol * rn, st->atol);
-  }
-  if (isnan(rn) || isinf(rn)) return KSP_DIVERGED_NANORINF;
-  if (rn <= st->ttol) return (rn < st->atol) ? KSP_CONVERGED_ATOL : KSP_CONVERGED_RTOL;
-  if (rn >= st->dtol * st->rnorm0) return KSP_DIVERGED_DTOL;
-  return 0;
-}
 
 // Fixed-order sum of nb partials by the calling block; result to all threads.
 __device__ inline double sum_partials(const double *part, int nb, double *ws) {
@@ -464,31 +428,6 @@ __global__ void __launch_bounds__(256) k_rhs2_boxes(const int *__restrict__ rsta
   for (int i = lo + threadIdx.x; i < hi; i += 256) rhs2[i] = (rho[i] - avg) * c_pot;
 }
 
-// sig[b] = || w b ||_2 over box b's rows (w = 1 if null); 1 for a zero box.
-__global__ void __launch_bounds__(256) k_box_norms(const int *__restrict__ rstart,
-                                                   const double *__restrict__ v,
-                                                   const double *__restrict__ w,
-                                                   double *__restrict__ sig) {
-  __shared__ double ws[4];
-  const int lo = rstart[blockIdx.x], hi = rstart[blockIdx.x + 1];
-  double a = 0.0;
-  for (int i = lo + threadIdx.x; i < hi; i += 256) {
-    const double e = w ? w[i] * v[i] : v[i];
-    a += e * e;
-  }
-  a = block_sum256(a, ws);
-  if (threadIdx.x == 0) sig[blockIdx.x] = (a > 0.0) ? sqrt(a) : 1.0;
-}
-
-// out = in / sig[box] (inverse = 0) or in * sig[box] (inverse = 1)
-__global__ void k_box_scale(int M, const int *__restrict__ rowbox, const double *__restrict__ sig,
-                            const double *__restrict__ in, double *__restrict__ out, int inverse) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= M) return;
-  const double q = sig[rowbox[i]];
-  out[i] = inverse ? in[i] * q : in[i] / q;
-}
-
 __global__ void k_rhs2_bounded(int M, const double *__restrict__ rho, const double *__restrict__ pbnd,
                                double c_pot, double *__restrict__ rhs2) {
   int i = blockIdx.x * 256 + threadIdx.x;
@@ -546,26 +485,6 @@ static GDev gdev_of(SolveState *S, int m) {
   g.hist = p;
   g.part = S->part.p;
   return g;
-}
-
-// Arnoldi steps to launch before the host next reads the solve's state.
-// Between reads the launches of a solve that has converged return at once (a
-// few microseconds each, but a multigrid cycle is tens of launches), so the
-// next read is placed where the residual, extrapolated at the rate observed
-// in this restart cycle, reaches the target (at most 16 steps ahead).  With
-// kernel timing on the state is read after every step, so that no timed
-// launch is one that returned at once.
-static int steps_to_next_read(const GState &h, bool timing) {
-  constexpr int kMax = 16;
-  if (timing) return 1;
-  double target = h.ttol;
-  if (h.fp32) target = std::max(target, 2e-6 * h.cyc_res0);  // the FP32 cycle's floor
-  if (h.it < 2 || !(h.res > 0.0) || !(h.cyc_res0 > h.res)) return kMax;
-  if (h.res <= target) return 1;
-  const double rate = std::pow(h.res / h.cyc_res0, 1.0 / h.it);
-  const double need = std::log(target / h.res) / std::log(rate);
-  if (!(need < kMax)) return kMax;
-  return std::max(1, (int)std::ceil(need));
 }
 
 static int read_state(apc_ctx *c, SolveState *S) {
@@ -744,6 +663,7 @@ static AmgOptions amg_options() {
   a.wlevels = env_int("APC_AMG_WLEVELS", a.wlevels);
   a.coarse_max = env_int("APC_AMG_COARSE", a.coarse_max);
   a.galerkin_sorted = env_int("APC_AMG_GALERKIN_SORTED", 0);
+  if (const char *v = getenv("APC_AMG_SHIFT")) a.shift = atoi(v);
   a.g8_rows = env_int("APC_AMG_G8", a.g8_rows);
   a.g4_rows = env_int("APC_AMG_G4", a.g4_rows);
   a.rg8_rows = env_int("APC_AMG_RG8", a.rg8_rows);
@@ -757,7 +677,7 @@ static AmgOptions amg_options() {
 // pinned: A is the periodic Poisson operator with its identity row.
 static int gmres(apc_ctx *c, const Op<double> &A, const double *b, double *x,
                  const apc_ksp_opts &o, bool pinned, int *its, int *reason, double *rnorm,
-                 int *sweeps) {
+                 int *sweeps, bool seg = false) {
   const int M = A.M;
   const int m = std::min(std::max(1, o.restart), MAX_RESTART);
   const bool fp32 = o.precision == 1;
@@ -778,6 +698,9 @@ static int gmres(apc_ctx *c, const Op<double> &A, const double *b, double *x,
       r = amg_setup(c, &S->amg, A, pinned, false, amg_options());
       if (r) return r;
     }
+    if (seg)
+      return run_cycles_seg<double>(c, A, A, S->V64.p, b, x, o, false, amg, S->z64.p, S->u64.p,
+                                    its, reason, rnorm, sweeps, nullptr, nullptr, nullptr, nullptr);
     return run_cycles<double>(c, A, A, S->V64.p, b, x, o, false, amg, S->z64.p, S->u64.p, its,
                               reason, rnorm, sweeps);
   }
@@ -799,39 +722,32 @@ static int gmres(apc_ctx *c, const Op<double> &A, const double *b, double *x,
     if (r) return r;
     APC_ALLOC(c, S->V64, (size_t)(m + 1) * M + 1);
     APC_ALLOC(c, S->Z32, (size_t)(m + 1) * M + 1);
+    if (seg)
+      return run_cycles_seg<double>(c, A, A, S->V64.p, b, x, o, false, true, S->z64.p, S->u64.p,
+                                    its, reason, rnorm, sweeps, &A32, S->Z32.p, S->z32.p,
+                                    S->u32.p);
     return run_cycles<double>(c, A, A, S->V64.p, b, x, o, false, true, S->z64.p, S->u64.p, its,
                               reason, rnorm, sweeps, &A32, S->Z32.p, S->z32.p, S->u32.p);
   }
+  if (seg)
+    return run_cycles_seg<float>(c, A, A32, S->V32.p, b, x, o, true, false, S->z32.p, S->u32.p,
+                                 its, reason, rnorm, sweeps, nullptr, nullptr, nullptr, nullptr);
   return run_cycles<float>(c, A, A32, S->V32.p, b, x, o, true, false, S->z32.p, S->u32.p, its,
                            reason, rnorm, sweeps);
 }
 
-// gmres over the rows of packed boxes.  The boxes' blocks are decoupled, so
-// scaling box b's right-hand side by 1/sig_b (zero initial guess) scales its
-// solution by the same factor.  With sig_b the norm of what the convergence
-// test measures (the left-Jacobi preconditioned right-hand side, or the
-// right-hand side itself under the right-preconditioned multigrid), every
-// box enters with norm 1, ||b|| = sqrt(nbox), and the global test ||r|| <=
-// (rtol / sqrt(nbox)) ||b|| bounds every box's own relative residual
-// ||r_b|| / ||b_b|| by rtol.  The solution is scaled back at the end.
-static int gmres_boxes(apc_ctx *c, const Op<double> &A, const double *b, double *x,
+// gmres over the rows of packed boxes: one Krylov space and one convergence
+// test per box (run_cycles_seg), so that every box's solve stops where a
+// context of that box alone stops.
+static int gmres_boxes(apc_ctx *c, int which, const Op<double> &A, const double *b, double *x,
                        const apc_ksp_opts &o, bool pinned, int *its, int *reason, double *rnorm,
                        int *sweeps) {
   if (c->nbox < 2) return gmres(c, A, b, x, o, pinned, its, reason, rnorm, sweeps);
-  const int M = A.M, nbox = c->nbox;
-  APC_ALLOC(c, c->pk_b, M + 1);
-  APC_ALLOC(c, c->pk_sig, nbox + 1);
-  hipLaunchKernelGGL(k_box_norms, dim3(nbox), dim3(256), 0, c->s, c->rstart.p, b,
-                     o.pc_type == 1 ? (const double *)nullptr : A.dinv, c->pk_sig.p);
-  hipLaunchKernelGGL(k_box_scale, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, c->rowbox.p,
-                     c->pk_sig.p, b, c->pk_b.p, 0);
-  apc_ksp_opts ob = o;
-  ob.rtol = o.rtol / sqrt((double)nbox);
-  int r = gmres(c, A, c->pk_b.p, x, ob, pinned, its, reason, rnorm, sweeps);
+  int r = gmres(c, A, b, x, o, pinned, its, reason, rnorm, sweeps, true);
   if (r) return r;
-  hipLaunchKernelGGL(k_box_scale, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, c->rowbox.p,
-                     c->pk_sig.p, x, x, 1);
-  APC_HIP(c, hipGetLastError());
+  c->box_its[which] = c->seg_its;
+  c->box_reason[which] = c->seg_reason;
+  c->box_res[which] = c->seg_res;
   return 0;
 }
 
@@ -850,7 +766,7 @@ int solve_density_potential(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_op
   Op<double> A2{c->nbr.p, c->sco.p, c->d2.p, c->f2.p, c->dinv2.p, M, c->prm.lm};
   apc_solve_info inf;
   memset(&inf, 0, sizeof(inf));
-  int r = gmres_boxes(c, A1, c->rhs1.p, c->rho.p, *o1, false, &inf.its_rho, &inf.reason_rho,
+  int r = gmres_boxes(c, 0, A1, c->rhs1.p, c->rho.p, *o1, false, &inf.its_rho, &inf.reason_rho,
                       &inf.rnorm_rho, &inf.refine_rho);
   if (r) return r;
   // right-hand side of the Poisson solve (solver.cpp:233-261)
@@ -868,7 +784,7 @@ int solve_density_potential(apc_ctx *c, const apc_ksp_opts *o1, const apc_ksp_op
     hipLaunchKernelGGL(k_rhs2_bounded, dim3(nblocks(M, 256)), dim3(256), 0, c->s, M, c->rho.p,
                        c->pbnd.p, c->c_pot, c->rhs2.p);
   }
-  r = gmres_boxes(c, A2, c->rhs2.p, c->phi.p, *o2, c->prm.bc == 1, &inf.its_phi, &inf.reason_phi,
+  r = gmres_boxes(c, 1, A2, c->rhs2.p, c->phi.p, *o2, c->prm.bc == 1, &inf.its_phi, &inf.reason_phi,
                   &inf.rnorm_phi, &inf.refine_phi);
   if (r) return r;
   inf.nc = c->nc;

@@ -76,7 +76,11 @@ struct apc_ctx {
   apc::DBuf<double> boxmass;            // per box: factor of the density constant
   apc::DBuf<uint64_t> pk_key, pk_key2;  // packed selection: rows sorted by box
   apc::DBuf<int> pk_val, pk_val2, pk_leaf, pk_type;
-  apc::DBuf<double> pk_b, pk_sig;       // packed solves: scaled right-hand side, box scales
+  // packed solves: per box, the last segmented solve's iterations, reasons
+  // and final residuals (seg_*), and those of the density [0] and
+  // potential [1] solves of the last apc_solver_solve
+  std::vector<int> seg_its, seg_reason, box_its[2], box_reason[2];
+  std::vector<double> seg_res, box_res[2];
   hipStream_t s = nullptr;
   int dev = 0;
   std::string err;

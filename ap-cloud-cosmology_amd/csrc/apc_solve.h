@@ -1,6 +1,9 @@
 // Types shared by the Krylov solver (apc_gmres.hip) and the multigrid
 // preconditioner (apc_amg.hip).
 #pragma once
+#include <algorithm>
+#include <cmath>
+
 #include "apc_internal.h"
 #include "apc_prims.h"
 
@@ -30,6 +33,28 @@ struct GState {
 };
 
 __device__ inline bool halted(const GState *st) { return st->reason != 0 || st->cycle_done != 0; }
+
+enum { KSP_CONVERGED_RTOL = 2, KSP_CONVERGED_ATOL = 3, KSP_DIVERGED_NULL = -2,
+       KSP_DIVERGED_ITS = -3, KSP_DIVERGED_DTOL = -4, KSP_DIVERGED_BREAKDOWN = -5,
+       KSP_DIVERGED_NANORINF = -9 };
+
+// KSPConvergedDefault: the first residual sets the target rtol * r0 (atol
+// below it); NaN/Inf and growth beyond dtol * r0 diverge.
+__device__ inline int converged(GState *st, int nit, double rn) {
+  if (nit == 0) {
+    st->rnorm0 = rn;
+    st->ttol = fmax(st->rtol * rn, st->atol);
+  }
+  if (isnan(rn) || isinf(rn)) return KSP_DIVERGED_NANORINF;
+  if (rn <= st->ttol) return (rn < st->atol) ? KSP_CONVERGED_ATOL : KSP_CONVERGED_RTOL;
+  if (rn >= st->dtol * st->rnorm0) return KSP_DIVERGED_DTOL;
+  return 0;
+}
+
+struct GDev {
+  GState *st;
+  double *HH, *cc, *ss, *grs, *sig, *h, *coef, *part, *hist;
+};
 
 // Sum over the row's neighbour slots in slot order.  The slots are taken in
 // groups of RG: the group's index and weight loads are issued together, then
@@ -101,6 +126,7 @@ struct AmgOptions {
   int coarse_max = 256;   // dense solve at or below this many rows
   int max_levels = 12;
   int galerkin_sorted = 0;  // form every coarse operator by the global sort (tests)
+  int shift = 1;            // shift the constant mode(s) of the periodic coarsest matrix
   // coarse levels with at most g8_rows / g4_rows rows run their sweeps and
   // residuals with 8 / 4 lanes per row; restrictions into at most rg8_rows
   // coarse rows use 8 lanes per coarse row.  The coarse levels are
@@ -120,6 +146,66 @@ int amg_setup(apc_ctx *c, AmgHier **h, const Op<double> &A, bool periodic, bool 
 template <class T>
 int amg_apply(apc_ctx *c, AmgHier *h, const Op<T> &A, const T *in, T *out, const GState *st);
 void amg_free(AmgHier *h);
+
+// Buffers of the Krylov solves, kept with the context between steps.
+struct SolveState {
+  DBuf<double> V64, dscratch, r64, part, z64, u64;
+  DBuf<float> V32, s32, d32, f32, dinv32, z32, u32, Z32;
+  DBuf<double> hess;  // HH, cc, ss, grs, sig, h, coef, hist
+  DBuf<GState> st;
+  GState *hst = nullptr;  // pinned mirror
+  AmgHier *amg = nullptr;
+  int mcap = 0, histcap = 0;
+  // segmented solves (packed boxes): per-box Hessenberg data and states, the
+  // tile table, and the pinned mirror of the states (nbox + 1: the summary)
+  DBuf<double> shess, spart;
+  DBuf<GState> sst;
+  DBuf<int> stile;
+  GState *hsst = nullptr;
+  int hsst_cap = 0;
+  ~SolveState() {  // the owner has drained its stream (apc_destroy)
+    V64.release(); dscratch.release(); r64.release(); part.release(); z64.release();
+    u64.release(); V32.release(); s32.release(); d32.release(); f32.release();
+    dinv32.release(); z32.release(); u32.release(); Z32.release(); hess.release(); st.release();
+    shess.release(); spart.release(); sst.release(); stile.release();
+    if (hst) hipHostFree(hst);
+    if (hsst) hipHostFree(hsst);
+    amg_free(amg);
+  }
+};
+
+// Arnoldi steps to launch before the host next reads the solve's state.
+// Between reads the launches of a solve that has converged return at once (a
+// few microseconds each, but a multigrid cycle is tens of launches), so the
+// next read is placed where the residual, extrapolated at the rate observed
+// in this restart cycle, reaches the target (at most 16 steps ahead).  With
+// kernel timing on the state is read after every step, so that no timed
+// launch is one that returned at once.
+inline int steps_to_next_read(const GState &h, bool timing) {
+  constexpr int kMax = 16;
+  if (timing) return 1;
+  double target = h.ttol;
+  if (h.fp32) target = std::max(target, 2e-6 * h.cyc_res0);  // the FP32 cycle's floor
+  if (h.it < 2 || !(h.res > 0.0) || !(h.cyc_res0 > h.res)) return kMax;
+  if (h.res <= target) return 1;
+  const double rate = std::pow(h.res / h.cyc_res0, 1.0 / h.it);
+  const double need = std::log(target / h.res) / std::log(rate);
+  if (!(need < kMax)) return kMax;
+  return std::max(1, (int)std::ceil(need));
+}
+
+// GMRES cycles over packed boxes (apc_gmres_seg.hip): one Krylov space, one
+// restart cycle and one convergence test per box (each box's solve follows
+// the single-box GMRES of apc_gmres.hip), every box in the same launches.
+// Arguments as the single-box cycles; per-box iterations and reasons go to
+// its_box / reason_box (nbox entries, may be null); *its is the largest
+// count, *reason the first failing box's reason or else the smallest
+// positive one, *rnorm the largest final residual.
+template <class T>
+int run_cycles_seg(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, const double *b,
+                   double *x, const apc_ksp_opts &o, bool fp32, bool amg, T *zb, T *ub, int *its,
+                   int *reason, double *rnorm, int *sweeps, const Op<float> *A32, float *Z32,
+                   float *v32, float *z32);
 int amg_levels(const AmgHier *h, int *rows, long long *nnz, int cap);
 // Level l >= 1 as CSR (rp n+1, col/val nnz) and the aggregate map of level l
 // (n entries, not for the coarsest); any pointer may be null.

@@ -130,6 +130,9 @@ def main():
     ap.add_argument("--max-it", type=int, default=1000)
     ap.add_argument("--precision", type=int, default=1,
                     help="measured form: 1 FP32 GMRES + FP64 refinement (default), 0 FP64")
+    ap.add_argument("--no-multibox", action="store_true", help="skip the packed multi-box leg")
+    ap.add_argument("--multibox-boxes", type=int, default=48)
+    ap.add_argument("--multibox-steps", type=int, default=20)
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -358,10 +361,118 @@ def main():
                           for k, v in kt.items()}
     ctx.close()
 
+    if not args.no_multibox:
+        out["multibox"] = multibox_leg(args)
     if not args.no_cpu:
         log("CPU baseline leg")
         out["cpu_baseline"] = cpu_baseline(args, w)
     print(json.dumps(out))
+
+
+def multibox_leg(args):
+    """The multi-box form: --multibox-boxes independent periodic boxes (EdS
+    Zel'dovich lattices, sides drawn from {16,24,32,48,64,96}, spectral
+    index from [-2.5,-1.5], RMS 0.01 box, seed 3) packed into one context
+    (apc_create_packed), so that every kernel of a step runs over all boxes
+    in one launch.  --multibox-steps steps of a=0.02..0.1 in the measured
+    precision (FP32 multigrid in FP64 GMRES for the Poisson solve) after one
+    warmup step, from the state after the half drift; every box's solve meets
+    rtol on its own residual.  The first step is also run in FP64 from the
+    same state for the force error.  Beside it: the same boxes as separate
+    contexts, one after another, for two steps (the form the packing
+    replaces)."""
+    import apcloud
+    from apcloud import ic
+
+    specs = ic.multibox_specs(nbox=args.multibox_boxes, seed=3)
+    boxes = [ic.multibox_box(s) for s in specs]
+    counts = np.array([len(b[0]) for b in boxes], np.int32)
+    N = int(counts.sum())
+    log("multibox leg: %d boxes, %d particles" % (len(boxes), N))
+    params = apcloud.make_params(bbox=(0, 1, 0, 1, 0, 1), depth=20, bc=1, c=0.0, lm=27,
+                                 leaf_capacity=8)
+    P = apcloud.PackedContext(params, counts)
+    P.set_particles(*[np.concatenate([b[q] for b in boxes]) for q in range(6)])
+    P.set_masses(1.0 / counts)
+    nsteps = args.multibox_steps
+    a_start, a_end = 0.02, 0.1
+    da = (a_end - a_start) / nsteps
+    sched = schedule(a_start, a_end, da)
+    _, _, cx0, _ = eds(a_start)
+    P.drift(cx0, 0.5 * da)
+    P.checkpoint(0)
+
+    def opts(precision):
+        rho = apcloud.ksp(rtol=args.rtol, restart=args.restart, max_it=args.max_it,
+                          precision=precision, pc="jacobi")
+        phi = apcloud.ksp(rtol=args.rtol, restart=args.restart, max_it=args.max_it,
+                          precision=precision, pc=args.pc)
+        return rho, phi
+
+    def step(ctx, j, o, cd=1.0):
+        t, dt = sched[j]
+        cpot, cf, _, cv = eds(t)
+        _, _, cxh, _ = eds(t + dt / 2)
+        return ctx.step(cd, cpot, cf, cv, cxh, dt, o[0], o[1])[2]
+
+    # force error of the first step: FP64 against the measured precision
+    step(P, 0, opts(0))
+    ref = P.particles()
+    P.checkpoint(0, restore=True)
+    step(P, 0, opts(args.precision))
+    got = P.particles()
+    d = np.stack([got[k] - ref[k] for k in (6, 7, 8)])
+    err = float(np.sqrt((d ** 2).sum() / (np.stack(ref[6:9]) ** 2).sum()))
+    P.checkpoint(0, restore=True)
+    o = opts(args.precision)
+    step(P, 0, o)  # warmup
+    P.checkpoint(0, restore=True)
+    P.sync()
+    its = []
+    t0 = time.perf_counter()
+    for j in range(len(sched)):
+        info = step(P, j, o)
+        its.append((info.its_rho, info.its_phi))
+        if info.reason_rho <= 0 or info.reason_phi <= 0:
+            raise RuntimeError("bench: multibox step %d did not converge: %r" % (j, its[-1]))
+    P.sync()
+    wall = time.perf_counter() - t0
+    rows = int(P.rows()[0][-1])
+    its_box = P.solve_info()[1].tolist()  # the last step's Poisson iterations per box
+    P.close()
+    # the same boxes one context each, stepped one after another
+    log("multibox leg: separate contexts")
+    ctxs = []
+    for b in boxes:
+        g = apcloud.Context(params)
+        g.set_particles(*b)
+        g.drift(cx0, 0.5 * da)
+        ctxs.append(g)
+    for g, n in zip(ctxs, counts):
+        step(g, 0, o, 1.0 / n)  # warmup (first-use allocations)
+    for g in ctxs:
+        g.sync()
+    t0 = time.perf_counter()
+    for j in (1, 2):
+        for g, n in zip(ctxs, counts):
+            step(g, j, o, 1.0 / n)
+    for g in ctxs:
+        g.sync()
+    wall_sep = (time.perf_counter() - t0) / 2
+    for g in ctxs:
+        g.close()
+    value = N * len(sched) / wall
+    return {"value": round(value, 1), "unit": "particle-steps/s",
+            "ms_per_step": round(1e3 * wall / len(sched), 2), "boxes": len(boxes),
+            "particles": N, "rows": rows, "sides": [s["n_side"] for s in specs],
+            "steps": len(sched), "gmres_its": its, "last_step_poisson_its_per_box": its_box,
+            "first_step_rel_rms_force_err_vs_fp64": err,
+            "separate_contexts_ms_per_step": round(1e3 * wall_sep, 2),
+            "speedup_vs_separate_contexts": round(1e3 * wall_sep / (1e3 * wall / len(sched)), 3),
+            "note": "one packed context (apc_create_packed): each kernel of a step over all "
+                    "boxes in one launch; %d steps a=0.02..0.1 after one warmup step, every "
+                    "box's solve to rtol %g on its own residual; separate contexts: the same "
+                    "boxes one context each, stepped in turn (2 steps)" % (len(sched), args.rtol)}
 
 
 def schedule(a_start, a_end, da):

@@ -110,15 +110,23 @@ int apc_destroy(apc_ctx *ctx);
  * apc_pack_rows), its computational nodes first, each box in the order a
  * context of that box alone gives.  Every box's potential system pins its
  * first row; the density right-hand side of box b carries mass[b]
- * (apc_pack_set_masses, default 1) times c_dens.  Each GMRES solve scales
- * every box to unit right-hand side and divides rtol by sqrt(nbox), so that
- * every box meets rtol on its own.  bc must be 1 (APC_ERR_UNSUPPORTED
- * otherwise); apc_set_comp and apc_freeze_topology refuse a packed context. */
+ * (apc_pack_set_masses, default 1) times c_dens.  Each GMRES solve keeps one
+ * Krylov space, restart cycle and convergence test per box, every box in
+ * the same launches; the solve info reports the largest iteration count.
+ * bc must be 1 (APC_ERR_UNSUPPORTED otherwise); apc_set_comp and
+ * apc_freeze_topology refuse a packed context. */
 int apc_create_packed(const apc_params *p, int nbox, const int *counts, apc_ctx **out);
 int apc_pack_set_masses(apc_ctx *ctx, const double *mass);
-/* row_start: nbox + 1 entries, comp_rows: nbox entries (either may be NULL),
- * valid after apc_select. */
+/* Rows of the last selection (apc_select or apc_step): row_start nbox + 1
+ * entries, comp_rows nbox entries (either may be NULL). */
 int apc_pack_rows(apc_ctx *ctx, int *row_start, int *comp_rows);
+/* Per box, the density and potential solves of the last apc_solver_solve /
+ * apc_step on a packed context: iterations, KSP reasons and final residual
+ * norms (nbox entries each; NULL pointers are skipped).  Each box's solve has
+ * its own Krylov space and convergence test, so these are what a context of
+ * that box alone reports (up to the summation order of the reductions). */
+int apc_pack_solve_info(apc_ctx *ctx, int *its_rho, int *its_phi, int *reason_rho,
+                        int *reason_phi, double *res_rho, double *res_phi);
 
 /* initialization.cpp:41-91 hands the path a Physical_particle; this copies
  * the caller's SoA arrays (particle.h:5-21) to HBM. */
