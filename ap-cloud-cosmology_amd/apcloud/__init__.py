@@ -38,8 +38,12 @@ EXPORTS = [
     "apc_batch_last_error", "apc_batch_set_particles", "apc_batch_run",
     "apc_morton_sort_noalloc", "apc_set_comp", "apc_solver_set_fit_threshold",
     "apc_fit_conditions", "apc_create_packed", "apc_pack_set_masses", "apc_pack_rows",
-    "apc_pack_solve_info",
+    "apc_pack_solve_info", "apc_set_knob", "apc_get_knob",
 ]
+
+# launch-shape knobs (APC_KNOB_* in include/apcloud.h), in enum order
+KNOBS = ["search_chunk", "mdot_blocks", "maxpy_blocks", "seg_blocks", "amg_g8_rows",
+         "amg_g4_rows", "amg_rg8_rows"]
 
 
 class Params(C.Structure):
@@ -93,6 +97,8 @@ def lib():
         L.apc_pack_set_masses.argtypes = [C.c_void_p, _D]
         L.apc_pack_rows.argtypes = [C.c_void_p, _I, _I]
         L.apc_pack_solve_info.argtypes = [C.c_void_p, _I, _I, _I, _I, _D, _D]
+        L.apc_set_knob.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.apc_get_knob.argtypes = [C.c_void_p, C.c_int, _I]
         L.apc_set_particles.argtypes = [C.c_void_p, C.c_int] + [_D] * 6
         L.apc_get_particles.argtypes = [C.c_void_p] + [_D] * 9
         L.apc_octree_build.argtypes = [C.c_void_p]
@@ -542,6 +548,17 @@ class Context:
         self._chk(self.L.apc_get_comp_sizes(self.h, C.byref(nc), C.byref(nv), None, None),
                   "apc_get_comp_sizes")
         return nc.value + nv.value
+
+    def set_knob(self, name, value):
+        self._chk(self.L.apc_set_knob(self.h, KNOBS.index(name), int(value)), "apc_set_knob")
+
+    def knobs(self):
+        out = {}
+        for k, name in enumerate(KNOBS):
+            v = C.c_int()
+            self._chk(self.L.apc_get_knob(self.h, k, C.byref(v)), "apc_get_knob")
+            out[name] = v.value
+        return out
 
     def set_timing(self, on=True):
         self._chk(self.L.apc_set_timing(self.h, 1 if on else 0), "apc_set_timing")

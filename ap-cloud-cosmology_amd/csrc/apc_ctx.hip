@@ -143,6 +143,13 @@ int apc_create(const apc_params *p, apc_ctx **out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return APC_ERR_NO_DEVICE;
   apc_ctx *c = new apc_ctx();
   c->prm = *p;
+  // APC_* environment overrides of the launch-shape knobs (hand sweeps)
+  static const char *const knob_env[APC_KNOB_COUNT] = {
+      "APC_NEIGH_XCHUNK", "APC_MDOT_BLOCKS", "APC_MAXPY_BLOCKS", "APC_SEG_BLOCKS",
+      "APC_AMG_G8",       "APC_AMG_G4",      "APC_AMG_RG8"};
+  for (int k = 0; k < APC_KNOB_COUNT; ++k)
+    if (const char *v = getenv(knob_env[k]))
+      if (atoi(v) >= 1) c->knob[k] = atoi(v);
   c->depth_req = p->depth;
   c->maxdepth_req = p->maxdepth;
   hipGetDevice(&c->dev);
@@ -214,6 +221,21 @@ int apc_pack_rows(apc_ctx *c, int *row_start, int *comp_rows) {
     if (row_start) row_start[b] = c->rstart_h[b];
     if (comp_rows && b < c->nbox) comp_rows[b] = c->rnc_h[b];
   }
+  return APC_OK;
+}
+
+int apc_set_knob(apc_ctx *c, int knob, int value) {
+  if (!c) return APC_ERR_ARG;
+  if (knob < 0 || knob >= APC_KNOB_COUNT || value < 1)
+    return fail(c, APC_ERR_ARG, "set_knob: knob %d value %d", knob, value);
+  c->knob[knob] = value;
+  return APC_OK;
+}
+
+int apc_get_knob(apc_ctx *c, int knob, int *value) {
+  if (!c || !value) return APC_ERR_ARG;
+  if (knob < 0 || knob >= APC_KNOB_COUNT) return fail(c, APC_ERR_ARG, "get_knob: knob %d", knob);
+  *value = c->knob[knob];
   return APC_OK;
 }
 

@@ -448,8 +448,6 @@ struct Cfg<double> { static constexpr int MAXPY_U = 4; };
 template <>
 struct Cfg<float> { static constexpr int MAXPY_U = 8; };
 constexpr int MDOT_R = 4, MDOT_U = 2;
-constexpr int MDOT_BLOCKS = 1024;
-constexpr int MAXPY_BLOCKS = 1024;
 
 static int env_int(const char *name, int dflt) {
   const char *v = getenv(name);
@@ -527,9 +525,9 @@ static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, 
   const size_t ld = (size_t)M;
   const unsigned gb = nblocks(M, 256);
   const long ntile = ((long)M + 256 * MDOT_R - 1) / (256 * MDOT_R);
-  // APC_MDOT_BLOCKS / APC_MAXPY_BLOCKS override the grid sizes (tuning runs)
-  static const int mdot_blocks = env_int("APC_MDOT_BLOCKS", MDOT_BLOCKS);
-  static const int maxpy_blocks = env_int("APC_MAXPY_BLOCKS", MAXPY_BLOCKS);
+  // grid sizes: the context's knobs (APC_KNOB_MDOT_BLOCKS / _MAXPY_BLOCKS)
+  const int mdot_blocks = c->knob[APC_KNOB_MDOT_BLOCKS];
+  const int maxpy_blocks = c->knob[APC_KNOB_MAXPY_BLOCKS];
   const unsigned gdot = (unsigned)std::min<long>(mdot_blocks, ntile);
   const unsigned gaxpy = (unsigned)std::min<long>(maxpy_blocks, gb);
   const unsigned gres = (unsigned)std::min<long>(RED_BLOCKS, gb);
@@ -655,8 +653,9 @@ static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, 
   return 0;
 }
 
-static AmgOptions amg_options() {
-  // APC_AMG_* override the multigrid settings (tuning runs)
+static AmgOptions amg_options(apc_ctx *c) {
+  // APC_AMG_* override the multigrid settings (tuning runs); the lanes per
+  // row of the coarse levels are the context's knobs
   AmgOptions a;
   a.nu = env_int("APC_AMG_NU", a.nu);
   a.gamma = env_int("APC_AMG_GAMMA", a.gamma);
@@ -664,9 +663,9 @@ static AmgOptions amg_options() {
   a.coarse_max = env_int("APC_AMG_COARSE", a.coarse_max);
   a.galerkin_sorted = env_int("APC_AMG_GALERKIN_SORTED", 0);
   if (const char *v = getenv("APC_AMG_SHIFT")) a.shift = atoi(v);
-  a.g8_rows = env_int("APC_AMG_G8", a.g8_rows);
-  a.g4_rows = env_int("APC_AMG_G4", a.g4_rows);
-  a.rg8_rows = env_int("APC_AMG_RG8", a.rg8_rows);
+  a.g8_rows = c->knob[APC_KNOB_AMG_G8_ROWS];
+  a.g4_rows = c->knob[APC_KNOB_AMG_G4_ROWS];
+  a.rg8_rows = c->knob[APC_KNOB_AMG_RG8_ROWS];
   if (const char *v = getenv("APC_AMG_OMEGA")) a.omega = atof(v);
   if (const char *v = getenv("APC_AMG_ALPHA")) a.alpha = atof(v);
   if (const char *v = getenv("APC_AMG_THETA")) a.theta = atof(v);
@@ -695,7 +694,7 @@ static int gmres(apc_ctx *c, const Op<double> &A, const double *b, double *x,
   }
   if (!fp32) {
     if (amg) {
-      r = amg_setup(c, &S->amg, A, pinned, false, amg_options());
+      r = amg_setup(c, &S->amg, A, pinned, false, amg_options(c));
       if (r) return r;
     }
     if (seg)
@@ -718,7 +717,7 @@ static int gmres(apc_ctx *c, const Op<double> &A, const double *b, double *x,
   if (amg) {
     // FP32 multigrid inside FP64 flexible GMRES (the FP64 Krylov cycle is the
     // refinement loop)
-    r = amg_setup(c, &S->amg, A, pinned, true, amg_options());
+    r = amg_setup(c, &S->amg, A, pinned, true, amg_options(c));
     if (r) return r;
     APC_ALLOC(c, S->V64, (size_t)(m + 1) * M + 1);
     APC_ALLOC(c, S->Z32, (size_t)(m + 1) * M + 1);

@@ -31,7 +31,6 @@
 namespace apc {
 
 constexpr int SG_TILE = 1024;  // rows per tile: 256 threads x 4 rows
-constexpr int SG_BLOCKS = 2048;
 
 struct SegDev {
   GState *st;   // [nseg]
@@ -495,7 +494,7 @@ int run_cycles_seg(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, con
   constexpr int MU = (sizeof(T) == 4) ? 8 : 4;
   const size_t ld = (size_t)M;
   const unsigned gb = nblocks(M, 256);
-  const unsigned gt = (unsigned)std::min(ntile, SG_BLOCKS);
+  const unsigned gt = (unsigned)std::min(ntile, c->knob[APC_KNOB_SEG_BLOCKS]);
   const double nnz = (double)M * A64.lm;
   const double bspmv = fp32 ? 8.0 * nnz + 20.0 * M : 12.0 * nnz + 40.0 * M;
   int itcount = 0, ncyc = 0;

@@ -1081,12 +1081,9 @@ int search_neighbors(apc_ctx *c) {
     int grid = std::min(M, c->search_waves);
     if (grid >= 8) grid -= grid % 8;  // a multiple of 8: node chunks dealt to the XCDs
     grid = std::max(1, grid);
-    // 256-node chunks; 16 to 4096 measured alike at 256^3 (58.3 -> 53.9 ms
-    // against a plain grid stride).  APC_NEIGH_XCHUNK overrides (tuning runs).
-    static const int xchunk = [] {
-      const char *v = getenv("APC_NEIGH_XCHUNK");
-      return v ? std::max(1, atoi(v)) : 256;
-    }();
+    // node chunks of the knob's size (default 256; 16 to 4096 measured alike
+    // at 256^3, 58.3 -> 53.9 ms against a plain grid stride)
+    const int xchunk = c->knob[APC_KNOB_SEARCH_CHUNK];
     hipLaunchKernelGGL(k_search_wave, dim3(grid), dim3(64), 0, c->s, M, S, lm,
                        std::min(fast_cap, WCAP), xchunk, F, (const double4 *)xyz, c->n24.p, c->n24cnt.p,
                        c->nbr.p, c->dist.p, c->status_dev.p, c->novf.p, c->visits.p,

@@ -39,12 +39,16 @@ accelerations and largest position difference per step; (2) the reference
 configuration (FP64, Jacobi, the same GMRES settings) for one step after one
 untimed step of it (first-use allocations): its time, iterations and its
 force difference from the converged FP64 step (skipped with
---no-ref-config); (3) the FP64 step with the same solver settings timed over
-K steps (skipped with --no-fp64); (4) the measured precision timed over K
-steps -> "value".  The roofline entry is for the kernel with the largest
-total device time (average launch time x launches) in one extra step of leg
-(4) run with the library's kernel timers on (the timed steps run without
-them); the next kernels by device time are listed beside it.
+--no-ref-config); then the launch shapes are autotuned (apcloud.tune, on the
+first schedule step of the measured form; --no-tune keeps the built-in
+shapes) and the choices recorded; (3) the FP64 step with the same solver
+settings timed over K steps (skipped with --no-fp64); (4) the measured
+precision timed over K steps -> "value".  The roofline entry is for the
+kernel with the largest total device time (average launch time x launches)
+in one extra step of leg (4) run with the library's kernel timers on (the
+timed steps run without them); the next kernels by device time are listed
+beside it.  (5) the multi-box form (see multibox_leg; --no-multibox skips
+it), tuned the same way.
 """
 import argparse
 import json
@@ -131,6 +135,8 @@ def main():
     ap.add_argument("--precision", type=int, default=1,
                     help="measured form: 1 FP32 GMRES + FP64 refinement (default), 0 FP64")
     ap.add_argument("--no-multibox", action="store_true", help="skip the packed multi-box leg")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the built-in launch shapes (no autotuning before the timed legs)")
     ap.add_argument("--multibox-boxes", type=int, default=48)
     ap.add_argument("--multibox-steps", type=int, default=20)
     ap.add_argument("--verbose", action="store_true")
@@ -284,6 +290,16 @@ def main():
             ctx.set_timing(False)
         return wall, infos, dts, phases, kt, info.nc + info.nv
 
+    # launch shapes: autotuned on the first schedule step of the measured
+    # form, from the state after the half drift, before any timed leg
+    tuned = None
+    if not args.no_tune:
+        from apcloud import tune
+        log("autotuning launch shapes")
+        ctx.checkpoint(0, restore=True)
+        o_t = opts(args.precision, args.pc)
+        tuned = tune.autotune(ctx, lambda: one_step(0, o_t))
+
     fp64 = None
     if not args.no_fp64 and args.precision != 0:
         wall64, infos64, dts64, _, _, _ = timed(0)
@@ -353,6 +369,7 @@ def main():
         "accuracy": acc,
         "roofline": roof,
         "roofline_next": others,
+        "autotune": tuned,
         "cpu_baseline": None,
     }
     if args.verbose:
@@ -425,6 +442,12 @@ def multibox_leg(args):
     err = float(np.sqrt((d ** 2).sum() / (np.stack(ref[6:9]) ** 2).sum()))
     P.checkpoint(0, restore=True)
     o = opts(args.precision)
+    tuned = None
+    if not args.no_tune:
+        from apcloud import tune
+        log("multibox leg: autotuning launch shapes")
+        tuned = tune.autotune(P, lambda: step(P, 0, o))
+        P.checkpoint(0, restore=True)
     step(P, 0, o)  # warmup
     P.checkpoint(0, restore=True)
     P.sync()
@@ -449,6 +472,9 @@ def multibox_leg(args):
         g.drift(cx0, 0.5 * da)
         ctxs.append(g)
     for g, n in zip(ctxs, counts):
+        if tuned:  # the packed context's tuned shapes (the tile grid has no use here)
+            for k, v in tuned["choices"].items():
+                g.set_knob(k, v)
         step(g, 0, o, 1.0 / n)  # warmup (first-use allocations)
     for g in ctxs:
         g.sync()
@@ -467,6 +493,7 @@ def multibox_leg(args):
             "particles": N, "rows": rows, "sides": [s["n_side"] for s in specs],
             "steps": len(sched), "gmres_its": its, "last_step_poisson_its_per_box": its_box,
             "first_step_rel_rms_force_err_vs_fp64": err,
+            "autotune": tuned,
             "separate_contexts_ms_per_step": round(1e3 * wall_sep, 2),
             "speedup_vs_separate_contexts": round(1e3 * wall_sep / (1e3 * wall / len(sched)), 3),
             "note": "one packed context (apc_create_packed): each kernel of a step over all "

@@ -97,6 +97,26 @@ int apc_device_count(int *n);
 int apc_create(const apc_params *p, apc_ctx **out);
 int apc_destroy(apc_ctx *ctx);
 
+/* Launch-shape knobs of this build's kernels, per context (no counterpart in
+ * the reference).  Defaults below; an APC_* environment variable of the same
+ * meaning (APC_NEIGH_XCHUNK, APC_MDOT_BLOCKS, APC_MAXPY_BLOCKS, APC_SEG_BLOCKS,
+ * APC_AMG_G8, APC_AMG_G4, APC_AMG_RG8) overrides a default at apc_create; the
+ * autotuner (apcloud.tune) sets them with apc_set_knob.  Every value >= 1 is
+ * valid and changes only speed, never results beyond the summation order of
+ * the reductions. */
+enum {
+  APC_KNOB_SEARCH_CHUNK = 0, /* nodes per chunk the neighbour search deals to the XCDs (256) */
+  APC_KNOB_MDOT_BLOCKS,      /* grid of the GMRES multi-dot (1024) */
+  APC_KNOB_MAXPY_BLOCKS,     /* grid of the GMRES orthogonalisation update (1024) */
+  APC_KNOB_SEG_BLOCKS,       /* grid of the packed boxes' tile kernels (2048) */
+  APC_KNOB_AMG_G8_ROWS,      /* multigrid levels of at most this many rows: 8 lanes per row (65536) */
+  APC_KNOB_AMG_G4_ROWS,      /* ... else of at most this many rows: 4 lanes per row (2^30) */
+  APC_KNOB_AMG_RG8_ROWS,     /* restrictions into at most this many rows: 8 lanes (2^30) */
+  APC_KNOB_COUNT
+};
+int apc_set_knob(apc_ctx *ctx, int knob, int value);
+int apc_get_knob(apc_ctx *ctx, int knob, int *value);
+
 /* Packed boxes: nbox (<= 64) independent periodic boxes of the same
  * parameters in one context.  No counterpart in the reference, which runs one
  * box per Controller (controller.cpp:16-219); a packed context runs every box
