@@ -10,8 +10,8 @@ the reference's Jacobi GMRES in FP64 and with the FP32 multigrid inside FP64
 GMRES: every box's solve takes the iterations of the box alone (each box
 keeps its own Krylov space and convergence test), accelerations within 1e-6
 relative and the positions' difference within 1e-6 of the displacement
-over the run.  A
-tree that the key budget's lowered depth would cut is refused."""
+over the run.  A tree that the key budget's lowered depth would cut is
+refused."""
 import numpy as np
 import pytest
 
