@@ -113,6 +113,25 @@ def traffic_from_profiles(kernel, workload, alg_bytes):
         return None
 
 
+def issue_from_profiles(kernel, workload):
+    """Vector / scalar instruction-issue utilisation of `kernel` from the
+    committed counter summary (profiles/pmc_issue.json: rocprofv3 SQ_INSTS_*
+    and GRBM_GUI_ACTIVE), or None.  For a kernel bound by instruction issue
+    rather than HBM this is its roofline position."""
+    try:
+        with open(os.path.join(REPO, "profiles", "pmc_issue.json")) as f:
+            d = json.load(f)
+        if d.get("workload") != workload:
+            return None
+        e = d["kernels"][kernel]
+        return {"valu_busy": e["valu_busy"], "salu_busy": e["salu_busy"],
+                "valu_insts_per_launch": e["valu_insts"],
+                "source": "profiles/pmc_issue.json (rocprofv3 SQ_INSTS_VALU x 4 cycles / "
+                          "(1024 SIMDs x GRBM_GUI_ACTIVE / 8))"}
+    except (OSError, KeyError, ValueError, TypeError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -324,6 +343,7 @@ def main():
         return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                 "traffic": traffic_from_profiles(name, args.workload, b),
+                "issue": issue_from_profiles(name, args.workload),
                 "bytes_per_launch": int(b), "avg_launch_ms": round(avg_ms, 4), "launches": cnt,
                 "total_ms": round(avg_ms * cnt, 3)}
 
