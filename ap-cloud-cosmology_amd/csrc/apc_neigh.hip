@@ -349,10 +349,6 @@ struct WaveLds {
 // k = 0..WCAP-1; written by the host before the search
 __constant__ double c_pen[WCAP];
 
-__device__ inline uint64_t lanes_below() {
-  const int lane = threadIdx.x & 63;
-  return lane ? (~0ull >> (64 - lane)) : 0ull;
-}
 
 // exclusive prefix sum over the 64 lanes; *total gets the sum
 __device__ inline int wave_excl_scan_i(int v, int *total) {
@@ -473,9 +469,8 @@ __device__ void wave_qsort(WaveLds &L, int lo0, int hi0, int need) {
       const bool gt = valid && (WD[valid ? p : lo] > pivot);
       const bool le = valid && !gt;
       const uint64_t mG = __ballot(gt), mL = __ballot(le);
-      const uint64_t lt = lanes_below();
-      if (gt) gp[nG + __popcll(mG & lt)] = p;
-      if (le) la[nL + __popcll(mL & lt)] = p;
+      if (gt) gp[nG + below(mG)] = p;
+      if (le) la[nL + below(mL)] = p;
       nG += __popcll(mG);
       nL += __popcll(mL);
     }
@@ -542,12 +537,11 @@ __device__ inline void octant_order(const WaveLds &L, int l, const int (&off)[8]
     const uint32_t v = valid ? (uint32_t)L.mo[e] : 0u;
     const int o = valid ? (int)(v >> 29) : 8;
     int rank = 0, pos = 0;
-    const uint64_t lt = lanes_below();
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const uint64_t mk = __ballot(o == q);
       if (o == q) {
-        rank = run[q] + __popcll(mk & lt);
+        rank = run[q] + below(mk);
         pos = off[q] + rank;
       }
       run[q] += __popcll(mk);
@@ -707,15 +701,14 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
         int o = 8;
         double dd = 0.0;
         if (fresh) o = place_p(F, pm, x0, y0, z0, &dd);
-        const uint64_t lt = lanes_below();
         const uint64_t mf = __ballot(fresh);
-        const int rank = __popcll(mf & lt);  // position among this chunk's new entries
+        const int rank = below(mf);  // position among this chunk's new entries
         int before = 0;                      // same-octant new entries before it
         int add[8];                          // new entries per octant (wave-uniform)
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
           const uint64_t mk = __ballot(fresh && o == q);
-          if (o == q) before = nn[q] + __popcll(mk & lt);
+          if (o == q) before = nn[q] + below(mk);
           add[q] = __popcll(mk);
         }
         // the reference's checks in its order: invalid index, octant full, list full

@@ -30,6 +30,14 @@ __device__ inline int block_excl_scan256(int v, int *ws, int *total) {
   return off + inc - v;
 }
 
+// set bits of the wave mask m in the lanes below this one (v_mbcnt_lo /
+// v_mbcnt_hi: two vector instructions where popcount(m & lanes-below mask)
+// takes four)
+__device__ inline int below(uint64_t m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 __device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

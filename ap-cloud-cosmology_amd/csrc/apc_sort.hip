@@ -85,7 +85,6 @@ __global__ void __launch_bounds__(256) k_rx_scatter(const uint64_t *__restrict__
   __shared__ int run[256];    // keys of digit t placed by earlier iterations
   __shared__ int wcnt[4][256];
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const unsigned long long lower = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   gbase[t] = offs[t * nblk + blockIdx.x];
   {  // exclusive scan of this tile's digit counts over the block
     const int c0 = hist[t * nblk + blockIdx.x];
@@ -120,7 +119,7 @@ __global__ void __launch_bounds__(256) k_rx_scatter(const uint64_t *__restrict__
       unsigned long long bal = __ballot((d >> b) & 1);
       peers &= ((d >> b) & 1) ? bal : ~bal;
     }
-    int rank = __popcll(peers & lower);
+    int rank = below(peers);
     int cnt = __popcll(peers);
     if (valid && rank == cnt - 1) wcnt[w][d] = cnt;
     __syncthreads();
