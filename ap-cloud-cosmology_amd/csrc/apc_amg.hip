@@ -1639,8 +1639,10 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
   Op<T> E = A0;
   Sell<T> C{L->soff.p, L->swid.p, L->scol.p, P.val, n};
   const bool ell = (l == 0);
-  // level-0 algorithmic bytes per row: each of the lm slots reads its index,
-  // weight and the gathered x; plus d, f, dinv, b, own x read and x written
+  // level-0 algorithmic bytes per row, the SpMV convention (s + 4) nnz + M
+  // (vectors): each of the lm slots reads its index and weight; the vector x
+  // counts once per row like every other vector (d, f, b, x, r for the
+  // residual; d, f, dinv, b, x in, x out for a sweep)
   const double sT = sizeof(T);
   const double lm0 = A0.lm;
   const bool f32 = sizeof(T) == 4;
@@ -1659,7 +1661,7 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
       return;
     }
     if (ell) {
-      KScope ks(c, f32 ? "amg_sweep0_32" : "amg_sweep0", (double)n * (lm0 * (4 + 2 * sT) + 6 * sT));
+      KScope ks(c, f32 ? "amg_sweep0_32" : "amg_sweep0", (double)n * (lm0 * (4 + sT) + 6 * sT));
       hipLaunchKernelGGL((k_cy_sweep<T, true>), dim3(g), dim3(256), 0, c->s, E, C, dinv, b, xin,
                          xout, o.omega, st);
     } else {
@@ -1705,7 +1707,7 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
   const int gam = (l < o.wlevels) ? o.gamma : 1;
   for (int q = 0; q < gam; ++q) {
     if (ell) {
-      KScope ks(c, f32 ? "amg_resid0_32" : "amg_resid0", (double)n * (lm0 * (4 + 2 * sT) + 5 * sT));
+      KScope ks(c, f32 ? "amg_resid0_32" : "amg_resid0", (double)n * (lm0 * (4 + sT) + 5 * sT));
       hipLaunchKernelGGL((k_cy_resid<T, true>), dim3(g), dim3(256), 0, c->s, E, C, b, cur, r, st);
     } else if (fuse_first && q == 0) {
       KScope ks(c, kResidName[l]);
