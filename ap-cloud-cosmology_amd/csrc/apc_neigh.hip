@@ -797,9 +797,42 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
         }
       __syncthreads();
     }
-    // octant 0 quicksorted on the squared distance, then penalised
-    if (nn[0] > 1) wave_qsort(L, 0, nn[0] - 1, nn[0]);
-    for (int q = lane; q < nn[0]; q += 64) L.d2[q] = c_pen[q] * L.d2[q];
+    // octant 0 quicksorted on the squared distance, then penalised.  With
+    // distinct keys (57% of the nodes with two or more at 256^3) the
+    // quicksort's order is the sorted order: every lane places its key at its
+    // rank at once.  With equal keys the order among them is the quicksort's
+    // own, replayed exactly by the wave sort.  (The final lists, where each
+    // octant's first entry keeps its raw distance, nearly always hold equal
+    // keys; they keep the replay in k_search_sort.)
+    {
+      const int n0 = nn[0];
+      bool placed = false;
+      if (n0 > 1 && n0 <= 64) {
+        double ke = 0.0;
+        int me = 0, lt = 0, eq = 0;
+        if (lane < n0) {
+          ke = L.d2[lane];
+          me = L.mo[lane];
+          for (int j = 0; j < n0; ++j) {
+            const double kj = L.d2[j];
+            lt += (kj < ke) ? 1 : 0;
+            eq += (kj == ke) ? 1 : 0;
+          }
+        }
+        if (__ballot(lane < n0 && eq > 1) == 0ull) {  // no key equals another
+          __syncthreads();
+          if (lane < n0) {
+            L.d2[lt] = c_pen[lt] * ke;
+            L.mo[lt] = me;
+          }
+          placed = true;
+        }
+      }
+      if (!placed) {
+        if (n0 > 1) wave_qsort(L, 0, n0 - 1, n0);
+        for (int q = lane; q < n0; q += 64) L.d2[q] = c_pen[q] * L.d2[q];
+      }
+    }
     __syncthreads();
     if (l <= SCAP) {
       // The final quicksort goes to k_search_sort, one lane per node.  It
