@@ -5,12 +5,12 @@ Per box, at the first step: the same computational rows in the same order
 (positions, cell lengths, particle counts, types; particle ranges shifted by
 the box's first particle), Neigh24 and the neighbour lists bit for bit
 (indices shifted by the box's first row), and the system rows bit for bit
-(the pinned first row of every box included).  Over two leapfrog steps with
-the reference's Jacobi GMRES in FP64 and with the FP32 multigrid inside FP64
-GMRES: every box's solve takes the iterations of the box alone (each box
-keeps its own Krylov space and convergence test), accelerations within 1e-6
-relative and the positions' difference within 1e-6 of the displacement
-over the run.  A tree that the key budget's lowered depth would cut is
+(the pinned first row of every box included).  Over two leapfrog steps in
+each of the four solver forms (Jacobi or multigrid, FP64 or an FP32 inner
+solve in FP64 refinement): every box's solve takes the iterations of the
+box alone (each box keeps its own Krylov space and convergence test),
+accelerations within 1e-6 relative and the positions' difference within
+1e-6 of the displacement over the run.  A tree that the key budget's lowered depth would cut is
 refused."""
 import numpy as np
 import pytest
@@ -87,15 +87,20 @@ def test_packed_rows_lists_and_systems_match_each_box_alone():
     P.close()
 
 
-@pytest.mark.parametrize("form", ["jacobi_fp64", "amg_fp32ir"])
+FORMS = {  # the four GMRES forms: Jacobi (left) or multigrid (right), FP64 or FP32 inner
+    "jacobi_fp64": dict(restart=200, max_it=3000),
+    "jacobi_fp32ir": dict(restart=200, max_it=3000, precision=1),
+    "amg_fp64": dict(restart=60, max_it=500, pc="amg"),
+    "amg_fp32ir": dict(restart=60, max_it=500, precision=1, pc="amg"),
+}
+
+
+@pytest.mark.parametrize("form", sorted(FORMS))
 def test_packed_steps_match_each_box_alone(form):
     bx = make_boxes(6, (12, 16, 24), 7)
     print("case: packed steps %s, sides" % form,
           [round(len(b[0]) ** (1 / 3)) for b in bx], flush=True)
-    if form == "jacobi_fp64":
-        opts = apcloud.ksp(rtol=1e-8, restart=200, max_it=3000)
-    else:
-        opts = apcloud.ksp(rtol=1e-8, restart=60, max_it=500, precision=1, pc="amg")
+    opts = apcloud.ksp(rtol=1e-8, **FORMS[form])
     P = packed_for(bx)
     singles = []
     for box in bx:
