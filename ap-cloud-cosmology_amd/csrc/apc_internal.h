@@ -159,7 +159,8 @@ struct apc_ctx {
   int search_pen = 0;          // its penalty table is on the device
   apc::DBuf<int> status_dev;
   apc::DBuf<unsigned long long> visits;   // Neigh24 entries visited, list entries sorted
-  apc::DBuf<uint8_t> scodes, slen;        // per node: sort codes of its list, list length
+  apc::DBuf<uint8_t> scodes, slen;        // per node: sort codes of its list; list length,
+                                          // then (at M) its octant-0 replay count
   apc::DBuf<int> sidx;                    // per node: its list (node indices)
   unsigned long long search_visits = 0;
   apc::SolveState *gm = nullptr;

@@ -567,6 +567,7 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
   // visit (index + centre) plus its output
   unsigned long long nvis = 0;
   unsigned long long nlist = 0;  // list entries handed to k_search_sort
+  unsigned long long nrank = 0;  // octant-0 rank codes handed to it
   const int lane = threadIdx.x;
   const int nq = 10 * lm;
   // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
@@ -797,19 +798,29 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
         }
       __syncthreads();
     }
-    // octant 0 quicksorted on the squared distance, then penalised.  With
-    // distinct keys (57% of the nodes with two or more at 256^3) the
-    // quicksort's order is the sorted order: every lane places its key at its
-    // rank at once.  With equal keys the order among them is the quicksort's
-    // own, replayed exactly by the wave sort.  (The final lists, where each
-    // octant's first entry keeps its raw distance, nearly always hold equal
-    // keys; they keep the replay in k_search_sort.)
+    // octant 0 quicksorted on the squared distance, then penalised.  The key
+    // at sorted position p is 1e4^p times the p-th smallest distance,
+    // whatever the order among equal distances, so each lane places its key
+    // at its rank (the keys below it, then the equal keys before it in list
+    // order).  The indices: with distinct keys (57% of the nodes with two or
+    // more at 256^3) the quicksort's order is the sorted order and each lane
+    // places its index with its key.  With equal keys their order is the
+    // quicksort's own: the indices stay in list order and k_search_sort
+    // replays the quicksort (one lane per node) on the rank codes written
+    // after the node's list codes, from byte r0 of its code row.  An octant
+    // over 64 entries, or a list sorted here (over SCAP) or with no room for
+    // the codes, takes the wave sort's exact replay.  (The final lists, where
+    // each octant's first entry keeps its raw distance, nearly always hold
+    // equal keys; their replay is k_search_sort's.)
+    const int r0 = (l + 7) & ~7;
+    int defer = 0;  // octant-0 entries whose order k_search_sort replays
+    int rc = 0;     // rank code of this lane's octant-0 entry (lane < defer)
     {
       const int n0 = nn[0];
       bool placed = false;
       if (n0 > 1 && n0 <= 64) {
         double ke = 0.0;
-        int me = 0, lt = 0, eq = 0;
+        int me = 0, lt = 0, eq = 0, eb = 0;
         if (lane < n0) {
           ke = L.d2[lane];
           me = L.mo[lane];
@@ -817,14 +828,18 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
             const double kj = L.d2[j];
             lt += (kj < ke) ? 1 : 0;
             eq += (kj == ke) ? 1 : 0;
+            eb += (kj == ke && j < lane) ? 1 : 0;
           }
         }
-        if (__ballot(lane < n0 && eq > 1) == 0ull) {  // no key equals another
+        const bool ties = __ballot(lane < n0 && eq > 1) != 0ull;
+        if (!ties || (l <= SCAP && r0 + n0 <= SCAP)) {
           __syncthreads();
           if (lane < n0) {
-            L.d2[lt] = c_pen[lt] * ke;
-            L.mo[lt] = me;
+            L.d2[lt + eb] = c_pen[lt + eb] * ke;
+            if (!ties) L.mo[lt] = me;
           }
+          rc = lt;
+          defer = ties ? n0 : 0;
           placed = true;
         }
       }
@@ -862,8 +877,13 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
         oc[64 + lane] = (uint8_t)c1;
         oi[64 + lane] = L.mo[64 + lane];
       }
-      if (lane == 0) slen[i0] = (uint8_t)l;
+      if (lane < defer) oc[r0 + lane] = (uint8_t)rc;
+      if (lane == 0) {
+        slen[i0] = (uint8_t)l;
+        slen[(size_t)M + i0] = (uint8_t)defer;
+      }
       nlist += (unsigned)l;
+      nrank += (unsigned)defer;
     } else {
       wave_qsort(L, 0, l - 1, lm);  // positions 0..lm-1 are the output
       for (int q = lane; q < lm; q += 64) {
@@ -879,6 +899,7 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
   }
   if (lane == 0 && nvis) atomicAdd(visits, nvis);  // one atomic per wave
   if (lane == 0 && nlist) atomicAdd(visits + 1, nlist);
+  if (lane == 0 && nrank) atomicAdd(visits + 2, nrank);
 }
 
 // The final quicksort of k_search_wave's lists of at most SCAP entries, one
@@ -938,6 +959,45 @@ __device__ inline int code_part(uint8_t *a, uint8_t *b, int lo, int hi) {
   return j;
 }
 
+// codes row[0..n-1] into a; bytes past n in its last word zeroed (the scans
+// read whole words)
+__device__ inline void load_codes(uint8_t *a, const uint2 *__restrict__ src, int n) {
+  for (int w = 0; 8 * w < n; ++w) {
+    uint2 v = src[w];
+    const int left = n - 8 * w;  // codes in this 8-byte word
+    if (left < 8) {
+      const uint64_t keep = ~0ull >> (64 - 8 * left);
+      uint64_t u = ((uint64_t)v.y << 32) | v.x;
+      u &= keep;
+      v = make_uint2((uint32_t)u, (uint32_t)(u >> 32));
+    }
+    *reinterpret_cast<uint2 *>(a + 8 * w) = v;
+  }
+}
+
+// the reference's quicksort (aquickSort) of (a, b)[0..hi] by one lane;
+// ranges lying wholly at or above `need` are left unsorted
+__device__ inline void lane_code_sort(uint8_t *a, uint8_t *b, int hi0, int need, uint16_t *stk,
+                                      int t) {
+  int top = 0;
+  stk[t] = (uint16_t)(0 | (hi0 << 8));
+  top = 1;
+  while (top > 0) {
+    const int pk = stk[(--top) * 64 + t];
+    int lo = pk & 0xff, hi = pk >> 8;
+    while (lo < hi && lo < need) {
+      const int j = code_part(a, b, lo, hi);
+      if (j - lo < hi - j) {
+        stk[(top++) * 64 + t] = (uint16_t)((j + 1) | (hi << 8));
+        hi = j - 1;
+      } else {
+        stk[(top++) * 64 + t] = (uint16_t)(lo | ((j - 1) << 8));
+        lo = j + 1;
+      }
+    }
+  }
+}
+
 __global__ void __launch_bounds__(64) k_search_sort(int M, int lm, Frame F,
                                                     const double4 *__restrict__ xyz,
                                                     const uint8_t *__restrict__ scodes,
@@ -953,41 +1013,26 @@ __global__ void __launch_bounds__(64) k_search_sort(int M, int lm, Frame F,
   if (i0 >= M) return;
   const int l = slen[i0];
   if (l == 0) return;  // sorted by k_search_wave or by the long-list pass
+  const int n0 = slen[(size_t)M + i0];
   uint8_t *a = sa + t * SSTR;
   uint8_t *b = sb + t * SSTR;
-  // the codes, bytes at and past l zeroed (the scans may read them); the
-  // positions 0..l-1 as payload
   const uint2 *src = reinterpret_cast<const uint2 *>(scodes + (size_t)i0 * SCAP);
+  // the positions 0..l-1 as payload
   for (int w = 0; 8 * w < l; ++w) {
-    uint2 v = src[w];
-    const int left = l - 8 * w;  // codes in this 8-byte word
-    if (left < 8) {
-      const uint64_t keep = (left <= 0) ? 0ull : (~0ull >> (64 - 8 * left));
-      uint64_t u = ((uint64_t)v.y << 32) | v.x;
-      u &= keep;
-      v = make_uint2((uint32_t)u, (uint32_t)(u >> 32));
-    }
-    *reinterpret_cast<uint2 *>(a + 8 * w) = v;
     const uint32_t base = 0x03020100u + 0x08080808u * (uint32_t)w;
     *reinterpret_cast<uint2 *>(b + 8 * w) = make_uint2(base, base + 0x04040404u);
   }
-  int top = 0;
-  stk[t] = (uint16_t)(0 | ((l - 1) << 8));
-  top = 1;
-  while (top > 0) {
-    const int pk = stk[(--top) * 64 + t];
-    int lo = pk & 0xff, hi = pk >> 8;
-    while (lo < hi && lo < lm) {
-      const int j = code_part(a, b, lo, hi);
-      if (j - lo < hi - j) {
-        stk[(top++) * 64 + t] = (uint16_t)((j + 1) | (hi << 8));
-        hi = j - 1;
-      } else {
-        stk[(top++) * 64 + t] = (uint16_t)(lo | ((j - 1) << 8));
-        lo = j + 1;
-      }
-    }
+  // octant 0 with equal keys: its quicksort (on the squared distance,
+  // solver.cpp:557-561) replayed on the rank codes first; the permutation of
+  // positions 0..n0-1 it leaves in b is the payload of those slots, whose
+  // list indices are in list order
+  if (n0 > 1) {
+    load_codes(a, src + ((l + 7) >> 3), n0);
+    lane_code_sort(a, b, n0 - 1, n0, stk, t);
   }
+  // the list's codes, bytes past l zeroed
+  load_codes(a, src, l);
+  lane_code_sort(a, b, l - 1, lm, stk, t);
   // output, eight slots at a time: their indices, then their centres, are
   // loaded together so the gathers overlap
   const double4 p0 = xyz[i0];
@@ -1020,8 +1065,8 @@ int search_neighbors(apc_ctx *c) {
   APC_ALLOC(c, c->dist, (size_t)lm * M + 1);
   APC_ALLOC(c, c->status_dev, 4);
   APC_HIP(c, hipMemsetAsync(c->status_dev.p, 0, 4 * sizeof(int), c->s));
-  APC_ALLOC(c, c->visits, 2);
-  APC_HIP(c, hipMemsetAsync(c->visits.p, 0, 2 * sizeof(unsigned long long), c->s));
+  APC_ALLOC(c, c->visits, 3);
+  APC_HIP(c, hipMemsetAsync(c->visits.p, 0, 3 * sizeof(unsigned long long), c->s));
   Frame F;
   {  // solver.cpp:464-475, computed on the host
     double alpha = 0.01, beta = 0.001, gamma = 0.0001;
@@ -1087,7 +1132,7 @@ int search_neighbors(apc_ctx *c) {
   if (wave) {
     APC_ALLOC(c, c->scodes, (size_t)M * SCAP + 16);
     APC_ALLOC(c, c->sidx, (size_t)M * SCAP + 4);
-    APC_ALLOC(c, c->slen, (size_t)M + 16);
+    APC_ALLOC(c, c->slen, 2 * (size_t)M + 16);
     if (!c->search_pen) {  // per context: the constant table lives on its device
       static const std::array<double, WCAP> pen = [] {
         std::array<double, WCAP> t{};
@@ -1121,23 +1166,26 @@ int search_neighbors(apc_ctx *c) {
                        c->status_dev.p, c->novf.p);
   }
   int st[3] = {0, 0, 0};
-  unsigned long long cnt2[2] = {0, 0};
+  unsigned long long cnt2[3] = {0, 0, 0};
   APC_HIP(c, hipMemcpyAsync(st, c->status_dev.p, sizeof(st), hipMemcpyDeviceToHost, c->s));
   APC_HIP(c, hipMemcpyAsync(cnt2, c->visits.p, sizeof(cnt2), hipMemcpyDeviceToHost, c->s));
   APC_HIP(c, hipStreamSynchronize(c->s));
-  const double nvis = (double)cnt2[0], nlist = (double)cnt2[1];
+  const double nvis = (double)cnt2[0], nlist = (double)cnt2[1], nrank = (double)cnt2[2];
   c->search_visits = cnt2[0];
   // algorithmic bytes of the wave search: 28 per Neigh24 entry visited (its
   // index and the centre of the node it names), the node's own centre and
-  // count, and per list entry handed on its code (1) and index (4)
-  if (search_scope >= 0) ktime_bytes(c, search_scope, 28.0 * nvis + 28.0 * M + 5.0 * nlist);
+  // count, per list entry handed on its code (1) and index (4), per node
+  // its list length and octant-0 replay count (2) and per octant-0 entry
+  // handed on its rank code (1)
+  if (search_scope >= 0)
+    ktime_bytes(c, search_scope, 28.0 * nvis + 30.0 * M + 5.0 * nlist + nrank);
   // the final sorts (every list in the wave search's LDS was complete: no
   // error was raised, so every node's length entry is this search's)
   if (wave && st[0] == 0) {
-    // per node: length 1, codes (read whole), per output slot the index
-    // (4), the neighbour's centre (24), index + distance written (12), own
-    // centre 24
-    KScope ks(c, "neighbor_sort", nlist + (25.0 + 40.0 * lm) * M);
+    // per node: length and replay count 2, codes (read whole) and octant-0
+    // rank codes, per output slot the index (4), the neighbour's centre
+    // (24), index + distance written (12), own centre 24
+    KScope ks(c, "neighbor_sort", nlist + nrank + (26.0 + 40.0 * lm) * M);
     hipLaunchKernelGGL(k_search_sort, dim3(nblocks(M, 64)), dim3(64), 0, c->s, M, lm, F,
                        (const double4 *)xyz, c->scodes.p, c->sidx.p, c->slen.p, c->nbr.p,
                        c->dist.p);
