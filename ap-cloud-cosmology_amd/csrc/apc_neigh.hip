@@ -704,18 +704,25 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
         if (fresh) o = place_p(F, pm, x0, y0, z0, &dd);
         const uint64_t mf = __ballot(fresh);
         const int rank = below(mf);  // position among this chunk's new entries
-        int before = 0;                      // same-octant new entries before it
-        int add[8];                          // new entries per octant (wave-uniform)
+        int add[8];                  // new entries per octant (wave-uniform; o is 8
+                                     // for every lane that adds none)
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const uint64_t mk = __ballot(fresh && o == q);
-          if (o == q) before = nn[q] + below(mk);
-          add[q] = __popcll(mk);
+        for (int q = 0; q < 8; ++q) add[q] = __popcll(__ballot(o == q));
+        // the reference's checks in its order: invalid index, octant full, list
+        // full.  An entry's same-octant predecessors are at most l + rank, so
+        // with nq >= cap the list fills first: the node then takes the long
+        // pass, which makes the checks with the full bound.
+        int before = 0;  // same-octant entries before this one
+        if (nq < cap) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const uint64_t mk = __ballot(o == q);
+            if (o == q) before = nn[q] + below(mk);
+          }
         }
-        // the reference's checks in its order: invalid index, octant full, list full
         int code = 0;
         if (invalid) code = APC_ERR_NEIGH_INVALID;
-        else if (fresh && before >= nq) code = APC_ERR_NEIGH_MANY;
+        else if (fresh && nq < cap && before >= nq) code = APC_ERR_NEIGH_MANY;
         else if (fresh && l + rank >= cap) code = 1;  // overflow: longer list needed
         const uint64_t me = __ballot(code != 0);
         if (me) {
