@@ -525,27 +525,43 @@ __device__ void wave_qsort(WaveLds &L, int lo0, int hi0, int need) {
 // per-octant rank of every list entry and its position in octant order
 // (octants ascending, insertion order within an octant); calls
 // f(r, e, m, o, rank, pos) for entry e = 64 r + lane
+//
+// The lanes of one octant are found from three ballots of the octant's bits:
+// lane j shares lane i's octant iff bit j of every bit-mask b_k equals bit k
+// of i's octant, so i's mask is valid & ~((b0 ^ x0) | (b1 ^ x1) | (b2 ^ x2))
+// with x_k all ones where i's bit k is set.  Lane q < 8 keeps octant q's
+// offset (off[q] plus the octant's entries in earlier rows), which every
+// lane reads at its own octant.
+__device__ inline uint64_t same_octant(uint64_t bv, uint64_t b0, uint64_t b1, uint64_t b2,
+                                       int o) {
+  const uint32_t x0 = -(uint32_t)(o & 1), x1 = -(uint32_t)((o >> 1) & 1),
+                 x2 = -(uint32_t)((o >> 2) & 1);
+  const uint32_t lo = ((uint32_t)b0 ^ x0) | ((uint32_t)b1 ^ x1) | ((uint32_t)b2 ^ x2);
+  const uint32_t hi = ((uint32_t)(b0 >> 32) ^ x0) | ((uint32_t)(b1 >> 32) ^ x1) |
+                      ((uint32_t)(b2 >> 32) ^ x2);
+  return bv & ~(((uint64_t)hi << 32) | lo);
+}
+
 template <class F>
 __device__ inline void octant_order(const WaveLds &L, int l, const int (&off)[8], F f) {
   const int lane = threadIdx.x & 63;
-  int run[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int offq = 0;  // lane q < 8: off[q]
+#pragma unroll
+  for (int q = 0; q < 8; ++q) offq = (lane == q) ? off[q] : offq;
+  int runq = 0;  // lane q < 8: octant q's entries in the rows before
 #pragma unroll
   for (int r = 0; r < WCAP / 64; ++r) {
     if (64 * r >= l) break;
     const int e = 64 * r + lane;
     const bool valid = e < l;
     const uint32_t v = valid ? (uint32_t)L.mo[e] : 0u;
-    const int o = valid ? (int)(v >> 29) : 8;
-    int rank = 0, pos = 0;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const uint64_t mk = __ballot(o == q);
-      if (o == q) {
-        rank = run[q] + below(mk);
-        pos = off[q] + rank;
-      }
-      run[q] += __popcll(mk);
-    }
+    const int o = (int)(v >> 29);  // 0 on lanes past the list
+    const uint64_t bv = __ballot(valid);
+    const uint64_t b0 = __ballot(o & 1), b1 = __ballot(o & 2), b2 = __ballot(o & 4);
+    const int base = __shfl((runq << 16) | offq, o, 64);  // run | off of octant o
+    const int rank = (base >> 16) + below(same_octant(bv, b0, b1, b2, o));
+    const int pos = (base & 0xffff) + rank;
+    if (64 * (r + 1) < l) runq += __popcll(same_octant(bv, b0, b1, b2, lane & 7));
     if (valid) f(r, e, (int)(v & 0x1fffffffu), o, rank, pos);
   }
 }
