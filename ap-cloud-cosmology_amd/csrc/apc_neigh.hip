@@ -934,8 +934,16 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
 // Ranges lying wholly at or above lm are left unsorted (nothing below them
 // changes).  The output rows are written slot-major by consecutive lanes,
 // i.e. coalesced.
-constexpr int SSTR = SCAP + 8;  // bytes per lane row (8-byte aligned; spreads rows over banks)
-constexpr int SSTK = 12;        // > log2(SCAP) + 1 pending ranges per lane
+//
+// The rows are packed: a lane's row holds its list's length rounded up to 8
+// bytes, at the exclusive prefix sum of those sizes over the wave, and a
+// workgroup sorts the lanes whose rows fit in SBUD bytes per array in one
+// pass, the rest in further passes.  Lists average 47 entries at 256^3
+// (78% below 64), so nearly every wave takes one pass with a quarter of the
+// LDS that rows of SCAP entries would hold, and twice the waves per CU.
+constexpr int SBUD = 4096;  // bytes per array (codes, positions) of one pass
+constexpr int SSTK = 12;    // > log2(SCAP) + 1 pending ranges per lane
+static_assert(SBUD >= SCAP, "every row fits a pass");
 
 __device__ inline uint32_t row_u32(const uint8_t *row, int d) {
   return *reinterpret_cast<const uint32_t *>(row + 4 * d);
@@ -1021,24 +1029,16 @@ __device__ inline void lane_code_sort(uint8_t *a, uint8_t *b, int hi0, int need,
   }
 }
 
-__global__ void __launch_bounds__(64) k_search_sort(int M, int lm, Frame F,
-                                                    const double4 *__restrict__ xyz,
-                                                    const uint8_t *__restrict__ scodes,
-                                                    const int *__restrict__ sidx,
-                                                    const uint8_t *__restrict__ slen,
-                                                    int *__restrict__ nbr,
-                                                    double *__restrict__ dist) {
-  __shared__ __align__(16) uint8_t sa[64 * SSTR];
-  __shared__ __align__(16) uint8_t sb[64 * SSTR];
-  __shared__ uint16_t stk[SSTK * 64];
+// one node's final sort (lane t's row at a, b) and its output rows
+__device__ inline void search_sort_row(int M, int lm, const Frame &F,
+                                       const double4 *__restrict__ xyz,
+                                       const uint8_t *__restrict__ scodes,
+                                       const int *__restrict__ sidx,
+                                       const uint8_t *__restrict__ slen, int *__restrict__ nbr,
+                                       double *__restrict__ dist, uint8_t *a, uint8_t *b,
+                                       uint16_t *stk, int i0, int l) {
   const int t = threadIdx.x;
-  const int i0 = blockIdx.x * 64 + t;
-  if (i0 >= M) return;
-  const int l = slen[i0];
-  if (l == 0) return;  // sorted by k_search_wave or by the long-list pass
   const int n0 = slen[(size_t)M + i0];
-  uint8_t *a = sa + t * SSTR;
-  uint8_t *b = sb + t * SSTR;
   const uint2 *src = reinterpret_cast<const uint2 *>(scodes + (size_t)i0 * SCAP);
   // the positions 0..l-1 as payload
   for (int w = 0; 8 * w < l; ++w) {
@@ -1076,6 +1076,35 @@ __global__ void __launch_bounds__(64) k_search_sort(int M, int lm, Frame F,
         dist[(size_t)(q0 + k) * M + i0] = sqrt(dd);
       }
     }
+  }
+}
+
+__global__ void __launch_bounds__(64) k_search_sort(int M, int lm, Frame F,
+                                                    const double4 *__restrict__ xyz,
+                                                    const uint8_t *__restrict__ scodes,
+                                                    const int *__restrict__ sidx,
+                                                    const uint8_t *__restrict__ slen,
+                                                    int *__restrict__ nbr,
+                                                    double *__restrict__ dist) {
+  __shared__ __align__(16) uint8_t sa[SBUD];
+  __shared__ __align__(16) uint8_t sb[SBUD];
+  __shared__ uint16_t stk[SSTK * 64];
+  const int t = threadIdx.x;
+  const int i0 = blockIdx.x * 64 + t;
+  // l = 0: no node, or sorted by k_search_wave or by the long-list pass
+  const int l = (i0 < M) ? slen[i0] : 0;
+  const int rsz = (l + 7) & ~7;  // the row's bytes
+  int tot = 0;
+  const int off = wave_excl_scan_i(rsz, &tot);
+  for (int base = 0; base < tot;) {
+    // this pass: the rows from base that end within SBUD bytes of it
+    const bool mine = rsz > 0 && off >= base && off + rsz <= base + SBUD;
+    const uint64_t later = __ballot(rsz > 0 && off + rsz > base + SBUD);
+    const int next = later ? __shfl(off, __ffsll((unsigned long long)later) - 1, 64) : tot;
+    if (mine) search_sort_row(M, lm, F, xyz, scodes, sidx, slen, nbr, dist, sa + (off - base),
+                              sb + (off - base), stk, i0, l);
+    base = next;
+    __syncthreads();
   }
 }
 
