@@ -390,6 +390,14 @@ __device__ inline int place_p(const Frame &F, const double4 pm, double x0, doubl
   return (xt > 0) + 2 * (yt > 0) + 4 * (zt > 0);
 }
 
+// x held in a vector register from here on (an empty asm with a VGPR
+// operand): the wave search's frame constants live in VGPRs, which its LDS
+// budget leaves spare, instead of SGPRs, which it spills
+__device__ inline double in_vgpr(double x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 // the same for node m
 __device__ inline int place_w(const Frame &F, const double4 *__restrict__ xyz, double x0, double y0,
                               double z0, int m, double *dd) {
@@ -567,7 +575,7 @@ __device__ inline void octant_order(const WaveLds &L, int l, const int (&off)[8]
 }
 
 __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int cap, int xchunk,
-                                                    Frame F,
+                                                    Frame Fk,
                                                     const double4 *__restrict__ xyz,
                                                     const int *__restrict__ n24,
                                                     const int *__restrict__ n24cnt,
@@ -578,6 +586,12 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
                                                     int *__restrict__ sidx,
                                                     uint8_t *__restrict__ slen) {
   __shared__ WaveLds L;
+  Frame F;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) F.A[k] = in_vgpr(Fk.A[k]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) F.len[k] = in_vgpr(Fk.len[k]);
+  F.bc = Fk.bc;
   // Neigh24 entries this wave visited (the node's own and every expansion
   // round's candidates): the search's algorithmic traffic is 28 bytes per
   // visit (index + centre) plus its output
