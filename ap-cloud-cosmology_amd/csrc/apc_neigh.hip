@@ -943,7 +943,7 @@ __global__ void __launch_bounds__(64) k_search_wave(int M, int S, int lm, int ca
 // lane per node.  Each lane keeps its node's codes (order-equivalent keys,
 // all below 128) and the list positions they carry in LDS rows of its own,
 // and runs the reference's partition scans (apartition, solver.cpp:600-632)
-// four codes per 32-bit read: for codes b, p < 128, b + (127 - p) sets bit 7
+// eight codes per 64-bit read: for codes b, p < 128, b + (127 - p) sets bit 7
 // of its byte exactly when b > p and never carries into the next byte.
 // Ranges lying wholly at or above lm are left unsorted (nothing below them
 // changes).  The output rows are written slot-major by consecutive lanes,
@@ -959,34 +959,37 @@ constexpr int SBUD = 4096;  // bytes per array (codes, positions) of one pass
 constexpr int SSTK = 12;    // > log2(SCAP) + 1 pending ranges per lane
 static_assert(SBUD >= SCAP, "every row fits a pass");
 
-__device__ inline uint32_t row_u32(const uint8_t *row, int d) {
-  return *reinterpret_cast<const uint32_t *>(row + 4 * d);
+// eight codes per read: rows start at multiples of 8 bytes and hold their
+// length rounded up to 8, so a scan's words stay in its row
+__device__ inline uint64_t row_u64(const uint8_t *row, int d) {
+  return *reinterpret_cast<const uint64_t *>(row + 8 * d);
 }
+constexpr uint64_t kHigh8 = 0x8080808080808080ull;
 
 // first x in [x0, r] with a[x] > p, or r + 1 (the reference's i-scan)
 __device__ inline int scan_gt(const uint8_t *a, int x0, int r, int p) {
-  const uint32_t add = 0x01010101u * (uint32_t)(127 - p);
-  int d = x0 >> 2;
-  uint32_t m = ((row_u32(a, d) + add) & 0x80808080u) & (0xffffffffu << (8 * (x0 & 3)));
+  const uint64_t add = 0x0101010101010101ull * (uint64_t)(127 - p);
+  int d = x0 >> 3;
+  uint64_t m = ((row_u64(a, d) + add) & kHigh8) & (~0ull << (8 * (x0 & 7)));
   while (m == 0) {
     ++d;
-    if (4 * d > r) return r + 1;
-    m = (row_u32(a, d) + add) & 0x80808080u;
+    if (8 * d > r) return r + 1;
+    m = (row_u64(a, d) + add) & kHigh8;
   }
-  const int x = 4 * d + (__builtin_ctz(m) >> 3);
+  const int x = 8 * d + (__builtin_ctzll(m) >> 3);
   return x <= r ? x : r + 1;
 }
 
 // last y <= y0 with a[y] <= p; a[lo] == p stops it at lo (the j-scan)
 __device__ inline int scan_le(const uint8_t *a, int y0, int p) {
-  const uint32_t add = 0x01010101u * (uint32_t)(127 - p);
-  int d = y0 >> 2;
-  uint32_t m = (~(row_u32(a, d) + add) & 0x80808080u) & (0xffffffffu >> (8 * (3 - (y0 & 3))));
+  const uint64_t add = 0x0101010101010101ull * (uint64_t)(127 - p);
+  int d = y0 >> 3;
+  uint64_t m = (~(row_u64(a, d) + add) & kHigh8) & (~0ull >> (8 * (7 - (y0 & 7))));
   while (m == 0) {
     --d;
-    m = ~(row_u32(a, d) + add) & 0x80808080u;
+    m = ~(row_u64(a, d) + add) & kHigh8;
   }
-  return 4 * d + ((31 - __builtin_clz(m)) >> 3);
+  return 8 * d + ((63 - __builtin_clzll(m)) >> 3);
 }
 
 __device__ inline int code_part(uint8_t *a, uint8_t *b, int lo, int hi) {
