@@ -403,7 +403,8 @@ __global__ void __launch_bounds__(256) k_amg_reduce(int n, int nc, const uint64_
 //     storage order (coalesced reads of the fine matrix);
 //  3. one 64-lane block per coarse row reads its span and takes the distinct
 //     coarse columns in increasing order by repeated wave minima, summing
-//     each over its entries with a fixed reduction tree (deterministic).  A
+//     each over its entries with a fixed reduction tree (deterministic; DPP row
+//     operations, no LDS crossbar).  A
 //     span of up to GAL_CAP entries is held in registers (as many per lane as
 //     the span needs); a longer one is re-read from memory for every column.
 //  Pass 0 of step 3 counts each coarse row's distinct columns; a span held in
@@ -418,11 +419,7 @@ __global__ void __launch_bounds__(256) k_amg_reduce(int n, int nc, const uint64_
 constexpr int GAL_CAP = 1024;
 constexpr int GAL_R = GAL_CAP / 64;
 
-__device__ inline int wave_min_int(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-  return v;
-}
+__device__ inline int wave_min_int(int v) { return wave_min_int_dpp(v); }
 
 // entry counts in row order (coalesced reads of the fine matrix) ...
 template <class V>
@@ -489,7 +486,7 @@ __device__ inline int gal_row_regs(int I, int E, SRC src, int *oj, double *oa, d
       if (J[r] == cur) sum += A[r];
       else if (J[r] > cur) nxt = min(nxt, J[r]);
     }
-    sum = wave_sum(sum);
+    sum = wave_sum_dpp(sum);
     if (lane == 0) {
       oj[u] = cur;
       oa[u] = sum;
@@ -535,7 +532,7 @@ __device__ inline int gal_row_stream(int I, int E, SRC src, bool write, int o, i
       if (j == cur) sum += a;
       else if (j > cur) nxt = min(nxt, j);
     }
-    sum = wave_sum(sum);
+    sum = wave_sum_dpp(sum);
     if (write && lane == 0) {
       col[o + u] = cur;
       val[o + u] = sum;
