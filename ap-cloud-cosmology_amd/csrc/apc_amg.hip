@@ -54,7 +54,17 @@ struct AmgLevel {
   DBuf<unsigned char> pass;
   DBuf<double> x, b, r, t;
   DBuf<float> x32, b32, r32, t32;
+  // level 0 with AmgOptions::ap: the product A P (rows of A, columns the
+  // aggregates), sliced ELL like the coarse levels; ap_nnz counts its
+  // entries, ap_len the padded slices
+  bool ap = false;
+  long long ap_nnz = 0, ap_len = 0;
+  DBuf<int> apoff, apwid, apcol;
+  DBuf<double> apval;
+  DBuf<float> apval32;
   void release() {
+    ap = false;
+    apoff.release(); apwid.release(); apcol.release(); apval.release(); apval32.release();
     box.release(); pass.release();
     soff.release(); swid.release(); scol.release(); sval.release(); sval32.release();
     rp.release(); col.release(); val.release(); dinv.release(); val32.release();
@@ -105,6 +115,8 @@ struct AmgHier {
   DBuf<double> ev, smax, diag;
   DBuf<int> gcol;     // Galerkin entries grouped by coarse row: coarse column
   DBuf<double> gval;  // and value
+  DBuf<int> apsn;     // level 0's A P: real entries per slice
+  DBuf<long long> aptot;
   hipStream_t s = nullptr;  // the owning context's stream (graphs are launched on it)
   ~AmgHier() {
     for (auto &g : graphs) g.release(s);
@@ -113,7 +125,7 @@ struct AmgHier {
     frozen.release(); bstart.release(); goff.release(); ioff.release(); bshift.release();
     key2.release(); idx.release(); idx2.release(); sa.release(); sb.release(); flag.release();
     pos.release(); cnt.release(); ev.release(); smax.release(); diag.release();
-    gcol.release(); gval.release();
+    gcol.release(); gval.release(); apsn.release(); aptot.release();
   }
 };
 
@@ -671,6 +683,105 @@ __global__ void k_amg_convert(size_t n, const S *__restrict__ a, D *__restrict__
     b[i] = (D)a[i];
 }
 
+// Entries per row of level 0's A P: the diagonal and up to kApMax - 1 slots.
+constexpr int kApMax = 32;
+
+// Level 0's product A P with the aggregation's piecewise-constant
+// prolongation: for every aggregate J that row i's entries meet (the
+// diagonal's aggregate first, then the slots' in slot order), the sum of
+// a_ij over the columns j in J, added in slot order.  Stored like the coarse
+// levels' sliced ELL: 64-row slices (one wave each), slot-major, slice s at
+// s * KMAX * 64, padded to the slice's longest row with (the row's own
+// aggregate, 0).  Per slice: its offset, width and number of real entries;
+// val32 (optional) gets the FP32 copy.
+template <int KMAX>
+__global__ void __launch_bounds__(256) k_ap_build(Op<double> A, const int *__restrict__ agg,
+                                                  int *__restrict__ soff, int *__restrict__ wid,
+                                                  int *__restrict__ snnz, int *__restrict__ col,
+                                                  double *__restrict__ val,
+                                                  float *__restrict__ val32) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int s = i >> 6, lane = threadIdx.x & 63;
+  if (s >= (A.M + 63) >> 6) return;  // s is uniform over a wave: whole waves leave
+  const bool live = i < A.M;
+  int key[KMAX];
+  double v[KMAX];
+  int m[KMAX - 1];
+  key[0] = live ? agg[i] : -1;
+  v[0] = live ? A.d[i] : 0.0;
+  const double fi = live ? A.f[i] : 0.0;
+#pragma unroll
+  for (int q = 0; q < KMAX - 1; ++q)
+    m[q] = (live && q < A.lm) ? A.nbr[(size_t)q * A.M + i] : A.M;
+#pragma unroll
+  for (int q = 0; q < KMAX - 1; ++q) {
+    const bool in = m[q] < A.M;
+    key[q + 1] = in ? agg[m[q]] : -1;
+    v[q + 1] = in ? fi * A.s[(size_t)q * A.M + i] : 0.0;
+  }
+  const long long base = (long long)s * KMAX * 64 + lane;
+  int cnt = 0;
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) {
+    bool first = key[k] >= 0;
+#pragma unroll
+    for (int k2 = 0; k2 < k; ++k2) first = first && key[k2] != key[k];
+    if (first) {
+      double sum = v[k];
+#pragma unroll
+      for (int k2 = k + 1; k2 < KMAX; ++k2) sum += (key[k2] == key[k]) ? v[k2] : 0.0;
+      const long long e = base + (long long)cnt * 64;
+      col[e] = key[k];
+      val[e] = sum;
+      if (val32) val32[e] = (float)sum;
+      ++cnt;
+    }
+  }
+  int w = cnt, tot = cnt;
+  for (int off = 32; off > 0; off >>= 1) {
+    w = max(w, __shfl_xor(w, off, 64));
+    tot += __shfl_xor(tot, off, 64);
+  }
+  if (live)
+    for (int k = cnt; k < w; ++k) {
+      const long long e = base + (long long)k * 64;
+      col[e] = key[0];
+      val[e] = 0.0;
+      if (val32) val32[e] = 0.0f;
+    }
+  if (lane == 0) {
+    soff[s] = s * KMAX;
+    wid[s] = w;
+    snnz[s] = tot;
+  }
+}
+
+// out[0] = padded entries (64 per slot of every slice), out[1] = real ones
+__global__ void __launch_bounds__(1024) k_ap_total(int ns, const int *__restrict__ wid,
+                                                   const int *__restrict__ snnz,
+                                                   long long *__restrict__ out) {
+  __shared__ long long a[1024], b[1024];
+  long long sw = 0, sn = 0;
+  for (int s = threadIdx.x; s < ns; s += 1024) {
+    sw += wid[s];
+    sn += snnz[s];
+  }
+  a[threadIdx.x] = sw;
+  b[threadIdx.x] = sn;
+  __syncthreads();
+  for (int h = 512; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      a[threadIdx.x] += a[threadIdx.x + h];
+      b[threadIdx.x] += b[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = a[0] * 64;
+    out[1] = b[0];
+  }
+}
+
 // [A | I] for Gauss-Jordan, A from CSR plus shift/n everywhere.
 __global__ void k_gj_fill(Csr<double> A, double shift, double *__restrict__ g) {
   const int n = A.n;
@@ -827,6 +938,67 @@ __global__ void k_cy_prolong(int n, const int *__restrict__ agg, const T *__rest
   if (i >= n) return;
   const double a = (pass && pass[i]) ? alpha_pass : alpha;
   x[i] = (T)((double)x[i] + a * (double)xc[agg[i]]);
+}
+
+// (A P e)_i on level 0: the row's sliced-ELL entries (width uniform over
+// the wave), sixteen index and weight loads issued before their gathers
+template <class T>
+__device__ inline double ap_row(const Sell<T> &P, const T *__restrict__ e, int i) {
+  const int s = i >> 6;
+  const long long base = (long long)P.soff[s] * 64 + (i & 63);
+  const int w = P.wid[s];
+  double acc = 0.0;
+  for (int k0 = 0; k0 < w; k0 += 16) {
+    int cc[16];
+    T vv[16], ee[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (k0 + u < w) {
+        const long long q = base + (long long)(k0 + u) * 64;
+        cc[u] = P.col[q];
+        vv[u] = P.val[q];
+      }
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (k0 + u < w) ee[u] = e[cc[u]];
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (k0 + u < w) acc += (double)vv[u] * (double)ee[u];
+  }
+  return acc;
+}
+
+// Level 0 after a coarse correction e that another follows: x += alpha e[agg]
+// and r -= alpha (A P e), i.e. r = b - A x for the corrected x without
+// applying A again.
+template <class T>
+__global__ void __launch_bounds__(256) k_cy_ap_resid(int n, const int *__restrict__ agg, Sell<T> P,
+                                                     const T *__restrict__ e, T *__restrict__ x,
+                                                     T *__restrict__ r, double alpha,
+                                                     const GState *st) {
+  if (st && halted(st)) return;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  x[i] = (T)((double)x[i] + alpha * (double)e[agg[i]]);
+  r[i] = (T)((double)r[i] - alpha * ap_row(P, e, i));
+}
+
+// Level 0 after the last coarse correction: the correction and the first
+// post-smoothing step, x2 = x + alpha e[agg], xout = x2 + omega D^-1 (b - A x2)
+// with b - A x2 = r - alpha (A P e) (r = b - A x on entry).
+template <class T>
+__global__ void __launch_bounds__(256) k_cy_ap_sweep(int n, const int *__restrict__ agg, Sell<T> P,
+                                                     const T *__restrict__ e,
+                                                     const T *__restrict__ dinv,
+                                                     const T *__restrict__ xin,
+                                                     const T *__restrict__ r, T *__restrict__ xout,
+                                                     double alpha, double omega, const GState *st) {
+  if (st && halted(st)) return;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const T x2 = (T)((double)xin[i] + alpha * (double)e[agg[i]]);
+  const double res = (double)r[i] - alpha * ap_row(P, e, i);
+  xout[i] = (T)((double)x2 + omega * (double)dinv[i] * res);
 }
 
 // packed boxes, coarsest level: x_i = (inverse of row i's box block) b, one
@@ -1205,6 +1377,32 @@ static int ensure_work(apc_ctx *c, AmgLevel *L, int n, bool fp32) {
   return 0;
 }
 
+// Level 0's A P (k_ap_build) once its aggregates are known; fp32 adds the
+// FP32 copy.  One read of the totals (the timers' byte counts).
+static int build_ap(apc_ctx *c, AmgHier *h, const Op<double> &A, AmgLevel *L, bool fp32) {
+  const int n = A.M, ns = (n + 63) / 64;
+  const long long cap = (long long)ns * kApMax * 64;
+  APC_ALLOC(c, L->apoff, ns + 1);
+  APC_ALLOC(c, L->apwid, ns + 1);
+  APC_ALLOC(c, h->apsn, ns + 1);
+  APC_ALLOC(c, h->aptot, 2);
+  APC_ALLOC(c, L->apcol, cap + 1);
+  APC_ALLOC(c, L->apval, cap + 1);
+  if (fp32) APC_ALLOC(c, L->apval32, cap + 1);
+  hipLaunchKernelGGL((k_ap_build<kApMax>), dim3(nblocks(n, 256)), dim3(256), 0, c->s, A, L->agg.p,
+                     L->apoff.p, L->apwid.p, h->apsn.p, L->apcol.p, L->apval.p,
+                     fp32 ? L->apval32.p : nullptr);
+  hipLaunchKernelGGL(k_ap_total, dim3(1), dim3(1024), 0, c->s, ns, (const int *)L->apwid.p,
+                     (const int *)h->apsn.p, h->aptot.p);
+  long long tot[2] = {0, 0};
+  APC_HIP(c, hipMemcpyAsync(tot, h->aptot.p, sizeof(tot), hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  L->ap_len = tot[0];
+  L->ap_nnz = tot[1];
+  L->ap = true;
+  return 0;
+}
+
 static int build_sell(apc_ctx *c, AmgHier *h, AmgLevel *L) {
   const int n = L->n, ns = (n + 63) / 64;
   APC_ALLOC(c, L->swid, ns + 1);
@@ -1474,6 +1672,7 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
   AmgLevel *L0 = level(0);
   L0->n = A.M;
   L0->nnz = A.M * (A.lm + 1);
+  L0->ap = false;
   int r;
   {
     KScope ks1(c, "amg_setup_work0");
@@ -1532,6 +1731,11 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
       else r = galerkin_rows(c, h, CsrView{Csr<double>{Lf->rp.p, Lf->col.p, Lf->val.p, n}}, Lf, n, nc, Lc);
     }
     if (r) return r;
+    if (l == 0 && o.ap && A.lm + 1 <= kApMax) {
+      KScope ks1(c, "amg_setup_ap");
+      r = build_ap(c, h, A, Lf, fp32);
+      if (r) return r;
+    }
     {
       KScope ks1(c, "amg_setup_work");
       r = ensure_work(c, Lc, nc, fp32);
@@ -1595,6 +1799,13 @@ template <>
 LevelPtrs<float> ptrs<float>(AmgLevel *L) {
   return {L->x32.p, L->b32.p, L->r32.p, L->t32.p, L->dinv32.p, L->sval32.p};
 }
+
+template <class T>
+static const T *ap_val(AmgLevel *L);
+template <>
+const double *ap_val<double>(AmgLevel *L) { return L->apval.p; }
+template <>
+const float *ap_val<float>(AmgLevel *L) { return L->apval32.p; }
 
 template <class T>
 static const T *cinv_of(AmgHier *h);
@@ -1702,8 +1913,18 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
   LevelPtrs<T> Pc = ptrs<T>(Lc);
   T *r = P.r;
   const int gam = (l < o.wlevels) ? o.gamma : 1;
+  // level 0 with A P: after the first residual, every later one (and the
+  // first post-smoothing step's) comes from r -= alpha (A P) e
+  const bool apf = ell && L->ap;
+  const Sell<T> AP{L->apoff.p, L->apwid.p, L->apcol.p, apf ? ap_val<T>(L) : nullptr, n};
+  const double ap_bytes = apf ? (double)n * (4 + 4 * sT) + (double)L->ap_nnz * (4 + sT) +
+                                    (double)Lc->n * sT
+                              : 0.0;
+  int post_done = 0;
   for (int q = 0; q < gam; ++q) {
-    if (ell) {
+    if (apf && q > 0) {
+      // r = b - A cur already (k_cy_ap_resid after the previous correction)
+    } else if (ell) {
       KScope ks(c, f32 ? "amg_resid0_32" : "amg_resid0", (double)n * (lm0 * (4 + sT) + 5 * sT));
       hipLaunchKernelGGL((k_cy_resid<T, true>), dim3(g), dim3(256), 0, c->s, E, C, b, cur, r, st);
     } else if (fuse_first && q == 0) {
@@ -1738,7 +1959,18 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
     }
     int rr = cycle<T>(c, h, A0, l + 1, Pc.b, Pc.x, Pc.t, st);
     if (rr) return rr;
-    {
+    if (apf && q + 1 < gam) {
+      KScope ks(c, f32 ? "amg_ap_resid0_32" : "amg_ap_resid0", ap_bytes);
+      hipLaunchKernelGGL((k_cy_ap_resid<T>), dim3(g), dim3(256), 0, c->s, n, (const int *)L->agg.p,
+                         AP, (const T *)Pc.x, cur, r, o.alpha, st);
+    } else if (apf && o.nu >= 1) {
+      KScope ks(c, f32 ? "amg_ap_sweep0_32" : "amg_ap_sweep0", ap_bytes);
+      hipLaunchKernelGGL((k_cy_ap_sweep<T>), dim3(g), dim3(256), 0, c->s, n, (const int *)L->agg.p,
+                         AP, (const T *)Pc.x, dinv, (const T *)cur, (const T *)r, alt, o.alpha,
+                         o.omega, st);
+      std::swap(cur, alt);
+      post_done = 1;
+    } else {
       KScope ks(c, ell ? (f32 ? "amg_prolong0_32" : "amg_prolong0") : kProlongName[l],
                 ell ? (double)n * (4 + 3 * sT) : 0.0);
       // packed boxes: a stopped box's rows take the first coarse correction
@@ -1748,7 +1980,7 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
                          o.alpha, pass, q == 0 ? 1.0 : 0.0, st);
     }
   }
-  for (int s = 0; s < o.nu; ++s) {
+  for (int s = post_done; s < o.nu; ++s) {
     sweep(cur, alt);
     std::swap(cur, alt);
   }

@@ -668,6 +668,7 @@ static AmgOptions amg_options(apc_ctx *c) {
   a.rg8_rows = c->knob[APC_KNOB_AMG_RG8_ROWS];
   if (const char *v = getenv("APC_AMG_OMEGA")) a.omega = atof(v);
   if (const char *v = getenv("APC_AMG_ALPHA")) a.alpha = atof(v);
+  if (const char *v = getenv("APC_AMG_AP")) a.ap = atoi(v);
   if (const char *v = getenv("APC_AMG_THETA")) a.theta = atof(v);
   return a;
 }

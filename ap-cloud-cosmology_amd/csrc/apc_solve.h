@@ -133,6 +133,11 @@ struct AmgOptions {
   // latency-bound (a thread's chain of dependent gathers); at 256^3 (levels
   // of 611k, 58k, 6.7k rows) this took the Poisson solve from 348 to 304 ms.
   int g8_rows = 65536, g4_rows = 1 << 30, rg8_rows = 1 << 30;
+  // level 0: after a coarse correction x += alpha P e, update the residual
+  // by r -= alpha (A P) e instead of recomputing b - A x (and fold the
+  // post-smoothing step's residual the same way); A P holds one entry per
+  // aggregate a row's stencil meets, about a third of A's entries
+  int ap = 1;
 };
 
 // Build the hierarchy for A (FP64); `periodic` shifts the near-null constant

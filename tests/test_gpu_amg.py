@@ -123,3 +123,32 @@ def test_coarse_operators_are_galerkin_products(bc, sorted_path, monkeypatch):
             cs = Ac.indices[Ac.indptr[r]:Ac.indptr[r + 1]]
             assert np.all(np.diff(cs) > 0)
         prev = Ac
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_ap_cycle_equals_the_plain_cycle(prec, monkeypatch):
+    """Level 0's residual updates through A P (r -= alpha (A P) e after each
+    coarse correction, and the first post-smoothing step on that residual)
+    are the plain cycle in exact arithmetic: after one preconditioned
+    Krylov step (x = y0 M^-1 r0) the two potentials agree to rounding, and
+    the converged solves agree to their tolerance in as many steps (+-1)."""
+    one, full = {}, {}
+    rho_opts = apcloud.ksp(rtol=1e-12, restart=60, max_it=2000)
+    for ap in (0, 1):
+        monkeypatch.setenv("APC_AMG_AP", str(ap))
+        g, _, _, _ = _solve_pair("plummer", 30000, 1, 26)
+        info = g.solve(rho_opts, apcloud.ksp(rtol=1e-14, restart=60, max_it=1, precision=prec,
+                                             pc="amg"))
+        assert info.its_phi == 1, info.its_phi
+        one[ap] = g.fields()["phi"]
+        info = g.solve(rho_opts, apcloud.ksp(rtol=1e-11, restart=60, max_it=2000, precision=prec,
+                                             pc="amg"))
+        assert info.reason_phi > 0, info.reason_phi
+        full[ap] = (g.fields()["phi"], info.its_phi)
+        g.close()
+    err = np.abs(one[1] - one[0]).max() / np.abs(one[0]).max()
+    assert err < (1e-12 if prec == 0 else 1e-5), err
+    assert np.abs(one[0]).max() > 0
+    err = np.abs(full[1][0] - full[0][0]).max() / np.abs(full[0][0]).max()
+    assert err < 1e-8, err
+    assert abs(full[1][1] - full[0][1]) <= 1, (full[0][1], full[1][1])

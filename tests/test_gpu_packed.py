@@ -117,12 +117,21 @@ def test_packed_steps_match_each_box_alone(form):
         assert (rr > 0).all() and (rp > 0).all()
         assert info.its_phi == its_phi.max() and info.its_rho == its_rho.max()
         for b, g in enumerate(singles):
+            # every box's solve is its own GMRES: the iterations of the box alone
+            # (a reduction order apart).  A box of uniform density (every row
+            # a leaf of the same size and count; see below) has a potential
+            # right-hand side of rounding noise, exactly zero in one form and
+            # not in the other: its potential solve's count is not compared
+            g.octree_build()
+            g.select()
+            gc = g.comp()
+            M = gc["nc"] + gc["nv"]
+            uniform = np.ptp(gc["np"][:M] / gc["h"][:M] ** 3) == 0
             _, _, gi, _ = g.step(1.0 / len(bx[b][0]), cpot, cf, cv, cx, dt, opts)
             assert gi.reason_rho > 0 and gi.reason_phi > 0
-            # every box's solve is its own GMRES: the iterations of the box alone
-            # (a reduction order apart)
-            assert abs(its_rho[b] - gi.its_rho) <= 1 and abs(its_phi[b] - gi.its_phi) <= 1, \
-                (b, its_rho[b], gi.its_rho, its_phi[b], gi.its_phi)
+            assert abs(its_rho[b] - gi.its_rho) <= 1, (b, its_rho[b], gi.its_rho)
+            assert uniform or abs(its_phi[b] - gi.its_phi) <= 1, \
+                (b, its_phi[b], gi.its_phi)
         print("step %d: packed its %d/%d" % (k, info.its_rho, info.its_phi), flush=True)
         t += dt
     Q = P.particles()
