@@ -62,9 +62,13 @@ struct AmgLevel {
   DBuf<int> apoff, apwid, apcol;
   DBuf<double> apval;
   DBuf<float> apval32;
+  // packed boxes, coarse levels: the pass flag of each aggregate's rows (an
+  // aggregate never joins rows of two boxes, so its rows share the flag)
+  DBuf<unsigned char> apass;
   void release() {
     ap = false;
     apoff.release(); apwid.release(); apcol.release(); apval.release(); apval32.release();
+    apass.release();
     box.release(); pass.release();
     soff.release(); swid.release(); scol.release(); sval.release(); sval32.release();
     rp.release(); col.release(); val.release(); dinv.release(); val32.release();
@@ -117,6 +121,8 @@ struct AmgHier {
   DBuf<double> gval;  // and value
   DBuf<int> apsn;     // level 0's A P: real entries per slice
   DBuf<long long> aptot;
+  DBuf<int> apiota, aprp, apcnt, apc;  // coarse levels' A P: rows as one-row groups, CSR
+  DBuf<double> apv, apdiag;
   hipStream_t s = nullptr;  // the owning context's stream (graphs are launched on it)
   ~AmgHier() {
     for (auto &g : graphs) g.release(s);
@@ -126,6 +132,8 @@ struct AmgHier {
     key2.release(); idx.release(); idx2.release(); sa.release(); sb.release(); flag.release();
     pos.release(); cnt.release(); ev.release(); smax.release(); diag.release();
     gcol.release(); gval.release(); apsn.release(); aptot.release();
+    apiota.release(); aprp.release(); apcnt.release(); apc.release(); apv.release();
+    apdiag.release();
   }
 };
 
@@ -661,18 +669,21 @@ __global__ void k_sell_width(int n, const int *__restrict__ rp, int *__restrict_
   if (lane == 0 && s * 64 < n) wid[s] = w;
 }
 
+// padding entries: value 0 and column padc[r] (the row itself without padc;
+// a rectangular matrix passes a valid column of each row)
 __global__ void k_sell_fill(int n, const int *__restrict__ rp, const int *__restrict__ col,
                             const double *__restrict__ val, const int *__restrict__ soff,
                             const int *__restrict__ wid, int *__restrict__ scol,
-                            double *__restrict__ sval) {
+                            double *__restrict__ sval, const int *__restrict__ padc) {
   const int r = blockIdx.x * 256 + threadIdx.x;
   if (r >= n) return;
   const int s = r >> 6, lane = r & 63;
   const long long base = (long long)soff[s] * 64 + lane;
   const int b = rp[r], e = rp[r + 1], w = wid[s];
+  const int pc = padc ? padc[r] : r;
   for (int k = 0; k < w; ++k) {
     const bool real = b + k < e;
-    scol[base + (long long)k * 64] = real ? col[b + k] : r;
+    scol[base + (long long)k * 64] = real ? col[b + k] : pc;
     sval[base + (long long)k * 64] = real ? val[b + k] : 0.0;
   }
 }
@@ -1136,6 +1147,64 @@ __global__ void __launch_bounds__(256) k_cy_restrict_g(int nc, const int *__rest
   if (sub == 0) bc[I] = (T)s;
 }
 
+// sum over J of (A P)_iJ a_J e_J on a coarse level (G lanes per row): a_J =
+// alpha, or alpha_pass where apass flags aggregate J (packed boxes)
+template <class T, int G>
+__device__ inline double ap_apply_g(const Sell<T> &P, const T *__restrict__ e,
+                                    const unsigned char *__restrict__ apass, double alpha,
+                                    double alpha_pass, int i, int sub) {
+  if (!apass) return alpha * sell_apply_g<T, G>(P, e, i, sub);
+  return sell_apply_gx<T, G>(
+      P, [&](int J) { return (T)((apass[J] ? alpha_pass : alpha) * (double)e[J]); }, i, sub);
+}
+
+// Coarse level after a coarse correction that another follows: x += a_i
+// e[agg_i] and r -= (A P) (a e) (k_cy_ap_resid with G lanes per row and the
+// packed boxes' pass rows)
+template <class T, int G>
+__global__ void __launch_bounds__(256) k_cy_ap_resid_g(Sell<T> P, const int *__restrict__ agg,
+                                                       const T *__restrict__ e, T *__restrict__ x,
+                                                       T *__restrict__ r, double alpha,
+                                                       const unsigned char *__restrict__ pass,
+                                                       const unsigned char *__restrict__ apass,
+                                                       double alpha_pass, const GState *st) {
+  if (st && halted(st)) return;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int i = t / G, sub = t % G;
+  if (i >= P.n) return;  // whole groups (G divides 64)
+  const double ape = ap_apply_g<T, G>(P, e, apass, alpha, alpha_pass, i, sub);
+  if (sub == 0) {
+    const double a = (pass && pass[i]) ? alpha_pass : alpha;
+    x[i] = (T)((double)x[i] + a * (double)e[agg[i]]);
+    r[i] = (T)((double)r[i] - ape);
+  }
+}
+
+// Coarse level after the last coarse correction: the correction and the
+// first post-smoothing step (k_cy_ap_sweep with G lanes per row)
+template <class T, int G>
+__global__ void __launch_bounds__(256) k_cy_ap_sweep_g(Sell<T> P, const int *__restrict__ agg,
+                                                       const T *__restrict__ e,
+                                                       const T *__restrict__ dinv,
+                                                       const T *__restrict__ xin,
+                                                       const T *__restrict__ r,
+                                                       T *__restrict__ xout, double alpha,
+                                                       double omega,
+                                                       const unsigned char *__restrict__ pass,
+                                                       const unsigned char *__restrict__ apass,
+                                                       double alpha_pass, const GState *st) {
+  if (st && halted(st)) return;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int i = t / G, sub = t % G;
+  if (i >= P.n) return;
+  const double ape = ap_apply_g<T, G>(P, e, apass, alpha, alpha_pass, i, sub);
+  if (sub == 0) {
+    const double a = (pass && pass[i]) ? alpha_pass : alpha;
+    const T x2 = (T)((double)xin[i] + a * (double)e[agg[i]]);
+    xout[i] = (T)((double)x2 + omega * (double)dinv[i] * ((double)r[i] - ape));
+  }
+}
+
 // ---------------------------------------------------------------- host side
 static int ilog2_ceil(uint64_t v) {
   int b = 0;
@@ -1403,22 +1472,96 @@ static int build_ap(apc_ctx *c, AmgHier *h, const Op<double> &A, AmgLevel *L, bo
   return 0;
 }
 
-static int build_sell(apc_ctx *c, AmgHier *h, AmgLevel *L) {
-  const int n = L->n, ns = (n + 63) / 64;
-  APC_ALLOC(c, L->swid, ns + 1);
-  APC_ALLOC(c, L->soff, ns + 1);
-  hipLaunchKernelGGL(k_sell_width, dim3(nblocks(ns, 4)), dim3(256), 0, c->s, n, L->rp.p, L->swid.p);
+// Sliced ELL (64-row slices, slot-major, each padded to its longest row)
+// from a CSR matrix of n rows; padding takes column padc[r] (nullptr: r).
+static int sell_from_csr(apc_ctx *c, AmgHier *h, int n, const int *rp, const int *col,
+                         const double *val, DBuf<int> &soff, DBuf<int> &swid, DBuf<int> &scol,
+                         DBuf<double> &sval, long long *len, const int *padc) {
+  const int ns = (n + 63) / 64;
+  APC_ALLOC(c, swid, ns + 1);
+  APC_ALLOC(c, soff, ns + 1);
+  hipLaunchKernelGGL(k_sell_width, dim3(nblocks(ns, 4)), dim3(256), 0, c->s, n, rp, swid.p);
   APC_ALLOC(c, h->cnt, 4);
-  int r = scan_exclusive(c, L->swid.p, L->soff.p, ns, h->cnt.p);
+  int r = scan_exclusive(c, swid.p, soff.p, ns, h->cnt.p);
   if (r) return r;
   int tot = 0;
   APC_HIP(c, hipMemcpyAsync(&tot, h->cnt.p, sizeof(int), hipMemcpyDeviceToHost, c->s));
   APC_HIP(c, hipStreamSynchronize(c->s));
-  L->sell_len = (long long)tot * 64;
-  APC_ALLOC(c, L->scol, L->sell_len + 1);
-  APC_ALLOC(c, L->sval, L->sell_len + 1);
-  hipLaunchKernelGGL(k_sell_fill, dim3(nblocks(n, 256)), dim3(256), 0, c->s, n, L->rp.p, L->col.p,
-                     L->val.p, L->soff.p, L->swid.p, L->scol.p, L->sval.p);
+  *len = (long long)tot * 64;
+  APC_ALLOC(c, scol, *len + 1);
+  APC_ALLOC(c, sval, *len + 1);
+  hipLaunchKernelGGL(k_sell_fill, dim3(nblocks(n, 256)), dim3(256), 0, c->s, n, rp, col, val,
+                     soff.p, swid.p, scol.p, sval.p, padc);
+  return 0;
+}
+
+static int build_sell(apc_ctx *c, AmgHier *h, AmgLevel *L) {
+  return sell_from_csr(c, h, L->n, L->rp.p, L->col.p, L->val.p, L->soff, L->swid, L->scol,
+                       L->sval, &L->sell_len, nullptr);
+}
+
+__global__ void k_ap_iota(int n, int *__restrict__ a) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i <= n) a[i] = i;
+}
+
+// apass[agg[i]] = pass[i] (every row of an aggregate carries the same flag)
+__global__ void k_ap_pass(int n, const int *__restrict__ agg, const unsigned char *__restrict__ pass,
+                          unsigned char *__restrict__ apass) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) apass[agg[i]] = pass[i];
+}
+
+// A coarse level's A P (level L, CSR rows rp/col/val, aggregates agg into
+// nc coarse rows): each row is its own group in the Galerkin row reduction
+// (k_gal_rows), so its entries are summed per aggregate, distinct aggregates
+// ascending; then sliced ELL like the level's own matrix.
+static int build_ap_coarse(apc_ctx *c, AmgHier *h, AmgLevel *L, int nc, bool fp32) {
+  const int n = L->n;
+  const size_t nnz = (size_t)L->nnz;
+  APC_ALLOC(c, h->gcol, nnz + 1);
+  APC_ALLOC(c, h->gval, nnz + 1);
+  APC_ALLOC(c, h->apiota, (size_t)n + 1);
+  APC_ALLOC(c, h->apcnt, (size_t)n + 1);
+  APC_ALLOC(c, h->aprp, (size_t)n + 1);
+  APC_ALLOC(c, h->apdiag, (size_t)n + 1);
+  APC_ALLOC(c, h->cnt, 4);
+  const unsigned g = nblocks(n + 1, 256);
+  hipLaunchKernelGGL(k_ap_iota, dim3(g), dim3(256), 0, c->s, n, h->apiota.p);
+  const CsrView v{Csr<double>{L->rp.p, L->col.p, L->val.p, n}};
+  hipLaunchKernelGGL((k_gal_emit<CsrView>), dim3(nblocks(n, 256)), dim3(256), 0, c->s, v, n,
+                     L->agg.p, (const int *)L->rp.p, h->gcol.p, h->gval.p);
+  const unsigned grid = (unsigned)std::min(n, 1 << 16);
+  hipLaunchKernelGGL(k_gal_rows, dim3(grid), dim3(64), 0, c->s, n, (const int *)h->apiota.p,
+                     (const int *)L->rp.p, h->gcol.p, h->gval.p, 0, h->apcnt.p,
+                     (const int *)nullptr, (int *)nullptr, (double *)nullptr, h->apdiag.p);
+  int r = scan_exclusive(c, h->apcnt.p, h->aprp.p, n, h->cnt.p);
+  if (r) return r;
+  int tot = 0;
+  APC_HIP(c, hipMemcpyAsync(&tot, h->cnt.p, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  APC_HIP(c, hipMemcpyAsync(h->aprp.p + n, h->cnt.p, sizeof(int), hipMemcpyDeviceToDevice, c->s));
+  APC_ALLOC(c, h->apc, (size_t)tot + 1);
+  APC_ALLOC(c, h->apv, (size_t)tot + 1);
+  hipLaunchKernelGGL(k_gal_rows, dim3(grid), dim3(64), 0, c->s, n, (const int *)h->apiota.p,
+                     (const int *)L->rp.p, h->gcol.p, h->gval.p, 1, (int *)nullptr,
+                     (const int *)h->aprp.p, h->apc.p, h->apv.p, h->apdiag.p);
+  // padding: the row's own aggregate (a column of the coarse vector)
+  r = sell_from_csr(c, h, n, h->aprp.p, h->apc.p, h->apv.p, L->apoff, L->apwid, L->apcol,
+                    L->apval, &L->ap_len, L->agg.p);
+  if (r) return r;
+  if (fp32) {
+    APC_ALLOC(c, L->apval32, L->ap_len + 1);
+    hipLaunchKernelGGL((k_amg_convert<double, float>), dim3(1024), dim3(256), 0, c->s,
+                       (size_t)L->ap_len, L->apval.p, L->apval32.p);
+  }
+  if (L->pass.p) {
+    APC_ALLOC(c, L->apass, (size_t)nc + 1);
+    hipLaunchKernelGGL(k_ap_pass, dim3(nblocks(n, 256)), dim3(256), 0, c->s, n, L->agg.p,
+                       (const unsigned char *)L->pass.p, L->apass.p);
+  }
+  L->ap_nnz = tot;
+  L->ap = true;
   return 0;
 }
 
@@ -1683,6 +1826,7 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
   int k = 0;  // deepest level built by this setup
   for (int l = 0;; ++l) {
     AmgLevel *Lf = level(l);
+    Lf->ap = false;
     // stop at the dense size (but coarsen level 0 at least once) or the cap
     if (packed ? (l > 0 && all_frozen()) : (l > 0 && n <= o.coarse_max)) break;
     if (l + 1 >= o.max_levels) break;
@@ -1731,9 +1875,9 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
       else r = galerkin_rows(c, h, CsrView{Csr<double>{Lf->rp.p, Lf->col.p, Lf->val.p, n}}, Lf, n, nc, Lc);
     }
     if (r) return r;
-    if (l == 0 && o.ap && A.lm + 1 <= kApMax) {
+    if (o.ap && (l > 0 || A.lm + 1 <= kApMax)) {
       KScope ks1(c, "amg_setup_ap");
-      r = build_ap(c, h, A, Lf, fp32);
+      r = (l == 0) ? build_ap(c, h, A, Lf, fp32) : build_ap_coarse(c, h, Lf, nc, fp32);
       if (r) return r;
     }
     {
@@ -1833,7 +1977,16 @@ static const char *const kProlongName[] = {
     "amg_prolong_l0", "amg_prolong_l1", "amg_prolong_l2", "amg_prolong_l3",
     "amg_prolong_l4", "amg_prolong_l5", "amg_prolong_l6", "amg_prolong_l7",
     "amg_prolong_l8", "amg_prolong_l9", "amg_prolong_l10", "amg_prolong_l11"};
+static const char *const kApResidName[] = {
+    "amg_ap_resid_l0", "amg_ap_resid_l1", "amg_ap_resid_l2", "amg_ap_resid_l3",
+    "amg_ap_resid_l4", "amg_ap_resid_l5", "amg_ap_resid_l6", "amg_ap_resid_l7",
+    "amg_ap_resid_l8", "amg_ap_resid_l9", "amg_ap_resid_l10", "amg_ap_resid_l11"};
+static const char *const kApSweepName[] = {
+    "amg_ap_sweep_l0", "amg_ap_sweep_l1", "amg_ap_sweep_l2", "amg_ap_sweep_l3",
+    "amg_ap_sweep_l4", "amg_ap_sweep_l5", "amg_ap_sweep_l6", "amg_ap_sweep_l7",
+    "amg_ap_sweep_l8", "amg_ap_sweep_l9", "amg_ap_sweep_l10", "amg_ap_sweep_l11"};
 static_assert(sizeof(kSweepName) / sizeof(kSweepName[0]) >= 12, "one name per level");
+static_assert(sizeof(kApSweepName) / sizeof(kApSweepName[0]) >= 12, "one name per level");
 
 template <class T>
 static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *x, T *tmp,
@@ -1913,9 +2066,9 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
   LevelPtrs<T> Pc = ptrs<T>(Lc);
   T *r = P.r;
   const int gam = (l < o.wlevels) ? o.gamma : 1;
-  // level 0 with A P: after the first residual, every later one (and the
-  // first post-smoothing step's) comes from r -= alpha (A P) e
-  const bool apf = ell && L->ap;
+  // with A P: after the first residual, every later one (and the first
+  // post-smoothing step's) comes from r -= alpha (A P) e
+  const bool apf = L->ap;
   const Sell<T> AP{L->apoff.p, L->apwid.p, L->apcol.p, apf ? ap_val<T>(L) : nullptr, n};
   const double ap_bytes = apf ? (double)n * (4 + 4 * sT) + (double)L->ap_nnz * (4 + sT) +
                                     (double)Lc->n * sT
@@ -1959,23 +2112,43 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
     }
     int rr = cycle<T>(c, h, A0, l + 1, Pc.b, Pc.x, Pc.t, st);
     if (rr) return rr;
+    // packed boxes: a stopped box's rows take the first coarse correction
+    // unscaled and none after it (its coarsest solve, passed through once)
+    const unsigned char *pass = (h->nbox > 1 && l > 0) ? L->pass.p : nullptr;
+    const unsigned char *apass = pass ? L->apass.p : nullptr;
+    const double apass_alpha = q == 0 ? 1.0 : 0.0;
+    const unsigned gg = nblocks((long)n * gl, 256);
     if (apf && q + 1 < gam) {
-      KScope ks(c, f32 ? "amg_ap_resid0_32" : "amg_ap_resid0", ap_bytes);
-      hipLaunchKernelGGL((k_cy_ap_resid<T>), dim3(g), dim3(256), 0, c->s, n, (const int *)L->agg.p,
-                         AP, (const T *)Pc.x, cur, r, o.alpha, st);
+      if (ell) {
+        KScope ks(c, f32 ? "amg_ap_resid0_32" : "amg_ap_resid0", ap_bytes);
+        hipLaunchKernelGGL((k_cy_ap_resid<T>), dim3(g), dim3(256), 0, c->s, n,
+                           (const int *)L->agg.p, AP, (const T *)Pc.x, cur, r, o.alpha, st);
+      } else {
+        KScope ks(c, kApResidName[l]);
+        auto *kern = gl == 8 ? k_cy_ap_resid_g<T, 8> : gl == 4 ? k_cy_ap_resid_g<T, 4>
+                                                               : k_cy_ap_resid_g<T, 1>;
+        hipLaunchKernelGGL(kern, dim3(gg), dim3(256), 0, c->s, AP, (const int *)L->agg.p,
+                           (const T *)Pc.x, cur, r, o.alpha, pass, apass, apass_alpha, st);
+      }
     } else if (apf && o.nu >= 1) {
-      KScope ks(c, f32 ? "amg_ap_sweep0_32" : "amg_ap_sweep0", ap_bytes);
-      hipLaunchKernelGGL((k_cy_ap_sweep<T>), dim3(g), dim3(256), 0, c->s, n, (const int *)L->agg.p,
-                         AP, (const T *)Pc.x, dinv, (const T *)cur, (const T *)r, alt, o.alpha,
-                         o.omega, st);
+      if (ell) {
+        KScope ks(c, f32 ? "amg_ap_sweep0_32" : "amg_ap_sweep0", ap_bytes);
+        hipLaunchKernelGGL((k_cy_ap_sweep<T>), dim3(g), dim3(256), 0, c->s, n,
+                           (const int *)L->agg.p, AP, (const T *)Pc.x, dinv, (const T *)cur,
+                           (const T *)r, alt, o.alpha, o.omega, st);
+      } else {
+        KScope ks(c, kApSweepName[l]);
+        auto *kern = gl == 8 ? k_cy_ap_sweep_g<T, 8> : gl == 4 ? k_cy_ap_sweep_g<T, 4>
+                                                               : k_cy_ap_sweep_g<T, 1>;
+        hipLaunchKernelGGL(kern, dim3(gg), dim3(256), 0, c->s, AP, (const int *)L->agg.p,
+                           (const T *)Pc.x, dinv, (const T *)cur, (const T *)r, alt, o.alpha,
+                           o.omega, pass, apass, apass_alpha, st);
+      }
       std::swap(cur, alt);
       post_done = 1;
     } else {
       KScope ks(c, ell ? (f32 ? "amg_prolong0_32" : "amg_prolong0") : kProlongName[l],
                 ell ? (double)n * (4 + 3 * sT) : 0.0);
-      // packed boxes: a stopped box's rows take the first coarse correction
-      // unscaled and none after it (its coarsest solve, passed through once)
-      const unsigned char *pass = (h->nbox > 1 && l > 0) ? L->pass.p : nullptr;
       hipLaunchKernelGGL((k_cy_prolong<T>), dim3(g), dim3(256), 0, c->s, n, L->agg.p, Pc.x, cur,
                          o.alpha, pass, q == 0 ? 1.0 : 0.0, st);
     }
