@@ -268,17 +268,17 @@ __global__ void __launch_bounds__(256) k_grad_adjoint(
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= M) return;
   const FitRow f = fit_row(cx, cy, cz, ch, dist, M, i, lm);
-  double R[45];
+  double mo[kMoments];
 #pragma unroll
-  for (int q = 0; q < 45; ++q) R[q] = 0.0;
+  for (int q = 0; q < kMoments; ++q) mo[q] = 0.0;
   for_slots(M, lm, i, nbr, dist, cx, cy, cz, NoExtra{},
             [&](int, int, double d, double x, double y, double z, double) {
-              double r[9];
-              const double w = fit_slot(G, f, d, x, y, z, r);
-#pragma unroll
-              for (int k = 0; k < 9; ++k) r[k] *= w;
-              gram_add(R, r);
+              double p[3];
+              const double w = fit_offset(G, f, d, x, y, z, p);
+              moments_add(mo, w * w, p[0], p[1], p[2]);
             });
+  double R[45];
+  gram_from_moments(mo, R);
   if (chol_inplace(R)) { status[0] = APC_ERR_RANK; status[1] = i; return; }
   const double ih2 = f.ih * f.ih;
   double dcf[9];

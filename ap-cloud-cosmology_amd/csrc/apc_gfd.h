@@ -37,12 +37,53 @@ __device__ inline void basis_row(double p1, double p2, double p3, double *r) {
   r[6] = p1 * p2; r[7] = p1 * p3; r[8] = p2 * p3;
 }
 
-// Add the outer product r r^T to the Gram matrix R (upper, packed as RI).
-__device__ inline void gram_add(double *R, const double *r) {
-#pragma unroll
-  for (int k = 0; k < 9; ++k)
-#pragma unroll
-    for (int q = k; q < 9; ++q) R[RI(k, q)] += r[k] * r[q];
+// The Gram matrix of a fit from moments.  Every basis entry is a monomial of
+// degree 1 or 2 in the scaled offset p, so every Gram entry sum over slots of
+// W b_k b_l (W = w^2) is a weighted moment sum W p1^a p2^b p3^c of degree 2
+// to 4, times 1, 1/2 or 1/4: 31 distinct sums instead of 45 (fewer registers
+// held across the slot loop, fewer products per slot).
+// Moment order: degree 2 (200 110 101 020 011 002), degree 3 (300 210 201 120
+// 111 102 030 021 012 003), degree 4 (400 310 301 220 211 202 130 121 112
+// 103 040 031 022 013 004).
+constexpr int kMoments = 31;
+
+__device__ inline void moments_add(double *m, double W, double p1, double p2, double p3) {
+  const double q11 = p1 * p1, q12 = p1 * p2, q13 = p1 * p3;
+  const double q22 = p2 * p2, q23 = p2 * p3, q33 = p3 * p3;
+  const double w11 = W * q11, w12 = W * q12, w13 = W * q13;
+  const double w22 = W * q22, w23 = W * q23, w33 = W * q33;
+  m[0] += w11; m[1] += w12; m[2] += w13; m[3] += w22; m[4] += w23; m[5] += w33;
+  m[6] += w11 * p1; m[7] += w11 * p2; m[8] += w11 * p3; m[9] += w22 * p1;
+  m[10] += w12 * p3; m[11] += w33 * p1; m[12] += w22 * p2; m[13] += w22 * p3;
+  m[14] += w33 * p2; m[15] += w33 * p3;
+  m[16] += w11 * q11; m[17] += w11 * q12; m[18] += w11 * q13; m[19] += w11 * q22;
+  m[20] += w11 * q23; m[21] += w11 * q33; m[22] += w12 * q22; m[23] += w12 * q23;
+  m[24] += w12 * q33; m[25] += w13 * q33; m[26] += w22 * q22; m[27] += w22 * q23;
+  m[28] += w22 * q33; m[29] += w23 * q33; m[30] += w33 * q33;
+}
+
+// R (upper, packed as RI) = the Gram matrix of the basis
+// [p1, p2, p3, p1^2/2, p2^2/2, p3^2/2, p1p2, p1p3, p2p3] from its moments
+__device__ inline void gram_from_moments(const double *m, double *R) {
+  const double h = 0.5, q = 0.25;
+  R[RI(0, 0)] = m[0];      R[RI(0, 1)] = m[1];      R[RI(0, 2)] = m[2];
+  R[RI(0, 3)] = h * m[6];  R[RI(0, 4)] = h * m[9];  R[RI(0, 5)] = h * m[11];
+  R[RI(0, 6)] = m[7];      R[RI(0, 7)] = m[8];      R[RI(0, 8)] = m[10];
+  R[RI(1, 1)] = m[3];      R[RI(1, 2)] = m[4];      R[RI(1, 3)] = h * m[7];
+  R[RI(1, 4)] = h * m[12]; R[RI(1, 5)] = h * m[14]; R[RI(1, 6)] = m[9];
+  R[RI(1, 7)] = m[10];     R[RI(1, 8)] = m[13];
+  R[RI(2, 2)] = m[5];      R[RI(2, 3)] = h * m[8];  R[RI(2, 4)] = h * m[13];
+  R[RI(2, 5)] = h * m[15]; R[RI(2, 6)] = m[10];     R[RI(2, 7)] = m[11];
+  R[RI(2, 8)] = m[14];
+  R[RI(3, 3)] = q * m[16]; R[RI(3, 4)] = q * m[19]; R[RI(3, 5)] = q * m[21];
+  R[RI(3, 6)] = h * m[17]; R[RI(3, 7)] = h * m[18]; R[RI(3, 8)] = h * m[20];
+  R[RI(4, 4)] = q * m[26]; R[RI(4, 5)] = q * m[28]; R[RI(4, 6)] = h * m[22];
+  R[RI(4, 7)] = h * m[23]; R[RI(4, 8)] = h * m[27];
+  R[RI(5, 5)] = q * m[30]; R[RI(5, 6)] = h * m[24]; R[RI(5, 7)] = h * m[25];
+  R[RI(5, 8)] = h * m[29];
+  R[RI(6, 6)] = m[19];     R[RI(6, 7)] = m[20];     R[RI(6, 8)] = m[23];
+  R[RI(7, 7)] = m[21];     R[RI(7, 8)] = m[24];
+  R[RI(8, 8)] = m[28];
 }
 
 // Cholesky factorisation in place: the Gram matrix B^T B becomes the upper
@@ -155,6 +196,15 @@ __device__ inline double weight_of(double d, double inv_md2) {
   constexpr double E4 = 0x1.2c155b8213cf4p-6;  // exp(-4) rounded to double
   constexpr double INV_NORM = 1.0 / (1.0 - E4);
   return (exp(-(d * d) * inv_md2) - E4) * INV_NORM;
+}
+
+// Neighbour slot of a fit: the weight and the wrapped offset (x_m - x_i) / h.
+__device__ inline double fit_offset(const Geo &G, const FitRow &f, double d, double x, double y,
+                                    double z, double *p) {
+  p[0] = pwrap_s(x - f.xi, G.len[0], G.bc) * f.ih;
+  p[1] = pwrap_s(y - f.yi, G.len[1], G.bc) * f.ih;
+  p[2] = pwrap_s(z - f.zi, G.len[2], G.bc) * f.ih;
+  return weight_of(d, f.inv_md2);
 }
 
 // Neighbour slot of a fit: the weight and the unweighted basis row of the

@@ -7,9 +7,9 @@
 //   Laplacian weights  s_j = w_j^2 * basis_j . y,  y = R^-1 R^-T (e3+e4+e5)
 //   (the reference's X3+X4+X5 column sums of X = (WB)^+ W), and
 //   gradient/Hessian   c = R^-1 R^-T B^T b (the semi-normal equations),
-//   R from the Cholesky factorisation of the Gram matrix B^T B, accumulated
-//   in one pass over the neighbours.  Both agree with the reference's QR to
-//   rounding of order cond(B)^2 u.
+//   R from the Cholesky factorisation of the Gram matrix B^T B, formed from
+//   31 weighted moments of the neighbours' offsets accumulated in one pass.
+//   Both agree with the reference's QR to rounding of order cond(B)^2 u.
 // Matrices are stored slot-major (slot j of all rows contiguous) so the
 // SpMV and the stencil writes coalesce.
 #include <algorithm>
@@ -51,7 +51,7 @@ __device__ inline void fit_max(const FitQueue &Q, double est, double *ws) {
   if (threadIdx.x == 0 && m > 0.0) atomicMax(Q.maxbits, (unsigned long long)__double_as_longlong(m));
 }
 
-__global__ void __launch_bounds__(256) k_stencil(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_stencil(
     int M, int lm, Geo G, int pin_row0, Boxes B, double c_dens, const double *__restrict__ cx,
     const double *__restrict__ cy, const double *__restrict__ cz, const double *__restrict__ ch,
     const int *__restrict__ cnp, const int *__restrict__ nbr, const double *__restrict__ dist,
@@ -69,17 +69,17 @@ __global__ void __launch_bounds__(256) k_stencil(
     // factor is as good as one through a QR of B (error cond(B)^2 u, small
     // below the threshold): nine square roots per row instead of a rotation
     // per basis entry.  Rows above the threshold go to k_fit_qr.
-    double R[45];  // the Gram matrix (upper, packed), then its factor in place
+    double mo[kMoments];
 #pragma unroll
-    for (int q = 0; q < 45; ++q) R[q] = 0.0;
+    for (int q = 0; q < kMoments; ++q) mo[q] = 0.0;
     for_slots(M, lm, i, nbr, dist, cx, cy, cz, NoExtra{},
               [&](int, int, double d, double x, double y, double z, double) {
-                double r[9];
-                const double w = fit_slot(G, f, d, x, y, z, r);
-#pragma unroll
-                for (int k = 0; k < 9; ++k) r[k] *= w;
-                gram_add(R, r);
+                double p[3];
+                const double w = fit_offset(G, f, d, x, y, z, p);
+                moments_add(mo, w * w, p[0], p[1], p[2]);
               });
+    double R[45];  // the Gram matrix (upper, packed), then its factor in place
+    gram_from_moments(mo, R);
     const bool deficient = chol_inplace(R);
     rhs1[i] = 1.0 * cnp[i] / ch[i] / ch[i] / ch[i] * (c_dens * box_mass(B, i));
     if (!fit_needs_qr(Q, i, R, deficient, &est)) {
@@ -158,22 +158,24 @@ __global__ void __launch_bounds__(256) k_gradient(
     // factorisation of the Gram matrix: one pass over the neighbours, nine
     // square roots per row.  The adjoint (k_grad_adjoint) transposes this
     // form.  Rows above the condition threshold go to k_fit_qr.
-    double R[45], Btb[9];
+    double mo[kMoments], Btb[9];
 #pragma unroll
-    for (int q = 0; q < 45; ++q) R[q] = 0.0;
+    for (int q = 0; q < kMoments; ++q) mo[q] = 0.0;
 #pragma unroll
     for (int q = 0; q < 9; ++q) Btb[q] = 0.0;
     for_slots(M, lm, i, nbr, dist, cx, cy, cz, PhiOrBv{phi, bv, M},
               [&](int, int, double d, double x, double y, double z, double v) {
-                double r[9];
-                const double w = fit_slot(G, f, d, x, y, z, r);
+                double p[3], r[9];
+                const double w = fit_offset(G, f, d, x, y, z, p);
+                const double W = w * w;
+                moments_add(mo, W, p[0], p[1], p[2]);
+                basis_row(p[0], p[1], p[2], r);
+                const double t = W * (v - p4) * c_field;
 #pragma unroll
-                for (int k = 0; k < 9; ++k) r[k] *= w;
-                const double rb = w * (v - p4) * c_field;
-                gram_add(R, r);
-#pragma unroll
-                for (int k = 0; k < 9; ++k) Btb[k] += r[k] * rb;
+                for (int k = 0; k < 9; ++k) Btb[k] += r[k] * t;
               });
+    double R[45];
+    gram_from_moments(mo, R);
     const bool deficient = chol_inplace(R);
     if (!fit_needs_qr(Q, i, R, deficient, &est)) {
       double cf[9];
