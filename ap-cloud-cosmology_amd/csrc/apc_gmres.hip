@@ -706,11 +706,19 @@ static int gmres(apc_ctx *c, const Op<double> &A, const double *b, double *x,
   }
   // FP32 copies of the operator for the inner cycles
   const size_t nnz = (size_t)M * A.lm;
+  const float *s32_old = S->s32.p;
   APC_ALLOC(c, S->s32, nnz + 1);
   APC_ALLOC(c, S->d32, M + 1);
   APC_ALLOC(c, S->f32, M + 1);
   APC_ALLOC(c, S->dinv32, M + 1);
-  hipLaunchKernelGGL(k_to_float, dim3(1024), dim3(256), 0, c->s, nnz, A.s, S->s32.p);
+  // the density and potential systems share the weights: one copy per build
+  if (!(S->s32.p == s32_old && S->s32_src == A.s && S->s32_gen == c->sys_gen &&
+        S->s32_n == nnz && A.s == c->sco.p)) {
+    hipLaunchKernelGGL(k_to_float, dim3(1024), dim3(256), 0, c->s, nnz, A.s, S->s32.p);
+    S->s32_src = A.s;
+    S->s32_gen = c->sys_gen;
+    S->s32_n = nnz;
+  }
   hipLaunchKernelGGL(k_to_float, dim3(256), dim3(256), 0, c->s, (size_t)M, A.d, S->d32.p);
   hipLaunchKernelGGL(k_to_float, dim3(256), dim3(256), 0, c->s, (size_t)M, A.f, S->f32.p);
   hipLaunchKernelGGL(k_to_float, dim3(256), dim3(256), 0, c->s, (size_t)M, A.dinv, S->dinv32.p);

@@ -117,6 +117,8 @@ struct apc_ctx {
   // solver
   double c_dens = 1, c_pot = 1, c_field = 1;
   int have_nbr = 0, have_sys = 0, have_fields = 0;
+  // bumped by every build of the GFD system (the Laplacian weights sco)
+  long long sys_gen = 0;
   apc::DBuf<int> nbr;      // lm x M (slot-major)
   apc::DBuf<double> dist;  // lm x M
   apc::DBuf<double> sco;   // lm x M Laplacian weights s_ij

@@ -161,6 +161,11 @@ struct SolveState {
   GState *hst = nullptr;  // pinned mirror
   AmgHier *amg = nullptr;
   int mcap = 0, histcap = 0;
+  // s32 holds the FP32 copy of these weights (the density and potential
+  // systems share them): source, system generation, entries
+  const double *s32_src = nullptr;
+  long long s32_gen = -1;
+  size_t s32_n = 0;
   // segmented solves (packed boxes): per-box Hessenberg data and states, the
   // tile table, and the pinned mirror of the states (nbox + 1: the summary)
   DBuf<double> shess, spart;

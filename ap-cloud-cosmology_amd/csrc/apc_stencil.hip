@@ -433,6 +433,7 @@ int build_stencils(apc_ctx *c) {
   c->stencil_qr_rows = c->fit_qr_rows;
   c->stencil_max_cond = c->fit_max_cond;
   c->have_sys = 1;
+  ++c->sys_gen;
   return 0;
 }
 
