@@ -994,6 +994,22 @@ __global__ void __launch_bounds__(256) k_cy_ap_resid(int n, const int *__restric
   r[i] = (T)((double)r[i] - alpha * ap_row(P, e, i));
 }
 
+// Level 0 without pre-smoothing, after the first of two coarse corrections:
+// x = alpha e[agg] (x was 0) and r = b - alpha (A P) e (the residual of x = 0
+// is b itself).
+template <class T>
+__global__ void __launch_bounds__(256) k_cy_ap_resid_first(int n, const int *__restrict__ agg,
+                                                           Sell<T> P, const T *__restrict__ e,
+                                                           const T *__restrict__ b,
+                                                           T *__restrict__ x, T *__restrict__ r,
+                                                           double alpha, const GState *st) {
+  if (st && halted(st)) return;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  x[i] = (T)(alpha * (double)e[agg[i]]);
+  r[i] = (T)((double)b[i] - alpha * ap_row(P, e, i));
+}
+
 // Level 0 after the last coarse correction: the correction and the first
 // post-smoothing step, x2 = x + alpha e[agg], xout = x2 + omega D^-1 (b - A x2)
 // with b - A x2 = r - alpha (A P e) (r = b - A x on entry).
@@ -1161,9 +1177,11 @@ __device__ inline double ap_apply_g(const Sell<T> &P, const T *__restrict__ e,
 // Coarse level after a coarse correction that another follows: x += a_i
 // e[agg_i] and r -= (A P) (a e) (k_cy_ap_resid with G lanes per row and the
 // packed boxes' pass rows)
-template <class T, int G>
+// FIRST: the level's visit has no pre-smoothing, x was 0 and r is b.
+template <class T, int G, bool FIRST>
 __global__ void __launch_bounds__(256) k_cy_ap_resid_g(Sell<T> P, const int *__restrict__ agg,
-                                                       const T *__restrict__ e, T *__restrict__ x,
+                                                       const T *__restrict__ e,
+                                                       const T *__restrict__ b, T *__restrict__ x,
                                                        T *__restrict__ r, double alpha,
                                                        const unsigned char *__restrict__ pass,
                                                        const unsigned char *__restrict__ apass,
@@ -1175,14 +1193,14 @@ __global__ void __launch_bounds__(256) k_cy_ap_resid_g(Sell<T> P, const int *__r
   const double ape = ap_apply_g<T, G>(P, e, apass, alpha, alpha_pass, i, sub);
   if (sub == 0) {
     const double a = (pass && pass[i]) ? alpha_pass : alpha;
-    x[i] = (T)((double)x[i] + a * (double)e[agg[i]]);
-    r[i] = (T)((double)r[i] - ape);
+    x[i] = (T)((FIRST ? 0.0 : (double)x[i]) + a * (double)e[agg[i]]);
+    r[i] = (T)((double)(FIRST ? b[i] : r[i]) - ape);
   }
 }
 
 // Coarse level after the last coarse correction: the correction and the
 // first post-smoothing step (k_cy_ap_sweep with G lanes per row)
-template <class T, int G>
+template <class T, int G, bool FIRST>
 __global__ void __launch_bounds__(256) k_cy_ap_sweep_g(Sell<T> P, const int *__restrict__ agg,
                                                        const T *__restrict__ e,
                                                        const T *__restrict__ dinv,
@@ -1200,7 +1218,7 @@ __global__ void __launch_bounds__(256) k_cy_ap_sweep_g(Sell<T> P, const int *__r
   const double ape = ap_apply_g<T, G>(P, e, apass, alpha, alpha_pass, i, sub);
   if (sub == 0) {
     const double a = (pass && pass[i]) ? alpha_pass : alpha;
-    const T x2 = (T)((double)xin[i] + a * (double)e[agg[i]]);
+    const T x2 = (T)((FIRST ? 0.0 : (double)xin[i]) + a * (double)e[agg[i]]);
     xout[i] = (T)((double)x2 + omega * (double)dinv[i] * ((double)r[i] - ape));
   }
 }
@@ -2054,11 +2072,15 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
   // coarse levels with one pre-smoothing step: the first step is formed
   // inside the first residual (k_cy_first_resid_g), one launch fewer
   const bool fuse_first = !ell && gl > 1 && o.nu == 1;
-  if (!fuse_first) {
+  // level 0 without pre-smoothing (A P form, W-cycle): x starts at 0, the
+  // first restriction takes the input b, the first correction sets x
+  const bool nopre = L->ap && l < o.nopre && (!ell || (l < o.wlevels && o.gamma == 2));
+  if (nopre) {
+  } else if (!fuse_first) {
     KScope ks(c, ell ? "amg_first0" : "amg_first_coarse");
     hipLaunchKernelGGL((k_cy_first<T>), dim3(g), dim3(256), 0, c->s, n, dinv, b, cur, o.omega, st);
   }
-  for (int s = 1; s < o.nu; ++s) {
+  for (int s = 1; s < o.nu && !nopre; ++s) {
     sweep(cur, alt);
     std::swap(cur, alt);
   }
@@ -2075,8 +2097,9 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
                               : 0.0;
   int post_done = 0;
   for (int q = 0; q < gam; ++q) {
-    if (apf && q > 0) {
-      // r = b - A cur already (k_cy_ap_resid after the previous correction)
+    if ((apf && q > 0) || (nopre && q == 0)) {
+      // r = b - A cur already (k_cy_ap_resid after the previous correction),
+      // or x = 0 and the residual is b (restricted directly)
     } else if (ell) {
       KScope ks(c, f32 ? "amg_resid0_32" : "amg_resid0", (double)n * (lm0 * (4 + sT) + 5 * sT));
       hipLaunchKernelGGL((k_cy_resid<T, true>), dim3(g), dim3(256), 0, c->s, E, C, b, cur, r, st);
@@ -2105,10 +2128,10 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
                 ell ? (double)n * (sT + 4) + (double)Lc->n * (4 + sT) : 0.0);
       if (Lc->n <= o.rg8_rows)
         hipLaunchKernelGGL((k_cy_restrict_g<T, 8>), dim3(nblocks((long)Lc->n * 8, 256)), dim3(256),
-                           0, c->s, Lc->n, L->mp.p, L->ml.p, r, Pc.b, st);
+                           0, c->s, Lc->n, L->mp.p, L->ml.p, (nopre && q == 0) ? b : r, Pc.b, st);
       else
         hipLaunchKernelGGL((k_cy_restrict<T>), dim3(nblocks(Lc->n, 256)), dim3(256), 0, c->s,
-                           Lc->n, L->mp.p, L->ml.p, r, Pc.b, st);
+                           Lc->n, L->mp.p, L->ml.p, (nopre && q == 0) ? b : r, Pc.b, st);
     }
     int rr = cycle<T>(c, h, A0, l + 1, Pc.b, Pc.x, Pc.t, st);
     if (rr) return rr;
@@ -2118,17 +2141,26 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
     const unsigned char *apass = pass ? L->apass.p : nullptr;
     const double apass_alpha = q == 0 ? 1.0 : 0.0;
     const unsigned gg = nblocks((long)n * gl, 256);
-    if (apf && q + 1 < gam) {
+    const bool first = nopre && q == 0;  // x was 0, the residual is b
+    if (ell && first) {
+      KScope ks(c, f32 ? "amg_ap_resid0_32" : "amg_ap_resid0", ap_bytes);
+      hipLaunchKernelGGL((k_cy_ap_resid_first<T>), dim3(g), dim3(256), 0, c->s, n,
+                         (const int *)L->agg.p, AP, (const T *)Pc.x, b, cur, r, o.alpha, st);
+    } else if (apf && q + 1 < gam) {
       if (ell) {
         KScope ks(c, f32 ? "amg_ap_resid0_32" : "amg_ap_resid0", ap_bytes);
         hipLaunchKernelGGL((k_cy_ap_resid<T>), dim3(g), dim3(256), 0, c->s, n,
                            (const int *)L->agg.p, AP, (const T *)Pc.x, cur, r, o.alpha, st);
       } else {
         KScope ks(c, kApResidName[l]);
-        auto *kern = gl == 8 ? k_cy_ap_resid_g<T, 8> : gl == 4 ? k_cy_ap_resid_g<T, 4>
-                                                               : k_cy_ap_resid_g<T, 1>;
+        auto *kern = first ? (gl == 8 ? k_cy_ap_resid_g<T, 8, true>
+                                      : gl == 4 ? k_cy_ap_resid_g<T, 4, true>
+                                                : k_cy_ap_resid_g<T, 1, true>)
+                           : (gl == 8 ? k_cy_ap_resid_g<T, 8, false>
+                                      : gl == 4 ? k_cy_ap_resid_g<T, 4, false>
+                                                : k_cy_ap_resid_g<T, 1, false>);
         hipLaunchKernelGGL(kern, dim3(gg), dim3(256), 0, c->s, AP, (const int *)L->agg.p,
-                           (const T *)Pc.x, cur, r, o.alpha, pass, apass, apass_alpha, st);
+                           (const T *)Pc.x, b, cur, r, o.alpha, pass, apass, apass_alpha, st);
       }
     } else if (apf && o.nu >= 1) {
       if (ell) {
@@ -2138,11 +2170,16 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
                            (const T *)r, alt, o.alpha, o.omega, st);
       } else {
         KScope ks(c, kApSweepName[l]);
-        auto *kern = gl == 8 ? k_cy_ap_sweep_g<T, 8> : gl == 4 ? k_cy_ap_sweep_g<T, 4>
-                                                               : k_cy_ap_sweep_g<T, 1>;
+        auto *kern = first ? (gl == 8 ? k_cy_ap_sweep_g<T, 8, true>
+                                      : gl == 4 ? k_cy_ap_sweep_g<T, 4, true>
+                                                : k_cy_ap_sweep_g<T, 1, true>)
+                           : (gl == 8 ? k_cy_ap_sweep_g<T, 8, false>
+                                      : gl == 4 ? k_cy_ap_sweep_g<T, 4, false>
+                                                : k_cy_ap_sweep_g<T, 1, false>);
+        // first: no pre-smoothing, so the residual entering is b itself
         hipLaunchKernelGGL(kern, dim3(gg), dim3(256), 0, c->s, AP, (const int *)L->agg.p,
-                           (const T *)Pc.x, dinv, (const T *)cur, (const T *)r, alt, o.alpha,
-                           o.omega, pass, apass, apass_alpha, st);
+                           (const T *)Pc.x, dinv, (const T *)cur, first ? b : (const T *)r, alt,
+                           o.alpha, o.omega, pass, apass, apass_alpha, st);
       }
       std::swap(cur, alt);
       post_done = 1;

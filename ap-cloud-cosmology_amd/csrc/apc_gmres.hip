@@ -669,6 +669,7 @@ static AmgOptions amg_options(apc_ctx *c) {
   if (const char *v = getenv("APC_AMG_OMEGA")) a.omega = atof(v);
   if (const char *v = getenv("APC_AMG_ALPHA")) a.alpha = atof(v);
   if (const char *v = getenv("APC_AMG_AP")) a.ap = atoi(v);
+  if (const char *v = getenv("APC_AMG_NOPRE")) a.nopre = atoi(v);
   if (const char *v = getenv("APC_AMG_THETA")) a.theta = atof(v);
   return a;
 }

@@ -134,6 +134,7 @@ def test_ap_cycle_equals_the_plain_cycle(prec, monkeypatch):
     the converged solves agree to their tolerance in as many steps (+-1)."""
     one, full = {}, {}
     rho_opts = apcloud.ksp(rtol=1e-12, restart=60, max_it=2000)
+    monkeypatch.setenv("APC_AMG_NOPRE", "0")  # the plain cycle has its pre-smoothing step
     for ap in (0, 1):
         monkeypatch.setenv("APC_AMG_AP", str(ap))
         g, _, _, _ = _solve_pair("plummer", 30000, 1, 26)
@@ -152,3 +153,24 @@ def test_ap_cycle_equals_the_plain_cycle(prec, monkeypatch):
     err = np.abs(full[1][0] - full[0][0]).max() / np.abs(full[0][0]).max()
     assert err < 1e-8, err
     assert abs(full[1][1] - full[0][1]) <= 1, (full[0][1], full[1][1])
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_cycle_without_level0_presmoothing_converges_to_the_same_solution(prec, monkeypatch):
+    """Level 0 without its pre-smoothing step (the default) is a different
+    preconditioner, not a different answer: the converged potential equals
+    the one of the cycle with pre-smoothing to the solves' tolerance, and
+    both take a similar number of Krylov steps."""
+    out = {}
+    rho_opts = apcloud.ksp(rtol=1e-12, restart=60, max_it=2000)
+    for nopre in (0, 1):
+        monkeypatch.setenv("APC_AMG_NOPRE", str(nopre))
+        g, _, _, _ = _solve_pair("plummer", 30000, 1, 27)
+        info = g.solve(rho_opts, apcloud.ksp(rtol=1e-11, restart=60, max_it=2000, precision=prec,
+                                             pc="amg"))
+        assert info.reason_phi > 0, info.reason_phi
+        out[nopre] = (g.fields()["phi"], info.its_phi)
+        g.close()
+    err = np.abs(out[1][0] - out[0][0]).max() / np.abs(out[0][0]).max()
+    assert err < 1e-8, err
+    assert out[1][1] <= out[0][1] + 6, (out[0][1], out[1][1])
