@@ -198,6 +198,19 @@ struct EllView {
   }
 };
 
+// Level 0's A P (sliced ELL, k_ap_build): row i's real entries, columns
+// coarse rows (count only: the Galerkin emission of P^T (A P)).
+struct ApSellView {
+  const int *soff, *col, *cnt;
+  const double *val;
+  __device__ int count(int i) const { return cnt[i]; }
+  template <class F>
+  __device__ void entries(int i, F f) const {
+    const long long base = (long long)soff[i >> 6] * 64 + (i & 63);
+    for (int k = 0; k < cnt[i]; ++k) f(col[base + (long long)k * 64], val[base + (long long)k * 64]);
+  }
+};
+
 struct CsrView {
   Csr<double> A;
   __device__ int n() const { return A.n; }
@@ -470,7 +483,7 @@ __global__ void __launch_bounds__(256) k_gal_emit(V v, int n, const int *__restr
   if (i >= n) return;
   int e = off_row[i];
   v.entries(i, [&](int j, double a) {
-    ej[e] = agg[j];
+    ej[e] = agg ? agg[j] : j;  // no map: the entries' columns are coarse already (A P)
     ea[e] = a;
     ++e;
   });
@@ -710,7 +723,8 @@ __global__ void __launch_bounds__(256) k_ap_build(Op<double> A, const int *__res
                                                   int *__restrict__ soff, int *__restrict__ wid,
                                                   int *__restrict__ snnz, int *__restrict__ col,
                                                   double *__restrict__ val,
-                                                  float *__restrict__ val32) {
+                                                  float *__restrict__ val32,
+                                                  int *__restrict__ rcnt) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int s = i >> 6, lane = threadIdx.x & 63;
   if (s >= (A.M + 63) >> 6) return;  // s is uniform over a wave: whole waves leave
@@ -753,13 +767,15 @@ __global__ void __launch_bounds__(256) k_ap_build(Op<double> A, const int *__res
     w = max(w, __shfl_xor(w, off, 64));
     tot += __shfl_xor(tot, off, 64);
   }
-  if (live)
+  if (live) {
+    rcnt[i] = cnt;
     for (int k = cnt; k < w; ++k) {
       const long long e = base + (long long)k * 64;
       col[e] = key[0];
       val[e] = 0.0;
       if (val32) val32[e] = 0.0f;
     }
+  }
   if (lane == 0) {
     soff[s] = s * KMAX;
     wid[s] = w;
@@ -1306,7 +1322,7 @@ static int group_members(apc_ctx *c, AmgHier *h, AmgLevel *Lf, int n, int nc) {
 // k_gal_rowoff, k_gal_emit, k_gal_rows).
 template <class V>
 static int galerkin_rows(apc_ctx *c, AmgHier *h, const V &v, AmgLevel *Lf, int n, int nc,
-                         AmgLevel *Lc) {
+                         AmgLevel *Lc, bool coarse_cols = false) {
   if ((size_t)Lf->nnz >= (size_t)INT32_MAX)  // entry offsets are int
     return fail(c, APC_ERR_UNSUPPORTED, "amg: %d stored entries on one level", Lf->nnz);
   APC_ALLOC(c, h->flag, (size_t)n + 1);  // entry counts, then coarse-row counts
@@ -1325,7 +1341,8 @@ static int galerkin_rows(apc_ctx *c, AmgHier *h, const V &v, AmgLevel *Lf, int n
   APC_ALLOC(c, h->gcol, (size_t)ne + 1);
   APC_ALLOC(c, h->gval, (size_t)ne + 1);
   hipLaunchKernelGGL(k_gal_rowoff, dim3(g), dim3(256), 0, c->s, n, Lf->ml.p, h->pos.p, h->sa.p);
-  hipLaunchKernelGGL((k_gal_emit<V>), dim3(g), dim3(256), 0, c->s, v, n, Lf->agg.p, h->sa.p,
+  hipLaunchKernelGGL((k_gal_emit<V>), dim3(g), dim3(256), 0, c->s, v, n,
+                     coarse_cols ? (const int *)nullptr : (const int *)Lf->agg.p, h->sa.p,
                      h->gcol.p, h->gval.p);
   const unsigned grid = (unsigned)std::min(nc, 1 << 16);
   APC_ALLOC(c, h->diag, nc + 1);
@@ -1473,12 +1490,13 @@ static int build_ap(apc_ctx *c, AmgHier *h, const Op<double> &A, AmgLevel *L, bo
   APC_ALLOC(c, L->apwid, ns + 1);
   APC_ALLOC(c, h->apsn, ns + 1);
   APC_ALLOC(c, h->aptot, 2);
+  APC_ALLOC(c, h->apcnt, (size_t)n + 1);
   APC_ALLOC(c, L->apcol, cap + 1);
   APC_ALLOC(c, L->apval, cap + 1);
   if (fp32) APC_ALLOC(c, L->apval32, cap + 1);
   hipLaunchKernelGGL((k_ap_build<kApMax>), dim3(nblocks(n, 256)), dim3(256), 0, c->s, A, L->agg.p,
                      L->apoff.p, L->apwid.p, h->apsn.p, L->apcol.p, L->apval.p,
-                     fp32 ? L->apval32.p : nullptr);
+                     fp32 ? L->apval32.p : nullptr, h->apcnt.p);
   hipLaunchKernelGGL(k_ap_total, dim3(1), dim3(1024), 0, c->s, ns, (const int *)L->apwid.p,
                      (const int *)h->apsn.p, h->aptot.p);
   long long tot[2] = {0, 0};
@@ -1887,17 +1905,25 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
     if (r) return r;
     AmgLevel *Lc = level(l + 1);
     {
+      if (o.ap && (l > 0 || A.lm + 1 <= kApMax)) {
+        KScope ks1(c, "amg_setup_ap");
+        r = (l == 0) ? build_ap(c, h, A, Lf, fp32) : build_ap_coarse(c, h, Lf, nc, fp32);
+        if (r) return r;
+      }
+      // with A P built, the coarse operator is P^T (A P): its rows sum the
+      // members' A P rows, a third of the entries of A's
       KScope ks1(c, "amg_setup_galerkin");
       if (o.galerkin_sorted) r = galerkin_sorted(c, h, A, Lf, l, n, nc, Lc);
+      else if (Lf->ap && l == 0)
+        r = galerkin_rows(c, h, ApSellView{Lf->apoff.p, Lf->apcol.p, h->apcnt.p, Lf->apval.p}, Lf,
+                          n, nc, Lc, true);
+      else if (Lf->ap)
+        r = galerkin_rows(c, h, CsrView{Csr<double>{h->aprp.p, h->apc.p, h->apv.p, n}}, Lf, n, nc,
+                          Lc, true);
       else if (l == 0) r = galerkin_rows_ell(c, h, A, Lf, n, nc, Lc);
       else r = galerkin_rows(c, h, CsrView{Csr<double>{Lf->rp.p, Lf->col.p, Lf->val.p, n}}, Lf, n, nc, Lc);
     }
     if (r) return r;
-    if (o.ap && (l > 0 || A.lm + 1 <= kApMax)) {
-      KScope ks1(c, "amg_setup_ap");
-      r = (l == 0) ? build_ap(c, h, A, Lf, fp32) : build_ap_coarse(c, h, Lf, nc, fp32);
-      if (r) return r;
-    }
     {
       KScope ks1(c, "amg_setup_work");
       r = ensure_work(c, Lc, nc, fp32);
