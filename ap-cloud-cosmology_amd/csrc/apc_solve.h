@@ -106,7 +106,8 @@ __device__ inline double row_apply(const Op<T> &A, const T *__restrict__ x, int 
 struct AmgHier;  // apc_amg.hip
 
 // Defaults from sweeps on the 256^3 Zel'dovich + Plummer system (7.5M rows):
-// one Jacobi sweep each side, damping 0.85, a W-cycle on the three finest
+// one Jacobi sweep each side (level 0: after the correction only, nopre),
+// damping 0.85, a W-cycle on the three finest
 // levels and V below, and a dense solve once a level has <= 256 rows (6
 // levels: 7.5M, 611k, 58k, 6.7k, 838, ~100) took the step from 287 to 248 ms
 // (40 -> 33 GMRES steps to rtol 1e-8) against a W-cycle everywhere with a
