@@ -43,7 +43,7 @@ EXPORTS = [
 
 # launch-shape knobs (APC_KNOB_* in include/apcloud.h), in enum order
 KNOBS = ["search_chunk", "mdot_blocks", "maxpy_blocks", "seg_blocks", "amg_g8_rows",
-         "amg_g4_rows", "amg_rg8_rows"]
+         "amg_g4_rows", "amg_rg8_rows", "amg_nopre", "amg_wlevels"]
 
 
 class Params(C.Structure):

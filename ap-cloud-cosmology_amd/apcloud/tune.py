@@ -1,5 +1,5 @@
-"""Launch-shape autotuner for a context's kernels (the APC_KNOB_* knobs of
-include/apcloud.h).
+"""Launch-shape and cycle-shape autotuner for a context's kernels (the
+APC_KNOB_* knobs of include/apcloud.h).
 
 The tuner times candidate values of every knob on steps of the caller's own
 workload and keeps the fastest: coordinate descent in knob order, one pass,
@@ -11,10 +11,21 @@ that changes the iteration count by a reduction-order rounding is not
 rewarded or penalised for it.  A candidate replaces the current value only
 when it is at least `margin` faster.  Knobs whose kernels the workload never
 launches (for instance the packed boxes' tile grid on a single box) are
-skipped.  The knobs change launch shapes only; results differ at most by the
-summation order of reductions.
+skipped.  These knobs change launch shapes only; results differ at most by
+the summation order of reductions.
+
+The multigrid cycle knobs (levels without pre-smoothing, W-cycle depth) are
+tuned first: they change the preconditioner and so the Krylov step count,
+and their cost is the wall time of the whole step (the best of `reps` runs,
+timers off).  Results then differ within the solves' tolerance.
 """
 import time
+
+# cycle-shape knobs, costed by the step's wall time (tuned first, in order)
+WALL = {
+    "amg_nopre": [0, 1, 2],
+    "amg_wlevels": [2, 3, 4],
+}
 
 # candidate values per knob (the built-in default is always among them)
 CANDIDATES = {
@@ -41,7 +52,7 @@ TIMERS = {
 }
 
 
-def autotune(ctx, step, knobs=None, slot=63, margin=0.02):
+def autotune(ctx, step, knobs=None, slot=63, margin=0.02, reps=2, clock=time.perf_counter):
     """Tune ctx's knobs on `step()`, a callable that runs one step of the
     workload on ctx.  The state at the call is saved in checkpoint `slot`,
     every trial starts from it, and it is restored at the end.  Returns
@@ -61,8 +72,35 @@ def autotune(ctx, step, knobs=None, slot=63, margin=0.02):
         ctx.set_timing(False)
         return kt
 
+    def measure_wall():
+        best = float("inf")
+        for _ in range(reps):
+            ctx.checkpoint(slot, restore=True)
+            ctx.sync()
+            t = clock()
+            step()
+            ctx.sync()
+            best = min(best, clock() - t)
+        return 1e3 * best
+
     trials = {}
+    for name in (knobs or list(WALL)):
+        if name not in WALL or name not in defaults:
+            continue
+        cur = ctx.knobs()[name]
+        res = {cur: measure_wall()}
+        for v in WALL[name]:
+            if v != cur:
+                ctx.set_knob(name, v)
+                res[v] = measure_wall()
+        best = min(res, key=res.get)
+        if res[best] > (1.0 - margin) * res[cur]:
+            best = cur
+        ctx.set_knob(name, best)
+        trials[name] = {int(v): round(ms, 3) for v, ms in sorted(res.items())}
     for name in knobs or list(CANDIDATES):
+        if name not in CANDIDATES or name not in defaults:
+            continue
         cur = ctx.knobs()[name]
         kt0 = measure()
         weights = {t: kt0[t][1] for t in TIMERS[name] if t in kt0 and kt0[t][1] > 0}

@@ -134,6 +134,9 @@ int apc_device_count(int *n) {
   return (k > 0) ? APC_OK : APC_ERR_NO_DEVICE;
 }
 
+// smallest valid value of a knob
+static int knob_min(int k) { return k == APC_KNOB_AMG_NOPRE ? 0 : 1; }
+
 int apc_create(const apc_params *p, apc_ctx **out) {
   if (!p || !out) return APC_ERR_ARG;
   *out = nullptr;
@@ -146,10 +149,11 @@ int apc_create(const apc_params *p, apc_ctx **out) {
   // APC_* environment overrides of the launch-shape knobs (hand sweeps)
   static const char *const knob_env[APC_KNOB_COUNT] = {
       "APC_NEIGH_XCHUNK", "APC_MDOT_BLOCKS", "APC_MAXPY_BLOCKS", "APC_SEG_BLOCKS",
-      "APC_AMG_G8",       "APC_AMG_G4",      "APC_AMG_RG8"};
+      "APC_AMG_G8",       "APC_AMG_G4",      "APC_AMG_RG8",      "APC_AMG_NOPRE",
+      "APC_AMG_WLEVELS"};
   for (int k = 0; k < APC_KNOB_COUNT; ++k)
     if (const char *v = getenv(knob_env[k]))
-      if (atoi(v) >= 1) c->knob[k] = atoi(v);
+      if (atoi(v) >= knob_min(k)) c->knob[k] = atoi(v);
   c->depth_req = p->depth;
   c->maxdepth_req = p->maxdepth;
   hipGetDevice(&c->dev);
@@ -226,7 +230,7 @@ int apc_pack_rows(apc_ctx *c, int *row_start, int *comp_rows) {
 
 int apc_set_knob(apc_ctx *c, int knob, int value) {
   if (!c) return APC_ERR_ARG;
-  if (knob < 0 || knob >= APC_KNOB_COUNT || value < 1)
+  if (knob < 0 || knob >= APC_KNOB_COUNT || value < knob_min(knob))
     return fail(c, APC_ERR_ARG, "set_knob: knob %d value %d", knob, value);
   c->knob[knob] = value;
   return APC_OK;

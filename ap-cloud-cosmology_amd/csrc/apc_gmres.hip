@@ -659,7 +659,6 @@ static AmgOptions amg_options(apc_ctx *c) {
   AmgOptions a;
   a.nu = env_int("APC_AMG_NU", a.nu);
   a.gamma = env_int("APC_AMG_GAMMA", a.gamma);
-  a.wlevels = env_int("APC_AMG_WLEVELS", a.wlevels);
   a.coarse_max = env_int("APC_AMG_COARSE", a.coarse_max);
   a.galerkin_sorted = env_int("APC_AMG_GALERKIN_SORTED", 0);
   if (const char *v = getenv("APC_AMG_SHIFT")) a.shift = atoi(v);
@@ -669,7 +668,8 @@ static AmgOptions amg_options(apc_ctx *c) {
   if (const char *v = getenv("APC_AMG_OMEGA")) a.omega = atof(v);
   if (const char *v = getenv("APC_AMG_ALPHA")) a.alpha = atof(v);
   if (const char *v = getenv("APC_AMG_AP")) a.ap = atoi(v);
-  if (const char *v = getenv("APC_AMG_NOPRE")) a.nopre = atoi(v);
+  a.nopre = c->knob[APC_KNOB_AMG_NOPRE];
+  a.wlevels = c->knob[APC_KNOB_AMG_WLEVELS];
   if (const char *v = getenv("APC_AMG_THETA")) a.theta = atof(v);
   return a;
 }

@@ -39,10 +39,11 @@ accelerations and largest position difference per step; (2) the reference
 configuration (FP64, Jacobi, the same GMRES settings) for one step after one
 untimed step of it (first-use allocations): its time, iterations and its
 force difference from the converged FP64 step (skipped with
---no-ref-config); then the launch shapes are autotuned (apcloud.tune, on the
-first schedule step of the measured form; --no-tune keeps the built-in
-shapes) and the choices recorded; (3) the FP64 step with the same solver
-settings timed over K steps (skipped with --no-fp64); (4) the measured
+--no-ref-config); then the launch shapes and the multigrid cycle shape are
+autotuned (apcloud.tune, on the second schedule step of the measured form;
+--no-tune keeps the built-in shapes) and the choices recorded, for both
+timed legs; (3) the FP64 step with the same solver settings and shapes
+timed over K steps (skipped with --no-fp64); (4) the measured
 precision timed over K steps -> "value".  The roofline entry is for the
 kernel with the largest total device time (average launch time x launches)
 in one extra step of leg (4) run with the library's kernel timers on (the
@@ -309,15 +310,18 @@ def main():
             ctx.set_timing(False)
         return wall, infos, dts, phases, kt, info.nc + info.nv
 
-    # launch shapes: autotuned on the first schedule step of the measured
-    # form, from the state after the half drift, before any timed leg
+    # launch and cycle shapes: autotuned on the second schedule step of the
+    # measured form (the first solves the freshly placed clusters cold, in
+    # about twice the Krylov steps of the others), before any timed leg
     tuned = None
     if not args.no_tune:
         from apcloud import tune
-        log("autotuning launch shapes")
+        log("autotuning launch and cycle shapes")
         ctx.checkpoint(0, restore=True)
         o_t = opts(args.precision, args.pc)
-        tuned = tune.autotune(ctx, lambda: one_step(0, o_t))
+        one_step(0, o_t)
+        tuned = tune.autotune(ctx, lambda: one_step(1, o_t))
+        ctx.checkpoint(0, restore=True)
 
     fp64 = None
     if not args.no_fp64 and args.precision != 0:
@@ -465,8 +469,9 @@ def multibox_leg(args):
     tuned = None
     if not args.no_tune:
         from apcloud import tune
-        log("multibox leg: autotuning launch shapes")
-        tuned = tune.autotune(P, lambda: step(P, 0, o))
+        log("multibox leg: autotuning launch and cycle shapes")
+        step(P, 0, o)
+        tuned = tune.autotune(P, lambda: step(P, 1, o))
         P.checkpoint(0, restore=True)
     step(P, 0, o)  # warmup
     P.checkpoint(0, restore=True)

@@ -97,13 +97,16 @@ int apc_device_count(int *n);
 int apc_create(const apc_params *p, apc_ctx **out);
 int apc_destroy(apc_ctx *ctx);
 
-/* Launch-shape knobs of this build's kernels, per context (no counterpart in
- * the reference).  Defaults below; an APC_* environment variable of the same
- * meaning (APC_NEIGH_XCHUNK, APC_MDOT_BLOCKS, APC_MAXPY_BLOCKS, APC_SEG_BLOCKS,
- * APC_AMG_G8, APC_AMG_G4, APC_AMG_RG8) overrides a default at apc_create; the
+/* Launch-shape and cycle-shape knobs of this build's kernels, per context (no
+ * counterpart in the reference).  Defaults below; an APC_* environment
+ * variable of the same meaning (APC_NEIGH_XCHUNK, APC_MDOT_BLOCKS,
+ * APC_MAXPY_BLOCKS, APC_SEG_BLOCKS, APC_AMG_G8, APC_AMG_G4, APC_AMG_RG8,
+ * APC_AMG_NOPRE, APC_AMG_WLEVELS) overrides a default at apc_create; the
  * autotuner (apcloud.tune) sets them with apc_set_knob.  Every value >= 1 is
- * valid and changes only speed, never results beyond the summation order of
- * the reductions. */
+ * valid (>= 0 for APC_KNOB_AMG_NOPRE).  The launch-shape knobs change only
+ * speed, never results beyond the summation order of the reductions; the two
+ * multigrid cycle knobs change the preconditioner, so the Krylov step count
+ * and the solution within the solve's tolerance. */
 enum {
   APC_KNOB_SEARCH_CHUNK = 0, /* nodes per chunk the neighbour search deals to the XCDs (256) */
   APC_KNOB_MDOT_BLOCKS,      /* grid of the GMRES multi-dot (1024) */
@@ -112,6 +115,8 @@ enum {
   APC_KNOB_AMG_G8_ROWS,      /* multigrid levels of at most this many rows: 8 lanes per row (65536) */
   APC_KNOB_AMG_G4_ROWS,      /* ... else of at most this many rows: 4 lanes per row (2^30) */
   APC_KNOB_AMG_RG8_ROWS,     /* restrictions into at most this many rows: 8 lanes (2^30) */
+  APC_KNOB_AMG_NOPRE,        /* multigrid: finest levels without pre-smoothing (1) */
+  APC_KNOB_AMG_WLEVELS,      /* multigrid: finest levels visited by a W-cycle (3) */
   APC_KNOB_COUNT
 };
 int apc_set_knob(apc_ctx *ctx, int knob, int value);

@@ -1,7 +1,8 @@
-"""The launch-shape autotuner on the GPU: it tunes a 32^3 step (FP32
-multigrid in FP64 GMRES), keeps candidate values, leaves the state where it
-found it, and the tuned knobs change the step's results by no more than the
-solve tolerance (they shape launches only)."""
+"""The autotuner on the GPU: it tunes a 32^3 step (FP32 multigrid in FP64
+GMRES), keeps candidate values, leaves the state where it found it, and the
+tuned knobs change the step's results by no more than the solve tolerance
+(launch shapes change only the summation order, the cycle shapes the
+preconditioner)."""
 import numpy as np
 import pytest
 
@@ -27,7 +28,8 @@ def test_autotune_a_small_step():
     res = tune.autotune(g, lambda: g.step(*args))
     print(res, flush=True)
     for k, v in res["choices"].items():
-        assert v == res["defaults"][k] or v in tune.CANDIDATES[k], (k, v)
+        assert v == res["defaults"][k] or v in tune.CANDIDATES.get(k, tune.WALL.get(k, [])), (k, v)
+    assert "amg_nopre" in res["trials"] and "amg_wlevels" in res["trials"]
     assert "seg_blocks" not in res["trials"]  # a single box never launches the tile kernels
     assert "search_chunk" in res["trials"] and "amg_g8_rows" in res["trials"]
     after = g.particles()

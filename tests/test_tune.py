@@ -70,3 +70,31 @@ def test_the_tuner_keeps_the_cheapest_candidate_of_every_knob():
     # one restore per trial and one at the end
     assert ctx.restores == len(step_calls) + 1
     assert not ctx.timing
+
+
+class WallCtx(FakeCtx):
+    """The cycle knobs: each step advances a fake clock by a function of the
+    knobs (the Krylov step count they cause)."""
+
+    def __init__(self, step_ms):
+        super().__init__(lambda t, k: 1.0, lambda t, k: 1)
+        self.k.update({"amg_nopre": 1, "amg_wlevels": 3})
+        self.step_ms, self.now = step_ms, 0.0
+
+    def clock(self):
+        return self.now
+
+    def step(self):
+        self.now += self.step_ms(self.k) * 1e-3
+
+
+def test_the_cycle_knobs_are_tuned_first_by_the_step_time():
+    def step_ms(k):
+        return {0: 80.0, 1: 90.0, 2: 95.0}[k["amg_nopre"]] + {2: 1.0, 3: 0.0, 4: 5.0}[k["amg_wlevels"]]
+    ctx = WallCtx(step_ms)
+    res = tune.autotune(ctx, ctx.step, clock=ctx.clock)
+    assert res["choices"]["amg_nopre"] == 0            # 11% faster: taken
+    assert res["choices"]["amg_wlevels"] == 3          # 1.2% slower / 6% slower: kept
+    assert res["trials"]["amg_nopre"] == {0: 80.0, 1: 90.0, 2: 95.0}
+    assert list(res["trials"])[:2] == ["amg_nopre", "amg_wlevels"]
+    assert not ctx.timing
