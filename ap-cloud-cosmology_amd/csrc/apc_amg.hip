@@ -119,6 +119,7 @@ struct AmgHier {
   DBuf<double> ev, smax, diag;
   DBuf<int> gcol;     // Galerkin entries grouped by coarse row: coarse column
   DBuf<double> gval;  // and value
+  DBuf<int> rcnt;     // undecided rows after each independent-set round of a batch
   DBuf<int> apsn;     // level 0's A P: real entries per slice
   DBuf<long long> aptot;
   DBuf<int> apiota, aprp, apcnt, apc;  // coarse levels' A P: rows as one-row groups, CSR
@@ -131,7 +132,7 @@ struct AmgHier {
     frozen.release(); bstart.release(); goff.release(); ioff.release(); bshift.release();
     key2.release(); idx.release(); idx2.release(); sa.release(); sb.release(); flag.release();
     pos.release(); cnt.release(); ev.release(); smax.release(); diag.release();
-    gcol.release(); gval.release(); apsn.release(); aptot.release();
+    gcol.release(); gval.release(); apsn.release(); aptot.release(); rcnt.release();
     apiota.release(); aprp.release(); apcnt.release(); apc.release(); apv.release();
     apdiag.release();
   }
@@ -262,7 +263,9 @@ __global__ void __launch_bounds__(256) k_amg_smax(V v, double *__restrict__ smax
 // 2 covered), b: states out.
 template <class V>
 __global__ void __launch_bounds__(256) k_amg_pick(V v, double theta, const double *__restrict__ smax,
-                                                  const int *__restrict__ a, int *__restrict__ b) {
+                                                  const int *__restrict__ a, int *__restrict__ b,
+                                                  const int *__restrict__ prev_und) {
+  if (prev_und && *prev_und == 0) return;  // the set settled in an earlier round
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= v.n()) return;
   int s = a[i];
@@ -286,7 +289,9 @@ __global__ void __launch_bounds__(256) k_amg_pick(V v, double theta, const doubl
 // cost more than the round's loads.
 template <class V>
 __global__ void __launch_bounds__(256) k_amg_cover(V v, double theta, const double *__restrict__ smax,
-                                                   int *__restrict__ b, int *__restrict__ undecided) {
+                                                   int *__restrict__ b, int *__restrict__ undecided,
+                                                   const int *__restrict__ prev_und) {
+  if (prev_und && *prev_und == 0) return;
   __shared__ int ws[4];
   int und = 0;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < v.n(); i += gridDim.x * 256) {
@@ -1273,18 +1278,36 @@ static int aggregate_level(apc_ctx *c, AmgHier *h, const V &v, int n, AmgLevel *
                        h->sa.p);
   else
     APC_HIP(c, hipMemsetAsync(h->sa.p, 0, sizeof(int) * n, c->s));
+  // rounds go out in batches of kRounds with one host read per batch: a
+  // round whose predecessor left no undecided row returns at once, and the
+  // set is read from the buffer of the first round that settled it
+  constexpr int kRounds = 8;
+  APC_ALLOC(c, h->rcnt, kRounds + 1);
   int *a = h->sa.p, *b = h->sb.p;
-  for (int round = 0;; ++round) {
-    if (round > 200) return fail(c, APC_ERR_UNSUPPORTED, "amg: independent set did not settle");
-    APC_HIP(c, hipMemsetAsync(h->cnt.p, 0, sizeof(int), c->s));
-    hipLaunchKernelGGL((k_amg_pick<V>), dim3(g), dim3(256), 0, c->s, v, o.theta, h->smax.p, a, b);
-    hipLaunchKernelGGL((k_amg_cover<V>), dim3(std::min(g, 2048u)), dim3(256), 0, c->s, v, o.theta,
-                       h->smax.p, b, h->cnt.p);
-    int und = 0;
-    APC_HIP(c, hipMemcpyAsync(&und, h->cnt.p, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  for (int batch = 0;; ++batch) {
+    if (batch * kRounds > 200)
+      return fail(c, APC_ERR_UNSUPPORTED, "amg: independent set did not settle");
+    APC_HIP(c, hipMemsetAsync(h->rcnt.p, 0, sizeof(int) * kRounds, c->s));
+    int *after[kRounds];
+    for (int k = 0; k < kRounds; ++k) {
+      const int *prev = k ? h->rcnt.p + k - 1 : nullptr;
+      hipLaunchKernelGGL((k_amg_pick<V>), dim3(g), dim3(256), 0, c->s, v, o.theta, h->smax.p, a, b,
+                         prev);
+      hipLaunchKernelGGL((k_amg_cover<V>), dim3(std::min(g, 2048u)), dim3(256), 0, c->s, v,
+                         o.theta, h->smax.p, b, h->rcnt.p + k, prev);
+      after[k] = b;
+      std::swap(a, b);
+    }
+    int und[kRounds];
+    APC_HIP(c, hipMemcpyAsync(und, h->rcnt.p, sizeof(und), hipMemcpyDeviceToHost, c->s));
     APC_HIP(c, hipStreamSynchronize(c->s));
-    std::swap(a, b);
-    if (und == 0) break;
+    int settled = -1;
+    for (int k = 0; k < kRounds && settled < 0; ++k)
+      if (und[k] == 0) settled = k;
+    if (settled >= 0) {
+      a = after[settled];
+      break;
+    }
   }
   hipLaunchKernelGGL(k_amg_rootflag, dim3(g), dim3(256), 0, c->s, n, a, h->flag.p);
   int r = scan_exclusive(c, h->flag.p, h->pos.p, n, h->cnt.p + 1);
