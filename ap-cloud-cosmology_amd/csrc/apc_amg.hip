@@ -199,19 +199,6 @@ struct EllView {
   }
 };
 
-// Level 0's A P (sliced ELL, k_ap_build): row i's real entries, columns
-// coarse rows (count only: the Galerkin emission of P^T (A P)).
-struct ApSellView {
-  const int *soff, *col, *cnt;
-  const double *val;
-  __device__ int count(int i) const { return cnt[i]; }
-  template <class F>
-  __device__ void entries(int i, F f) const {
-    const long long base = (long long)soff[i >> 6] * 64 + (i & 63);
-    for (int k = 0; k < cnt[i]; ++k) f(col[base + (long long)k * 64], val[base + (long long)k * 64]);
-  }
-};
-
 struct CsrView {
   Csr<double> A;
   __device__ int n() const { return A.n; }
@@ -456,6 +443,8 @@ __global__ void __launch_bounds__(256) k_amg_reduce(int n, int nc, const uint64_
 //  setup per step at 256^3).
 constexpr int GAL_CAP = 1024;
 constexpr int GAL_R = GAL_CAP / 64;
+// Entries per row of level 0's A P: the diagonal and up to kApMax - 1 slots.
+constexpr int kApMax = 32;
 
 __device__ inline int wave_min_int(int v) { return wave_min_int_dpp(v); }
 
@@ -673,6 +662,59 @@ __global__ void __launch_bounds__(64) k_gal_rows_ell(int nc, Op<double> Am,
   }
 }
 
+// The level-0 product P^T (A P) from A P's sliced ELL without an emission
+// step: coarse row I's entries are its members' A P rows, member k's entry
+// q (q below the members' widest row, W) at e = k W + q, taken in member
+// order; A P is kApMax wide at most, so pass 0 writes row I's results at
+// kApMax mp[I].
+__global__ void __launch_bounds__(64) k_gal_rows_ap(int nc, const int *__restrict__ soff,
+                                                    const int *__restrict__ pcol,
+                                                    const double *__restrict__ pval,
+                                                    const int *__restrict__ pcnt,
+                                                    const int *__restrict__ mp,
+                                                    const int *__restrict__ ml, int pass,
+                                                    int *__restrict__ ucnt,
+                                                    const int *__restrict__ rp,
+                                                    int *__restrict__ col,
+                                                    double *__restrict__ val,
+                                                    int *__restrict__ oj, double *__restrict__ oa,
+                                                    double *__restrict__ diag) {
+  const int lane = threadIdx.x;
+  for (int I = blockIdx.x; I < nc; I += gridDim.x) {
+    const int k0 = mp[I], nmem = mp[I + 1] - k0;
+    int W = 0;
+    for (int k = lane; k < nmem; k += 64) W = max(W, pcnt[ml[k0 + k]]);
+    for (int o = 32; o > 0; o >>= 1) W = max(W, __shfl_xor(W, o, 64));
+    const int E = nmem * W;
+    const long base = (long)k0 * kApMax;
+    auto src = [&](int e, int &J, double &A) {
+      const int k = e / W, q = e - k * W;
+      const int i = ml[k0 + k];
+      if (q < pcnt[i]) {
+        const long long p = (long long)soff[i >> 6] * 64 + (i & 63) + (long long)q * 64;
+        J = pcol[p];
+        A = pval[p];
+      }
+    };
+    if (pass == 0) {
+      const int u = (E <= GAL_CAP) ? gal_row_regs_any(I, E, src, oj + base, oa + base, diag)
+                                   : gal_row_stream(I, E, src, false, 0, nullptr, nullptr, nullptr);
+      if (lane == 0) ucnt[I] = u;
+    } else {
+      const int o = rp[I];
+      if (E <= GAL_CAP) {
+        const int u = rp[I + 1] - o;
+        for (int q = lane; q < u; q += 64) {
+          col[o + q] = oj[base + q];
+          val[o + q] = oa[base + q];
+        }
+      } else {
+        (void)gal_row_stream(I, E, src, true, o, col, val, diag);
+      }
+    }
+  }
+}
+
 __global__ void k_amg_dinv(int n, const double *__restrict__ diag, double *__restrict__ dinv) {
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) dinv[i] = (diag[i] != 0.0) ? 1.0 / diag[i] : 0.0;
@@ -711,9 +753,6 @@ __global__ void k_amg_convert(size_t n, const S *__restrict__ a, D *__restrict__
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
     b[i] = (D)a[i];
 }
-
-// Entries per row of level 0's A P: the diagonal and up to kApMax - 1 slots.
-constexpr int kApMax = 32;
 
 // Level 0's product A P with the aggregation's piecewise-constant
 // prolongation: for every aggregate J that row i's entries meet (the
@@ -1436,6 +1475,48 @@ static int galerkin_rows_ell(apc_ctx *c, AmgHier *h, const Op<double> &A, AmgLev
   return 0;
 }
 
+// P^T (A P) of level 0 into Lc (CSR) from A P's sliced ELL (k_gal_rows_ap)
+static int galerkin_rows_ap(apc_ctx *c, AmgHier *h, AmgLevel *Lf, int n, int nc, AmgLevel *Lc) {
+  const size_t W = (size_t)kApMax;
+  if (W * (size_t)n >= (size_t)INT32_MAX)  // coarse-row offsets are int
+    return fail(c, APC_ERR_UNSUPPORTED, "amg: %zu entries of A P", W * (size_t)n);
+  APC_ALLOC(c, h->gcol, W * n + 1);
+  APC_ALLOC(c, h->gval, W * n + 1);
+  APC_ALLOC(c, h->flag, (size_t)nc + 1);
+  APC_ALLOC(c, h->cnt, 4);
+  APC_ALLOC(c, h->diag, nc + 1);
+  APC_HIP(c, hipMemsetAsync(h->diag.p, 0, sizeof(double) * nc, c->s));  // both passes write it
+  const unsigned grid = (unsigned)std::min(nc, 1 << 16);
+  hipLaunchKernelGGL(k_gal_rows_ap, dim3(grid), dim3(64), 0, c->s, nc, (const int *)Lf->apoff.p,
+                     (const int *)Lf->apcol.p, (const double *)Lf->apval.p,
+                     (const int *)h->apcnt.p, (const int *)Lf->mp.p, (const int *)Lf->ml.p, 0,
+                     h->flag.p, (const int *)nullptr, (int *)nullptr, (double *)nullptr,
+                     h->gcol.p, h->gval.p, h->diag.p);
+  APC_ALLOC(c, h->sb, (size_t)nc + 1);
+  int r = scan_exclusive(c, h->flag.p, h->sb.p, nc, h->cnt.p + 1);
+  if (r) return r;
+  int nnz = 0;
+  APC_HIP(c, hipMemcpyAsync(&nnz, h->cnt.p + 1, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  Lc->n = nc;
+  Lc->nnz = nnz;
+  APC_ALLOC(c, Lc->rp, nc + 1);
+  APC_ALLOC(c, Lc->col, nnz + 1);
+  APC_ALLOC(c, Lc->val, nnz + 1);
+  APC_ALLOC(c, Lc->dinv, nc + 1);
+  APC_HIP(c, hipMemcpyAsync(Lc->rp.p, h->sb.p, sizeof(int) * nc, hipMemcpyDeviceToDevice, c->s));
+  APC_HIP(c, hipMemcpyAsync(Lc->rp.p + nc, h->cnt.p + 1, sizeof(int), hipMemcpyDeviceToDevice, c->s));
+  hipLaunchKernelGGL(k_gal_rows_ap, dim3(grid), dim3(64), 0, c->s, nc, (const int *)Lf->apoff.p,
+                     (const int *)Lf->apcol.p, (const double *)Lf->apval.p,
+                     (const int *)h->apcnt.p, (const int *)Lf->mp.p, (const int *)Lf->ml.p, 1,
+                     (int *)nullptr, (const int *)Lc->rp.p, Lc->col.p, Lc->val.p, h->gcol.p,
+                     h->gval.p, h->diag.p);
+  hipLaunchKernelGGL(k_amg_dinv, dim3(nblocks(nc, 256)), dim3(256), 0, c->s, nc, h->diag.p,
+                     Lc->dinv.p);
+  APC_HIP(c, hipGetLastError());
+  return 0;
+}
+
 // Galerkin product into level Lc (CSR) from `ne` generated entries
 static int galerkin_reduce(apc_ctx *c, AmgHier *h, size_t ne, int nc, AmgLevel *Lc) {
   if (ne >= (size_t)INT32_MAX) return fail(c, APC_ERR_UNSUPPORTED, "amg: too many entries");
@@ -1938,8 +2019,7 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
       KScope ks1(c, "amg_setup_galerkin");
       if (o.galerkin_sorted) r = galerkin_sorted(c, h, A, Lf, l, n, nc, Lc);
       else if (Lf->ap && l == 0)
-        r = galerkin_rows(c, h, ApSellView{Lf->apoff.p, Lf->apcol.p, h->apcnt.p, Lf->apval.p}, Lf,
-                          n, nc, Lc, true);
+        r = galerkin_rows_ap(c, h, Lf, n, nc, Lc);
       else if (Lf->ap)
         r = galerkin_rows(c, h, CsrView{Csr<double>{h->aprp.p, h->apc.p, h->apv.p, n}}, Lf, n, nc,
                           Lc, true);
