@@ -755,12 +755,13 @@ __global__ void k_amg_convert(size_t n, const S *__restrict__ a, D *__restrict__
 }
 
 // Level 0's product A P with the aggregation's piecewise-constant
-// prolongation: for every aggregate J that row i's entries meet (the
-// diagonal's aggregate first, then the slots' in slot order), the sum of
-// a_ij over the columns j in J, added in slot order.  Stored like the coarse
-// levels' sliced ELL: 64-row slices (one wave each), slot-major, slice s at
-// s * KMAX * 64, padded to the slice's longest row with (the row's own
-// aggregate, 0).  Per slice: its offset, width and number of real entries;
+// prolongation: for every aggregate J that row i's entries meet (its
+// diagonal and its slots), the sum of a_ij over the columns j in J, added in
+// slot order, the aggregates in increasing order (repeated minima over the
+// row's entries held in registers).  Stored like the coarse levels' sliced
+// ELL: 64-row slices (one wave each), slot-major, slice s at s * KMAX * 64,
+// padded to the slice's longest row with (the row's own aggregate, 0).  Per
+// slice: its offset, width and number of real entries; per row its count;
 // val32 (optional) gets the FP32 copy.
 template <int KMAX>
 __global__ void __launch_bounds__(256) k_ap_build(Op<double> A, const int *__restrict__ agg,
@@ -776,7 +777,8 @@ __global__ void __launch_bounds__(256) k_ap_build(Op<double> A, const int *__res
   int key[KMAX];
   double v[KMAX];
   int m[KMAX - 1];
-  key[0] = live ? agg[i] : -1;
+  const int own = live ? agg[i] : INT_MAX;
+  key[0] = own;
   v[0] = live ? A.d[i] : 0.0;
   const double fi = live ? A.f[i] : 0.0;
 #pragma unroll
@@ -785,26 +787,28 @@ __global__ void __launch_bounds__(256) k_ap_build(Op<double> A, const int *__res
 #pragma unroll
   for (int q = 0; q < KMAX - 1; ++q) {
     const bool in = m[q] < A.M;
-    key[q + 1] = in ? agg[m[q]] : -1;
+    key[q + 1] = in ? agg[m[q]] : INT_MAX;
     v[q + 1] = in ? fi * A.s[(size_t)q * A.M + i] : 0.0;
   }
   const long long base = (long long)s * KMAX * 64 + lane;
+  int cur = INT_MAX;
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) cur = min(cur, key[k]);
   int cnt = 0;
+  while (cur != INT_MAX) {
+    double sum = 0.0;
+    int nxt = INT_MAX;
 #pragma unroll
-  for (int k = 0; k < KMAX; ++k) {
-    bool first = key[k] >= 0;
-#pragma unroll
-    for (int k2 = 0; k2 < k; ++k2) first = first && key[k2] != key[k];
-    if (first) {
-      double sum = v[k];
-#pragma unroll
-      for (int k2 = k + 1; k2 < KMAX; ++k2) sum += (key[k2] == key[k]) ? v[k2] : 0.0;
-      const long long e = base + (long long)cnt * 64;
-      col[e] = key[k];
-      val[e] = sum;
-      if (val32) val32[e] = (float)sum;
-      ++cnt;
+    for (int k = 0; k < KMAX; ++k) {
+      if (key[k] == cur) sum += v[k];
+      else if (key[k] > cur) nxt = min(nxt, key[k]);
     }
+    const long long e = base + (long long)cnt * 64;
+    col[e] = cur;
+    val[e] = sum;
+    if (val32) val32[e] = (float)sum;
+    ++cnt;
+    cur = nxt;
   }
   int w = cnt, tot = cnt;
   for (int off = 32; off > 0; off >>= 1) {
@@ -815,7 +819,7 @@ __global__ void __launch_bounds__(256) k_ap_build(Op<double> A, const int *__res
     rcnt[i] = cnt;
     for (int k = cnt; k < w; ++k) {
       const long long e = base + (long long)k * 64;
-      col[e] = key[0];
+      col[e] = own;
       val[e] = 0.0;
       if (val32) val32[e] = 0.0f;
     }
