@@ -54,21 +54,28 @@ struct AmgLevel {
   DBuf<unsigned char> pass;
   DBuf<double> x, b, r, t;
   DBuf<float> x32, b32, r32, t32;
-  // level 0 with AmgOptions::ap: the product A P (rows of A, columns the
-  // aggregates), sliced ELL like the coarse levels; ap_nnz counts its
-  // entries, ap_len the padded slices
+  // with AmgOptions::ap: the product A P (rows of A, columns the
+  // aggregates).  Coarse levels: sliced ELL like their A.  Level 0:
+  // compacted slices (k_ap_build): slot k of a 64-row slice holds only the
+  // rows with more than k entries, in row order, after slots 0..k-1; apn
+  // counts each row's entries, apss / apmask give each slot's start in its
+  // slice and its rows (bit = row within the slice).  ap_nnz counts the
+  // entries, ap_len the slots x 64 of padded slices.
   bool ap = false;
   long long ap_nnz = 0, ap_len = 0;
   DBuf<int> apoff, apwid, apcol;
   DBuf<double> apval;
   DBuf<float> apval32;
+  DBuf<unsigned char> apn;
+  DBuf<int> apss;
+  DBuf<unsigned long long> apmask;
   // packed boxes, coarse levels: the pass flag of each aggregate's rows (an
   // aggregate never joins rows of two boxes, so its rows share the flag)
   DBuf<unsigned char> apass;
   void release() {
     ap = false;
     apoff.release(); apwid.release(); apcol.release(); apval.release(); apval32.release();
-    apass.release();
+    apn.release(); apss.release(); apmask.release(); apass.release();
     box.release(); pass.release();
     soff.release(); swid.release(); scol.release(); sval.release(); sval32.release();
     rp.release(); col.release(); val.release(); dinv.release(); val32.release();
@@ -662,15 +669,18 @@ __global__ void __launch_bounds__(64) k_gal_rows_ell(int nc, Op<double> Am,
   }
 }
 
-// The level-0 product P^T (A P) from A P's sliced ELL without an emission
-// step: coarse row I's entries are its members' A P rows, member k's entry
-// q (q below the members' widest row, W) at e = k W + q, taken in member
-// order; A P is kApMax wide at most, so pass 0 writes row I's results at
-// kApMax mp[I].
+// The level-0 product P^T (A P) from A P's compacted slices without an
+// emission step: coarse row I's entries are its members' A P rows, member
+// k's entry q (q below the members' widest row, W) at e = k W + q, taken in
+// member order; A P is kApMax wide at most, so pass 0 writes row I's
+// results at kApMax mp[I].  Row i's entry q: slot q's start in i's slice
+// plus the slot's rows before i.
 __global__ void __launch_bounds__(64) k_gal_rows_ap(int nc, const int *__restrict__ soff,
                                                     const int *__restrict__ pcol,
                                                     const double *__restrict__ pval,
-                                                    const int *__restrict__ pcnt,
+                                                    const unsigned char *__restrict__ pcnt,
+                                                    const int *__restrict__ sstart,
+                                                    const unsigned long long *__restrict__ smask,
                                                     const int *__restrict__ mp,
                                                     const int *__restrict__ ml, int pass,
                                                     int *__restrict__ ucnt,
@@ -691,7 +701,9 @@ __global__ void __launch_bounds__(64) k_gal_rows_ap(int nc, const int *__restric
       const int k = e / W, q = e - k * W;
       const int i = ml[k0 + k];
       if (q < pcnt[i]) {
-        const long long p = (long long)soff[i >> 6] * 64 + (i & 63) + (long long)q * 64;
+        const int sq = (i >> 6) * kApMax + q;
+        const long long p = (long long)soff[i >> 6] * 64 + sstart[sq] +
+                            __popcll(smask[sq] & ((1ull << (i & 63)) - 1));
         J = pcol[p];
         A = pval[p];
       }
@@ -758,18 +770,22 @@ __global__ void k_amg_convert(size_t n, const S *__restrict__ a, D *__restrict__
 // prolongation: for every aggregate J that row i's entries meet (its
 // diagonal and its slots), the sum of a_ij over the columns j in J, added in
 // slot order, the aggregates in increasing order (repeated minima over the
-// row's entries held in registers).  Stored like the coarse levels' sliced
-// ELL: 64-row slices (one wave each), slot-major, slice s at s * KMAX * 64,
-// padded to the slice's longest row with (the row's own aggregate, 0).  Per
-// slice: its offset, width and number of real entries; per row its count;
-// val32 (optional) gets the FP32 copy.
+// row's entries held in registers).  Stored in compacted slices: 64-row
+// slices (one wave each), slice s at s * KMAX * 64; slot k holds the k-th
+// entry of every row with more than k entries, in row order, right after
+// slot k - 1 (no padding: a slot's rows are its bit mask, a row's place the
+// mask's bits below it).  Per slice: its offset (in units of 64), its
+// width (largest count) and number of entries, and per slot its start and
+// mask; per row its count.  val32 (optional) gets the FP32 copy.
 template <int KMAX>
 __global__ void __launch_bounds__(256) k_ap_build(Op<double> A, const int *__restrict__ agg,
                                                   int *__restrict__ soff, int *__restrict__ wid,
                                                   int *__restrict__ snnz, int *__restrict__ col,
                                                   double *__restrict__ val,
                                                   float *__restrict__ val32,
-                                                  int *__restrict__ rcnt) {
+                                                  unsigned char *__restrict__ ncnt,
+                                                  int *__restrict__ sstart,
+                                                  unsigned long long *__restrict__ smask) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int s = i >> 6, lane = threadIdx.x & 63;
   if (s >= (A.M + 63) >> 6) return;  // s is uniform over a wave: whole waves leave
@@ -790,44 +806,42 @@ __global__ void __launch_bounds__(256) k_ap_build(Op<double> A, const int *__res
     key[q + 1] = in ? agg[m[q]] : INT_MAX;
     v[q + 1] = in ? fi * A.s[(size_t)q * A.M + i] : 0.0;
   }
-  const long long base = (long long)s * KMAX * 64 + lane;
+  const long long base = (long long)s * KMAX * 64;
   int cur = INT_MAX;
 #pragma unroll
   for (int k = 0; k < KMAX; ++k) cur = min(cur, key[k]);
-  int cnt = 0;
-  while (cur != INT_MAX) {
-    double sum = 0.0;
-    int nxt = INT_MAX;
+  // slot k: every row with a k-th aggregate left; a row has at most KMAX
+  // aggregates, so k < KMAX
+  int cnt = 0, pos = 0, k = 0;
+  for (;; ++k) {
+    const unsigned long long mk = __ballot(cur != INT_MAX);
+    if (mk == 0) break;
+    if (lane == 0) {
+      sstart[s * KMAX + k] = pos;
+      smask[s * KMAX + k] = mk;
+    }
+    if (cur != INT_MAX) {
+      double sum = 0.0;
+      int nxt = INT_MAX;
 #pragma unroll
-    for (int k = 0; k < KMAX; ++k) {
-      if (key[k] == cur) sum += v[k];
-      else if (key[k] > cur) nxt = min(nxt, key[k]);
+      for (int q = 0; q < KMAX; ++q) {
+        if (key[q] == cur) sum += v[q];
+        else if (key[q] > cur) nxt = min(nxt, key[q]);
+      }
+      const long long e = base + pos + below(mk);
+      col[e] = cur;
+      val[e] = sum;
+      if (val32) val32[e] = (float)sum;
+      ++cnt;
+      cur = nxt;
     }
-    const long long e = base + (long long)cnt * 64;
-    col[e] = cur;
-    val[e] = sum;
-    if (val32) val32[e] = (float)sum;
-    ++cnt;
-    cur = nxt;
+    pos += __popcll(mk);
   }
-  int w = cnt, tot = cnt;
-  for (int off = 32; off > 0; off >>= 1) {
-    w = max(w, __shfl_xor(w, off, 64));
-    tot += __shfl_xor(tot, off, 64);
-  }
-  if (live) {
-    rcnt[i] = cnt;
-    for (int k = cnt; k < w; ++k) {
-      const long long e = base + (long long)k * 64;
-      col[e] = own;
-      val[e] = 0.0;
-      if (val32) val32[e] = 0.0f;
-    }
-  }
+  if (live) ncnt[i] = (unsigned char)cnt;
   if (lane == 0) {
     soff[s] = s * KMAX;
-    wid[s] = w;
-    snnz[s] = tot;
+    wid[s] = k;
+    snnz[s] = pos;
   }
 }
 
@@ -1015,13 +1029,28 @@ __global__ void k_cy_prolong(int n, const int *__restrict__ agg, const T *__rest
   x[i] = (T)((double)x[i] + a * (double)xc[agg[i]]);
 }
 
-// (A P e)_i on level 0: the row's sliced-ELL entries (width uniform over
-// the wave), sixteen index and weight loads issued before their gathers
+// Level 0's A P in compacted slices (k_ap_build): per slice its offset (in
+// units of 64 entries) and width, per row its count
 template <class T>
-__device__ inline double ap_row(const Sell<T> &P, const T *__restrict__ e, int i) {
+struct CSell {
+  const int *soff, *wid;
+  const unsigned char *cnt;
+  const int *col;
+  const T *val;
+  int n;
+};
+
+// (A P e)_i on level 0, called by every live row of the wave (row i of the
+// wave's slice): slot k's rows are the lanes whose count exceeds k, a row's
+// entry the slot's start plus its rank among them.  Sixteen index and
+// weight loads are issued before their gathers; the additions run in slot
+// order.
+template <class T>
+__device__ inline double ap_row(const CSell<T> &P, const T *__restrict__ e, int i) {
   const int s = i >> 6;
-  const long long base = (long long)P.soff[s] * 64 + (i & 63);
+  long long base = (long long)P.soff[s] * 64;
   const int w = P.wid[s];
+  const int c = P.cnt[i];
   double acc = 0.0;
   for (int k0 = 0; k0 < w; k0 += 16) {
     int cc[16];
@@ -1029,16 +1058,21 @@ __device__ inline double ap_row(const Sell<T> &P, const T *__restrict__ e, int i
 #pragma unroll
     for (int u = 0; u < 16; ++u)
       if (k0 + u < w) {
-        const long long q = base + (long long)(k0 + u) * 64;
-        cc[u] = P.col[q];
-        vv[u] = P.val[q];
+        const bool on = k0 + u < c;
+        const uint64_t mk = __ballot(on);
+        if (on) {
+          const long long q = base + below(mk);
+          cc[u] = P.col[q];
+          vv[u] = P.val[q];
+        }
+        base += __popcll(mk);
       }
 #pragma unroll
     for (int u = 0; u < 16; ++u)
-      if (k0 + u < w) ee[u] = e[cc[u]];
+      if (k0 + u < c) ee[u] = e[cc[u]];
 #pragma unroll
     for (int u = 0; u < 16; ++u)
-      if (k0 + u < w) acc += (double)vv[u] * (double)ee[u];
+      if (k0 + u < c) acc += (double)vv[u] * (double)ee[u];
   }
   return acc;
 }
@@ -1047,7 +1081,7 @@ __device__ inline double ap_row(const Sell<T> &P, const T *__restrict__ e, int i
 // and r -= alpha (A P e), i.e. r = b - A x for the corrected x without
 // applying A again.
 template <class T>
-__global__ void __launch_bounds__(256) k_cy_ap_resid(int n, const int *__restrict__ agg, Sell<T> P,
+__global__ void __launch_bounds__(256) k_cy_ap_resid(int n, const int *__restrict__ agg, CSell<T> P,
                                                      const T *__restrict__ e, T *__restrict__ x,
                                                      T *__restrict__ r, double alpha,
                                                      const GState *st) {
@@ -1063,7 +1097,7 @@ __global__ void __launch_bounds__(256) k_cy_ap_resid(int n, const int *__restric
 // is b itself).
 template <class T>
 __global__ void __launch_bounds__(256) k_cy_ap_resid_first(int n, const int *__restrict__ agg,
-                                                           Sell<T> P, const T *__restrict__ e,
+                                                           CSell<T> P, const T *__restrict__ e,
                                                            const T *__restrict__ b,
                                                            T *__restrict__ x, T *__restrict__ r,
                                                            double alpha, const GState *st) {
@@ -1078,7 +1112,7 @@ __global__ void __launch_bounds__(256) k_cy_ap_resid_first(int n, const int *__r
 // post-smoothing step, x2 = x + alpha e[agg], xout = x2 + omega D^-1 (b - A x2)
 // with b - A x2 = r - alpha (A P e) (r = b - A x on entry).
 template <class T>
-__global__ void __launch_bounds__(256) k_cy_ap_sweep(int n, const int *__restrict__ agg, Sell<T> P,
+__global__ void __launch_bounds__(256) k_cy_ap_sweep(int n, const int *__restrict__ agg, CSell<T> P,
                                                      const T *__restrict__ e,
                                                      const T *__restrict__ dinv,
                                                      const T *__restrict__ xin,
@@ -1479,7 +1513,7 @@ static int galerkin_rows_ell(apc_ctx *c, AmgHier *h, const Op<double> &A, AmgLev
   return 0;
 }
 
-// P^T (A P) of level 0 into Lc (CSR) from A P's sliced ELL (k_gal_rows_ap)
+// P^T (A P) of level 0 into Lc (CSR) from A P's compacted slices (k_gal_rows_ap)
 static int galerkin_rows_ap(apc_ctx *c, AmgHier *h, AmgLevel *Lf, int n, int nc, AmgLevel *Lc) {
   const size_t W = (size_t)kApMax;
   if (W * (size_t)n >= (size_t)INT32_MAX)  // coarse-row offsets are int
@@ -1493,7 +1527,9 @@ static int galerkin_rows_ap(apc_ctx *c, AmgHier *h, AmgLevel *Lf, int n, int nc,
   const unsigned grid = (unsigned)std::min(nc, 1 << 16);
   hipLaunchKernelGGL(k_gal_rows_ap, dim3(grid), dim3(64), 0, c->s, nc, (const int *)Lf->apoff.p,
                      (const int *)Lf->apcol.p, (const double *)Lf->apval.p,
-                     (const int *)h->apcnt.p, (const int *)Lf->mp.p, (const int *)Lf->ml.p, 0,
+                     (const unsigned char *)Lf->apn.p, (const int *)Lf->apss.p,
+                     (const unsigned long long *)Lf->apmask.p, (const int *)Lf->mp.p,
+                     (const int *)Lf->ml.p, 0,
                      h->flag.p, (const int *)nullptr, (int *)nullptr, (double *)nullptr,
                      h->gcol.p, h->gval.p, h->diag.p);
   APC_ALLOC(c, h->sb, (size_t)nc + 1);
@@ -1512,7 +1548,9 @@ static int galerkin_rows_ap(apc_ctx *c, AmgHier *h, AmgLevel *Lf, int n, int nc,
   APC_HIP(c, hipMemcpyAsync(Lc->rp.p + nc, h->cnt.p + 1, sizeof(int), hipMemcpyDeviceToDevice, c->s));
   hipLaunchKernelGGL(k_gal_rows_ap, dim3(grid), dim3(64), 0, c->s, nc, (const int *)Lf->apoff.p,
                      (const int *)Lf->apcol.p, (const double *)Lf->apval.p,
-                     (const int *)h->apcnt.p, (const int *)Lf->mp.p, (const int *)Lf->ml.p, 1,
+                     (const unsigned char *)Lf->apn.p, (const int *)Lf->apss.p,
+                     (const unsigned long long *)Lf->apmask.p, (const int *)Lf->mp.p,
+                     (const int *)Lf->ml.p, 1,
                      (int *)nullptr, (const int *)Lc->rp.p, Lc->col.p, Lc->val.p, h->gcol.p,
                      h->gval.p, h->diag.p);
   hipLaunchKernelGGL(k_amg_dinv, dim3(nblocks(nc, 256)), dim3(256), 0, c->s, nc, h->diag.p,
@@ -1598,13 +1636,15 @@ static int build_ap(apc_ctx *c, AmgHier *h, const Op<double> &A, AmgLevel *L, bo
   APC_ALLOC(c, L->apwid, ns + 1);
   APC_ALLOC(c, h->apsn, ns + 1);
   APC_ALLOC(c, h->aptot, 2);
-  APC_ALLOC(c, h->apcnt, (size_t)n + 1);
+  APC_ALLOC(c, L->apn, (size_t)n + 1);
+  APC_ALLOC(c, L->apss, (size_t)ns * kApMax + 1);
+  APC_ALLOC(c, L->apmask, (size_t)ns * kApMax + 1);
   APC_ALLOC(c, L->apcol, cap + 1);
   APC_ALLOC(c, L->apval, cap + 1);
   if (fp32) APC_ALLOC(c, L->apval32, cap + 1);
   hipLaunchKernelGGL((k_ap_build<kApMax>), dim3(nblocks(n, 256)), dim3(256), 0, c->s, A, L->agg.p,
                      L->apoff.p, L->apwid.p, h->apsn.p, L->apcol.p, L->apval.p,
-                     fp32 ? L->apval32.p : nullptr, h->apcnt.p);
+                     fp32 ? L->apval32.p : nullptr, L->apn.p, L->apss.p, L->apmask.p);
   hipLaunchKernelGGL(k_ap_total, dim3(1), dim3(1024), 0, c->s, ns, (const int *)L->apwid.p,
                      (const int *)h->apsn.p, h->aptot.p);
   long long tot[2] = {0, 0};
@@ -2225,7 +2265,10 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
   // post-smoothing step's) comes from r -= alpha (A P) e
   const bool apf = L->ap;
   const Sell<T> AP{L->apoff.p, L->apwid.p, L->apcol.p, apf ? ap_val<T>(L) : nullptr, n};
-  const double ap_bytes = apf ? (double)n * (4 + 4 * sT) + (double)L->ap_nnz * (4 + sT) +
+  const CSell<T> AP0{L->apoff.p, L->apwid.p, L->apn.p, L->apcol.p,
+                     apf ? ap_val<T>(L) : nullptr, n};
+  const double ap_bytes = apf ? (double)n * (4 + 4 * sT + (ell ? 1 : 0)) +
+                                    (double)L->ap_nnz * (4 + sT) +
                                     (double)Lc->n * sT
                               : 0.0;
   int post_done = 0;
@@ -2278,12 +2321,12 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
     if (ell && first) {
       KScope ks(c, f32 ? "amg_ap_resid0_32" : "amg_ap_resid0", ap_bytes);
       hipLaunchKernelGGL((k_cy_ap_resid_first<T>), dim3(g), dim3(256), 0, c->s, n,
-                         (const int *)L->agg.p, AP, (const T *)Pc.x, b, cur, r, o.alpha, st);
+                         (const int *)L->agg.p, AP0, (const T *)Pc.x, b, cur, r, o.alpha, st);
     } else if (apf && q + 1 < gam) {
       if (ell) {
         KScope ks(c, f32 ? "amg_ap_resid0_32" : "amg_ap_resid0", ap_bytes);
         hipLaunchKernelGGL((k_cy_ap_resid<T>), dim3(g), dim3(256), 0, c->s, n,
-                           (const int *)L->agg.p, AP, (const T *)Pc.x, cur, r, o.alpha, st);
+                           (const int *)L->agg.p, AP0, (const T *)Pc.x, cur, r, o.alpha, st);
       } else {
         KScope ks(c, kApResidName[l]);
         auto *kern = first ? (gl == 8 ? k_cy_ap_resid_g<T, 8, true>
@@ -2299,7 +2342,7 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
       if (ell) {
         KScope ks(c, f32 ? "amg_ap_sweep0_32" : "amg_ap_sweep0", ap_bytes);
         hipLaunchKernelGGL((k_cy_ap_sweep<T>), dim3(g), dim3(256), 0, c->s, n,
-                           (const int *)L->agg.p, AP, (const T *)Pc.x, dinv, (const T *)cur,
+                           (const int *)L->agg.p, AP0, (const T *)Pc.x, dinv, (const T *)cur,
                            (const T *)r, alt, o.alpha, o.omega, st);
       } else {
         KScope ks(c, kApSweepName[l]);
