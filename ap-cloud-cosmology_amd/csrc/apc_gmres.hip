@@ -203,10 +203,12 @@ __global__ void __launch_bounds__(256) k_mdot_fin(int nv, int nb, GDev g) {
 // ---- w -= sum_q h_q sig_q w_q ; ||w||^2 partials ----
 // Grid-stride over rows; the update is accumulated in FP64 whatever the
 // storage type, U basis vectors per step (U independent loads in flight per
-// row).
+// row).  w32 (flexible form, may be null): the FP32 copy of the new w, the
+// next step's multigrid input.
 template <class T, int U>
 __global__ void __launch_bounds__(256) k_maxpy(int M, int nv, const T *__restrict__ V, size_t ld,
-                                               T *__restrict__ w, GDev g) {
+                                               T *__restrict__ w, float *__restrict__ w32,
+                                               GDev g) {
   extern __shared__ double cq[];  // nv coefficients
   __shared__ double ws[4];
   if (halted(g.st)) return;
@@ -230,6 +232,7 @@ __global__ void __launch_bounds__(256) k_maxpy(int M, int nv, const T *__restric
     for (; q < nv; ++q) s -= cq[q] * (double)vi[(size_t)q * ld];
     const T sv = (T)s;
     w[i] = sv;
+    if (w32) w32[i] = (float)sv;
     acc += (double)sv * (double)sv;
   }
   acc = block_sum256(acc, ws);
@@ -564,8 +567,9 @@ static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, 
       T *w = V + (size_t)(j + 1) * ld;
       if (flex) {
         float *zj = Z32 + (size_t)j * ld;
-        hipLaunchKernelGGL(k_to_float_gated, dim3(gb), dim3(256), 0, c->s, M, (const double *)vj,
-                           v32, (const GState *)S->st.p);
+        if (j == 0)  // later steps: the previous step's k_maxpy wrote v32
+          hipLaunchKernelGGL(k_to_float_gated, dim3(gb), dim3(256), 0, c->s, M,
+                             (const double *)vj, v32, (const GState *)S->st.p);
         r = amg_apply<float>(c, S->amg, *A32, v32, z32, S->st.p);
         if (r) return r;
         KScope ks(c, "gmres_spmv_z", 12.0 * nnz + 32.0 * M);
@@ -593,9 +597,10 @@ static int run_cycles(apc_ctx *c, const Op<double> &A64, const Op<T> &AT, T *V, 
         hipLaunchKernelGGL(k_mdot_fin, dim3(nblocks(nv, 4)), dim3(256), 0, c->s, nv, (int)gdot, g);
       }
       {
-        KScope ks(c, fp32 ? "gmres_maxpy32" : "gmres_maxpy", (double)sizeof(T) * M * (nv + 2));
+        KScope ks(c, fp32 ? "gmres_maxpy32" : "gmres_maxpy", (double)sizeof(T) * M * (nv + 2) +
+                                                                (flex ? 4.0 * M : 0.0));
         hipLaunchKernelGGL((k_maxpy<T, MU>), dim3(gaxpy), dim3(256), nv * sizeof(double), c->s, M,
-                           nv, V, ld, w, g);
+                           nv, V, ld, w, flex ? v32 : nullptr, g);
       }
       {
         KScope ks(c, "gmres_hess");
