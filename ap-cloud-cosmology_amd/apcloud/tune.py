@@ -24,7 +24,7 @@ import time
 # cycle-shape knobs, costed by the step's wall time (tuned first, in order)
 WALL = {
     "amg_nopre": [0, 1, 2],
-    "amg_wlevels": [2, 3, 4],
+    "amg_wlevels": [2, 3],
 }
 
 # candidate values per knob (the built-in default is always among them)

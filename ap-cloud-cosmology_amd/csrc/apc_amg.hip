@@ -131,6 +131,10 @@ struct AmgHier {
   DBuf<long long> aptot;
   DBuf<int> apiota, aprp, apcnt, apc;  // coarse levels' A P: rows as one-row groups, CSR
   DBuf<double> apv, apdiag;
+  // geometric aggregation: the octree cell (x, y, z, level) of every row of
+  // the level being aggregated, and of the next level's rows
+  DBuf<int4> gcell, gcell2;
+  bool geo_prev = false;  // the last level built was aggregated geometrically
   hipStream_t s = nullptr;  // the owning context's stream (graphs are launched on it)
   ~AmgHier() {
     for (auto &g : graphs) g.release(s);
@@ -141,7 +145,7 @@ struct AmgHier {
     pos.release(); cnt.release(); ev.release(); smax.release(); diag.release();
     gcol.release(); gval.release(); apsn.release(); aptot.release(); rcnt.release();
     apiota.release(); aprp.release(); apcnt.release(); apc.release(); apv.release();
-    apdiag.release();
+    apdiag.release(); gcell.release(); gcell2.release();
   }
 };
 
@@ -1401,6 +1405,157 @@ static int aggregate_level(apc_ctx *c, AmgHier *h, const V &v, int n, AmgLevel *
   return 0;
 }
 
+// ---- geometric aggregation: an aggregate is every row whose octree cell
+// lies in one ancestor cell `up` levels above it (the rows are the octree's
+// computational nodes, so a level-0 aggregate is a compact block of up to
+// 8^up leaves, and a coarse row is its ancestor cell).
+
+// level 0: each row's cell from its centre and width (width = extent / 2^L)
+__global__ void k_geo_cells0(int n, const double *__restrict__ cx, const double *__restrict__ cy,
+                             const double *__restrict__ cz, const double *__restrict__ ch,
+                             double x0, double ex, double y0, double ey, double z0, double ez,
+                             int4 *__restrict__ cell) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int L = min(max((int)rint(log2(ex / ch[i])), 0), 21);
+  const double s = ldexp(1.0, L);
+  const int top = (1 << L) - 1;
+  const int ix = min(max((int)floor((cx[i] - x0) / ex * s), 0), top);
+  const int iy = min(max((int)floor((cy[i] - y0) / ey * s), 0), top);
+  const int iz = min(max((int)floor((cz[i] - z0) / ez * s), 0), top);
+  cell[i] = make_int4(ix, iy, iz, L);
+}
+
+// key of row i's ancestor cell: the Morton code of its corner at kGeoRes
+// bits per axis, then its level (5 bits); the sort groups an ancestor's
+// rows and orders the ancestors along the Morton curve.  bad counts cells
+// whose ancestor lies deeper than kGeoRes.
+constexpr int kGeoRes = 19;
+__device__ inline uint64_t geo_key(int4 c, int up) {
+  const int la = max(c.w - up, 0), sh = c.w - la;
+  const int up2 = kGeoRes - min(la, kGeoRes);
+  const uint64_t m = spread3((uint32_t)(c.x >> sh) << up2) |
+                     (spread3((uint32_t)(c.y >> sh) << up2) << 1) |
+                     (spread3((uint32_t)(c.z >> sh) << up2) << 2);
+  return (m << 5) | (uint64_t)la;
+}
+
+__global__ void k_geo_keys(int n, const int4 *__restrict__ cell, int up,
+                           uint64_t *__restrict__ key, int *__restrict__ idx,
+                           int *__restrict__ bad) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int4 c = cell[i];
+  if (c.w - up > kGeoRes) atomicAdd(bad, 1);
+  key[i] = geo_key(c, up);
+  idx[i] = i;
+}
+
+// packed boxes: after the sort by cell, the box of each sorted row as the
+// key of a last stable pass (box-major aggregates, never two boxes in one)
+__global__ void k_geo_boxkeys(int n, const int *__restrict__ idx, const int *__restrict__ box,
+                              uint64_t *__restrict__ key) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s < n) key[s] = (uint64_t)box[idx[s]];
+}
+
+__global__ void k_geo_flags_box(int n, const int *__restrict__ idx, const int *__restrict__ box,
+                                const int4 *__restrict__ cell, int up, int *__restrict__ f) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  if (s == 0) { f[0] = 1; return; }
+  const int i = idx[s], j = idx[s - 1];
+  f[s] = (box[i] != box[j] || geo_key(cell[i], up) != geo_key(cell[j], up)) ? 1 : 0;
+}
+
+__global__ void k_geo_flags(int n, const uint64_t *__restrict__ key, int *__restrict__ f) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s < n) f[s] = (s == 0 || key[s] != key[s - 1]) ? 1 : 0;
+}
+
+// sorted position s: its row joins aggregate a (the runs of equal keys in
+// order); the members list is the sorted rows (row order within a run: the
+// sort is stable), a run's start its offset; the run's first row writes
+// the aggregate's cell
+__global__ void k_geo_assign(int n, int nc, const int *__restrict__ idx,
+                             const int *__restrict__ f, const int *__restrict__ pos,
+                             const int4 *__restrict__ cell, int up, int *__restrict__ agg,
+                             int *__restrict__ mp, int *__restrict__ ml,
+                             int4 *__restrict__ ccell) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  const int i = idx[s];
+  const int a = pos[s] + f[s] - 1;
+  agg[i] = a;
+  ml[s] = i;
+  if (f[s]) {
+    mp[a] = s;
+    const int4 c = cell[i];
+    const int la = max(c.w - up, 0), sh = c.w - la;
+    ccell[a] = make_int4(c.x >> sh, c.y >> sh, c.z >> sh, la);
+  }
+  if (s == n - 1) mp[nc] = n;
+}
+
+
+// Level l's aggregates by ancestor cells `up` levels above its rows' cells
+// (level 0: the computational nodes' cells; below: the cells the previous
+// call left in h->gcell).  Writes agg, the members (mp, ml) and the next
+// level's cells; *nc_out = the number of aggregates.  box (packed boxes):
+// each row's box; the aggregates are then box-major.
+static int aggregate_geo(apc_ctx *c, AmgHier *h, int l, int n, int up, AmgLevel *Lf,
+                         int *nc_out, const int *box) {
+  const unsigned g = nblocks(n, 256);
+  APC_ALLOC(c, h->key, n + 1);
+  APC_ALLOC(c, h->key2, n + 1);
+  APC_ALLOC(c, h->idx, n + 1);
+  APC_ALLOC(c, h->idx2, n + 1);
+  APC_ALLOC(c, h->flag, n + 1);
+  APC_ALLOC(c, h->pos, n + 1);
+  APC_ALLOC(c, h->cnt, 4);
+  APC_ALLOC(c, h->gcell2, n + 1);
+  if (l == 0) {
+    APC_ALLOC(c, h->gcell, n + 1);
+    const double *bb = c->prm.bbox;
+    hipLaunchKernelGGL(k_geo_cells0, dim3(g), dim3(256), 0, c->s, n, c->cx.p, c->cy.p, c->cz.p,
+                       c->ch.p, bb[0], bb[1] - bb[0], bb[2], bb[3] - bb[2], bb[4], bb[5] - bb[4],
+                       h->gcell.p);
+  }
+  APC_HIP(c, hipMemsetAsync(h->cnt.p, 0, 4 * sizeof(int), c->s));
+  hipLaunchKernelGGL(k_geo_keys, dim3(g), dim3(256), 0, c->s, n, (const int4 *)h->gcell.p, up,
+                     h->key.p, h->idx.p, h->cnt.p + 2);
+  int r = radix_sort_pairs(c, h->key.p, h->idx.p, h->key2.p, h->idx2.p, n, 3 * kGeoRes + 5);
+  if (r) return r;
+  if (box) {
+    hipLaunchKernelGGL(k_geo_boxkeys, dim3(g), dim3(256), 0, c->s, n, (const int *)h->idx.p, box,
+                       h->key.p);
+    r = radix_sort_pairs(c, h->key.p, h->idx.p, h->key2.p, h->idx2.p, n, std::max(1, c->boxbits));
+    if (r) return r;
+    hipLaunchKernelGGL(k_geo_flags_box, dim3(g), dim3(256), 0, c->s, n, (const int *)h->idx.p,
+                       box, (const int4 *)h->gcell.p, up, h->flag.p);
+  } else {
+    hipLaunchKernelGGL(k_geo_flags, dim3(g), dim3(256), 0, c->s, n, (const uint64_t *)h->key.p,
+                       h->flag.p);
+  }
+  r = scan_exclusive(c, h->flag.p, h->pos.p, n, h->cnt.p + 1);
+  if (r) return r;
+  int hc[3] = {0, 0, 0};
+  APC_HIP(c, hipMemcpyAsync(hc, h->cnt.p, sizeof(hc), hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  if (hc[2]) return fail(c, APC_ERR_UNSUPPORTED, "amg: %d cells too deep to aggregate", hc[2]);
+  const int nc = hc[1];
+  APC_ALLOC(c, Lf->agg, n + 1);
+  APC_ALLOC(c, Lf->mp, nc + 1);
+  APC_ALLOC(c, Lf->ml, n + 1);
+  hipLaunchKernelGGL(k_geo_assign, dim3(g), dim3(256), 0, c->s, n, nc, (const int *)h->idx.p,
+                     (const int *)h->flag.p, (const int *)h->pos.p, (const int4 *)h->gcell.p, up,
+                     Lf->agg.p, Lf->mp.p, Lf->ml.p, h->gcell2.p);
+  APC_HIP(c, hipGetLastError());
+  std::swap(h->gcell, h->gcell2);
+  *nc_out = nc;
+  return 0;
+}
+
 // rows of each aggregate, grouped in row order (restriction order)
 static int group_members(apc_ctx *c, AmgHier *h, AmgLevel *Lf, int n, int nc) {
   APC_ALLOC(c, h->key, n + 1);
@@ -1996,6 +2151,12 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
       if (!f) return false;
     return true;
   };
+  // the rows are the computational nodes, whose cells the geometric
+  // aggregation reads
+  const bool geo_rows = c->have_comp && c->nc + c->nv == A.M && c->cx.p && c->ch.p &&
+                        c->prm.depth - o.geo0 <= kGeoRes;
+  h->geo_prev = false;
+  if (!(h->o.alpha > 0.0)) h->o.alpha = (o.geo0 > 0 && geo_rows) ? 1.5 : 1.6;
   AmgLevel *L0 = level(0);
   L0->n = A.M;
   L0->nnz = A.M * (A.lm + 1);
@@ -2017,12 +2178,18 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
     const int *boxf = packed ? (l == 0 ? (const int *)c->rowbox.p : (const int *)Lf->box.p)
                              : nullptr;
     int nc = 0;
+    // geometric aggregation where asked for and the rows are the context's
+    // computational nodes (single box); a coarse level only below a
+    // geometric one (its rows are then ancestor cells)
+    const bool geo = l == 0 ? (o.geo0 > 0 && geo_rows) : (!packed && o.geoc > 0 && h->geo_prev);
     {
       KScope ks1(c, "amg_setup_aggregate");
-      if (l == 0) r = aggregate_level(c, h, EllView{A}, n, Lf, &nc, boxf);
+      if (geo) r = aggregate_geo(c, h, l, n, l == 0 ? o.geo0 : o.geoc, Lf, &nc, boxf);
+      else if (l == 0) r = aggregate_level(c, h, EllView{A}, n, Lf, &nc, boxf);
       else r = aggregate_level(c, h, CsrView{Csr<double>{Lf->rp.p, Lf->col.p, Lf->val.p, n}}, n, Lf, &nc, boxf);
     }
     if (r) return r;
+    h->geo_prev = geo;
     if (nc < 1 || nc >= n) break;  // no coarsening: this level is the coarsest
     if (packed) {
       AmgLevel *Ln = level(l + 1);
@@ -2046,7 +2213,7 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
                                 hipMemcpyHostToDevice, c->s));
       APC_HIP(c, hipStreamSynchronize(c->s));  // the flags' host vector changes on the next level
     }
-    {
+    if (!geo) {  // (the geometric aggregation lists the members itself)
       KScope ks1(c, "amg_setup_group");
       r = group_members(c, h, Lf, n, nc);
     }

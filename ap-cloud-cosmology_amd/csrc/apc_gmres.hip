@@ -673,6 +673,8 @@ static AmgOptions amg_options(apc_ctx *c) {
   if (const char *v = getenv("APC_AMG_OMEGA")) a.omega = atof(v);
   if (const char *v = getenv("APC_AMG_ALPHA")) a.alpha = atof(v);
   if (const char *v = getenv("APC_AMG_AP")) a.ap = atoi(v);
+  if (const char *v = getenv("APC_AMG_GEO0")) a.geo0 = atoi(v);
+  if (const char *v = getenv("APC_AMG_GEOC")) a.geoc = atoi(v);
   a.nopre = c->knob[APC_KNOB_AMG_NOPRE];
   a.wlevels = c->knob[APC_KNOB_AMG_WLEVELS];
   if (const char *v = getenv("APC_AMG_THETA")) a.theta = atof(v);

@@ -155,7 +155,7 @@ struct apc_ctx {
   int nsearch_long = 0;
   // launch-shape knobs (APC_KNOB_*): built-in defaults, the APC_* environment
   // overrides read at creation, apc_set_knob (the autotuner) after
-  int knob[APC_KNOB_COUNT] = {256, 1024, 1024, 2048, 65536, 1 << 30, 1 << 30, 1, 3};
+  int knob[APC_KNOB_COUNT] = {256, 1024, 1024, 2048, 65536, 1 << 30, 1 << 30, 2, 3};
   int search_slots = 0;        // resident threads of the neighbour search
   int search_waves = 0;        // resident waves of the wave-per-node search
   int search_pen = 0;          // its penalty table is on the device

@@ -120,7 +120,9 @@ struct AmgHier;  // apc_amg.hip
 struct AmgOptions {
   int nu = 1;             // Jacobi sweeps before and after the coarse correction
   double omega = 0.85;    // Jacobi damping
-  double alpha = 1.6;     // scaling of the coarse correction
+  // scaling of the coarse correction; 0: 1.5 below a geometric level 0,
+  // 1.6 below an independent-set one
+  double alpha = 0.0;
   int gamma = 2;          // 1: V-cycle, 2: W-cycle
   int wlevels = 3;        // gamma applies on the first wlevels levels, V below
   double theta = 0.0;     // strength threshold (fraction of the row's largest |a_ij|)
@@ -146,6 +148,11 @@ struct AmgOptions {
   // step (FP64 154.3 -> 142.5); without pre-smoothing on levels 0-1, 0-2 or
   // all, 35-40 steps and no further gain
   int nopre = 1;
+  // aggregates by octree cells instead of independent sets: level 0's rows
+  // grouped by their ancestor geo0 levels up, each coarse level's by the
+  // ancestor geoc levels up (0: independent-set aggregation; geoc only below
+  // a geometric level)
+  int geo0 = 1, geoc = 0;
 };
 
 // Build the hierarchy for A (FP64); `periodic` shifts the near-null constant

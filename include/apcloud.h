@@ -115,7 +115,7 @@ enum {
   APC_KNOB_AMG_G8_ROWS,      /* multigrid levels of at most this many rows: 8 lanes per row (65536) */
   APC_KNOB_AMG_G4_ROWS,      /* ... else of at most this many rows: 4 lanes per row (2^30) */
   APC_KNOB_AMG_RG8_ROWS,     /* restrictions into at most this many rows: 8 lanes (2^30) */
-  APC_KNOB_AMG_NOPRE,        /* multigrid: finest levels without pre-smoothing (1) */
+  APC_KNOB_AMG_NOPRE,        /* multigrid: finest levels without pre-smoothing (2) */
   APC_KNOB_AMG_WLEVELS,      /* multigrid: finest levels visited by a W-cycle (3) */
   APC_KNOB_COUNT
 };
