@@ -2156,7 +2156,11 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
   const bool geo_rows = c->have_comp && c->nc + c->nv == A.M && c->cx.p && c->ch.p &&
                         c->prm.depth - o.geo0 <= kGeoRes;
   h->geo_prev = false;
-  if (!(h->o.alpha > 0.0)) h->o.alpha = (o.geo0 > 0 && geo_rows) ? 1.5 : 1.6;
+  {  // correction scales (AmgOptions::alpha, alpha0)
+    const bool g0 = o.geo0 > 0 && geo_rows;
+    if (!(h->o.alpha > 0.0)) h->o.alpha = g0 ? 1.5 : 1.6;
+    if (!(h->o.alpha0 > 0.0)) h->o.alpha0 = h->o.alpha;
+  }
   AmgLevel *L0 = level(0);
   L0->n = A.M;
   L0->nnz = A.M * (A.lm + 1);
@@ -2438,6 +2442,8 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
                                     (double)L->ap_nnz * (4 + sT) +
                                     (double)Lc->n * sT
                               : 0.0;
+  // the correction scale of this level (level 0 may differ)
+  const double alpha = (l == 0 && o.alpha0 > 0.0) ? o.alpha0 : o.alpha;
   int post_done = 0;
   for (int q = 0; q < gam; ++q) {
     if ((apf && q > 0) || (nopre && q == 0)) {
@@ -2488,12 +2494,12 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
     if (ell && first) {
       KScope ks(c, f32 ? "amg_ap_resid0_32" : "amg_ap_resid0", ap_bytes);
       hipLaunchKernelGGL((k_cy_ap_resid_first<T>), dim3(g), dim3(256), 0, c->s, n,
-                         (const int *)L->agg.p, AP0, (const T *)Pc.x, b, cur, r, o.alpha, st);
+                         (const int *)L->agg.p, AP0, (const T *)Pc.x, b, cur, r, alpha, st);
     } else if (apf && q + 1 < gam) {
       if (ell) {
         KScope ks(c, f32 ? "amg_ap_resid0_32" : "amg_ap_resid0", ap_bytes);
         hipLaunchKernelGGL((k_cy_ap_resid<T>), dim3(g), dim3(256), 0, c->s, n,
-                           (const int *)L->agg.p, AP0, (const T *)Pc.x, cur, r, o.alpha, st);
+                           (const int *)L->agg.p, AP0, (const T *)Pc.x, cur, r, alpha, st);
       } else {
         KScope ks(c, kApResidName[l]);
         auto *kern = first ? (gl == 8 ? k_cy_ap_resid_g<T, 8, true>
@@ -2503,14 +2509,14 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
                                       : gl == 4 ? k_cy_ap_resid_g<T, 4, false>
                                                 : k_cy_ap_resid_g<T, 1, false>);
         hipLaunchKernelGGL(kern, dim3(gg), dim3(256), 0, c->s, AP, (const int *)L->agg.p,
-                           (const T *)Pc.x, b, cur, r, o.alpha, pass, apass, apass_alpha, st);
+                           (const T *)Pc.x, b, cur, r, alpha, pass, apass, apass_alpha, st);
       }
     } else if (apf && o.nu >= 1) {
       if (ell) {
         KScope ks(c, f32 ? "amg_ap_sweep0_32" : "amg_ap_sweep0", ap_bytes);
         hipLaunchKernelGGL((k_cy_ap_sweep<T>), dim3(g), dim3(256), 0, c->s, n,
                            (const int *)L->agg.p, AP0, (const T *)Pc.x, dinv, (const T *)cur,
-                           (const T *)r, alt, o.alpha, o.omega, st);
+                           (const T *)r, alt, alpha, o.omega, st);
       } else {
         KScope ks(c, kApSweepName[l]);
         auto *kern = first ? (gl == 8 ? k_cy_ap_sweep_g<T, 8, true>
@@ -2522,7 +2528,7 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
         // first: no pre-smoothing, so the residual entering is b itself
         hipLaunchKernelGGL(kern, dim3(gg), dim3(256), 0, c->s, AP, (const int *)L->agg.p,
                            (const T *)Pc.x, dinv, (const T *)cur, first ? b : (const T *)r, alt,
-                           o.alpha, o.omega, pass, apass, apass_alpha, st);
+                           alpha, o.omega, pass, apass, apass_alpha, st);
       }
       std::swap(cur, alt);
       post_done = 1;
@@ -2530,7 +2536,7 @@ static int cycle(apc_ctx *c, AmgHier *h, const Op<T> &A0, int l, const T *b, T *
       KScope ks(c, ell ? (f32 ? "amg_prolong0_32" : "amg_prolong0") : kProlongName[l],
                 ell ? (double)n * (4 + 3 * sT) : 0.0);
       hipLaunchKernelGGL((k_cy_prolong<T>), dim3(g), dim3(256), 0, c->s, n, L->agg.p, Pc.x, cur,
-                         o.alpha, pass, q == 0 ? 1.0 : 0.0, st);
+                         alpha, pass, q == 0 ? 1.0 : 0.0, st);
     }
   }
   for (int s = post_done; s < o.nu; ++s) {

@@ -672,6 +672,7 @@ static AmgOptions amg_options(apc_ctx *c) {
   a.rg8_rows = c->knob[APC_KNOB_AMG_RG8_ROWS];
   if (const char *v = getenv("APC_AMG_OMEGA")) a.omega = atof(v);
   if (const char *v = getenv("APC_AMG_ALPHA")) a.alpha = atof(v);
+  if (const char *v = getenv("APC_AMG_ALPHA0")) a.alpha0 = atof(v);
   if (const char *v = getenv("APC_AMG_AP")) a.ap = atoi(v);
   if (const char *v = getenv("APC_AMG_GEO0")) a.geo0 = atoi(v);
   if (const char *v = getenv("APC_AMG_GEOC")) a.geoc = atoi(v);

@@ -120,9 +120,14 @@ struct AmgHier;  // apc_amg.hip
 struct AmgOptions {
   int nu = 1;             // Jacobi sweeps before and after the coarse correction
   double omega = 0.85;    // Jacobi damping
-  // scaling of the coarse correction; 0: 1.5 below a geometric level 0,
-  // 1.6 below an independent-set one
+  // scaling of the coarse correction on levels >= 1 and on level 0; 0:
+  // 1.5 with a geometric level 0 (at 256^3 1.6 on the coarse levels made
+  // the schedule's cold first solve take 266 steps instead of 61; 1.6-1.8 on
+  // level 0 alone are 2-4% faster, but the converged solves' true residual
+  // then drifts to 2.2-2.7 rtol from the Krylov estimate), else 1.6;
+  // alpha0 0: alpha
   double alpha = 0.0;
+  double alpha0 = 0.0;
   int gamma = 2;          // 1: V-cycle, 2: W-cycle
   int wlevels = 3;        // gamma applies on the first wlevels levels, V below
   double theta = 0.0;     // strength threshold (fraction of the row's largest |a_ij|)
