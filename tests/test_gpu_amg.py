@@ -7,7 +7,7 @@ import pytest
 
 import apcloud
 from helpers import cloud, relabel
-from test_gpu_parity import pair, _selected
+from test_gpu_parity import BB, pair, _selected
 
 pytestmark = pytest.mark.gpu
 
@@ -43,6 +43,47 @@ def test_amg_solution_matches_oracle(kind, n, bc, prec):
         ref = of[f][pm]
         err = np.abs(gf[f] - ref).max() / (np.abs(ref).max() + 1e-300)
         assert err < tol, (f, err)
+
+
+def _spread3(v):
+    out = np.zeros(len(v), np.uint64)
+    for b in range(21):
+        out |= ((v.astype(np.uint64) >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b)
+    return out
+
+
+@pytest.mark.parametrize("kind,bc", [("plummer", 1), ("ball", 0)])
+def test_level0_aggregates_are_parent_cells(kind, bc):
+    """Level 0's aggregates are the octree's parent cells: two rows share an
+    aggregate exactly when their cells have the same parent (cells from the
+    rows' centres and widths, recomputed here), and the aggregates are
+    numbered in increasing order of the parent's Morton code (corner at 19
+    bits per axis) and level."""
+    g, _, gc, _ = _solve_pair(kind, 20000, bc, 29)
+    info = g.solve(apcloud.ksp(rtol=1e-10, restart=60, max_it=2000, pc="amg"))
+    assert info.reason_phi > 0
+    _, agg = g.amg_level(0)
+    M = gc["nc"] + gc["nv"]
+    assert len(agg) == M
+    lo = np.array(BB[0::2]), np.array(BB[1::2])
+    ext = lo[1] - lo[0]
+    L = np.rint(np.log2(ext[0] / gc["h"][:M])).astype(np.int64)
+    pc = []
+    for a, c in enumerate("xyz"):
+        i = np.floor((gc[c][:M] - lo[0][a]) / ext[a] * 2.0 ** L).astype(np.int64)
+        pc.append(i >> 1)  # the parent's coordinates at level L - 1
+    la = L - 1
+    m = sum(_spread3(pc[a] << (19 - la)) << np.uint64(a) for a in range(3))
+    key = (m << np.uint64(5)) | la.astype(np.uint64)
+    # the same partition: one aggregate per parent key and one key per aggregate
+    uk, ki = np.unique(key, return_inverse=True)
+    pairs = np.unique(np.stack([ki, agg]), axis=1)
+    assert pairs.shape[1] == len(uk) == agg.max() + 1, (pairs.shape, len(uk), agg.max())
+    # numbered along the keys
+    order = np.empty(len(uk), np.int64)
+    order[pairs[1]] = pairs[0]
+    assert np.all(np.diff(order) > 0)
+    g.close()
 
 
 def test_amg_fp32_flexible_matches_fp64():
