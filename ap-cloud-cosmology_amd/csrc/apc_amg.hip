@@ -1426,28 +1426,38 @@ __global__ void k_geo_cells0(int n, const double *__restrict__ cx, const double 
   cell[i] = make_int4(ix, iy, iz, L);
 }
 
-// key of row i's ancestor cell: the Morton code of its corner at kGeoRes
-// bits per axis, then its level (5 bits); the sort groups an ancestor's
-// rows and orders the ancestors along the Morton curve.  bad counts cells
-// whose ancestor lies deeper than kGeoRes.
+// key of row i's ancestor cell: the Morton code of its corner at R bits per
+// axis (R = the deepest ancestor level present, at most kGeoRes), then its
+// level (5 bits); the sort groups an ancestor's rows and orders the
+// ancestors along the Morton curve.  bad counts cells whose ancestor lies
+// deeper than R.
 constexpr int kGeoRes = 19;
-__device__ inline uint64_t geo_key(int4 c, int up) {
+__device__ inline uint64_t geo_key(int4 c, int up, int R) {
   const int la = max(c.w - up, 0), sh = c.w - la;
-  const int up2 = kGeoRes - min(la, kGeoRes);
+  const int up2 = R - min(la, R);
   const uint64_t m = spread3((uint32_t)(c.x >> sh) << up2) |
                      (spread3((uint32_t)(c.y >> sh) << up2) << 1) |
                      (spread3((uint32_t)(c.z >> sh) << up2) << 2);
   return (m << 5) | (uint64_t)la;
 }
 
-__global__ void k_geo_keys(int n, const int4 *__restrict__ cell, int up,
+// the deepest ancestor level over the rows (*lev starts at 0), so that the
+// keys need only 3 bits per level of it
+__global__ void k_geo_maxlev(int n, const int4 *__restrict__ cell, int up, int *__restrict__ lev) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  int la = (i < n) ? max(cell[i].w - up, 0) : 0;
+  for (int o = 32; o > 0; o >>= 1) la = max(la, __shfl_xor(la, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(lev, la);
+}
+
+__global__ void k_geo_keys(int n, const int4 *__restrict__ cell, int up, int R,
                            uint64_t *__restrict__ key, int *__restrict__ idx,
                            int *__restrict__ bad) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const int4 c = cell[i];
-  if (c.w - up > kGeoRes) atomicAdd(bad, 1);
-  key[i] = geo_key(c, up);
+  if (c.w - up > R) atomicAdd(bad, 1);
+  key[i] = geo_key(c, up, R);
   idx[i] = i;
 }
 
@@ -1460,12 +1470,13 @@ __global__ void k_geo_boxkeys(int n, const int *__restrict__ idx, const int *__r
 }
 
 __global__ void k_geo_flags_box(int n, const int *__restrict__ idx, const int *__restrict__ box,
-                                const int4 *__restrict__ cell, int up, int *__restrict__ f) {
+                                const int4 *__restrict__ cell, int up, int R,
+                                int *__restrict__ f) {
   const int s = blockIdx.x * 256 + threadIdx.x;
   if (s >= n) return;
   if (s == 0) { f[0] = 1; return; }
   const int i = idx[s], j = idx[s - 1];
-  f[s] = (box[i] != box[j] || geo_key(cell[i], up) != geo_key(cell[j], up)) ? 1 : 0;
+  f[s] = (box[i] != box[j] || geo_key(cell[i], up, R) != geo_key(cell[j], up, R)) ? 1 : 0;
 }
 
 __global__ void k_geo_flags(int n, const uint64_t *__restrict__ key, int *__restrict__ f) {
@@ -1522,9 +1533,15 @@ static int aggregate_geo(apc_ctx *c, AmgHier *h, int l, int n, int up, AmgLevel 
                        h->gcell.p);
   }
   APC_HIP(c, hipMemsetAsync(h->cnt.p, 0, 4 * sizeof(int), c->s));
-  hipLaunchKernelGGL(k_geo_keys, dim3(g), dim3(256), 0, c->s, n, (const int4 *)h->gcell.p, up,
+  hipLaunchKernelGGL(k_geo_maxlev, dim3(g), dim3(256), 0, c->s, n, (const int4 *)h->gcell.p, up,
+                     h->cnt.p + 3);
+  int R = 0;
+  APC_HIP(c, hipMemcpyAsync(&R, h->cnt.p + 3, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  APC_HIP(c, hipStreamSynchronize(c->s));
+  if (R > kGeoRes) return fail(c, APC_ERR_UNSUPPORTED, "amg: cells too deep to aggregate");
+  hipLaunchKernelGGL(k_geo_keys, dim3(g), dim3(256), 0, c->s, n, (const int4 *)h->gcell.p, up, R,
                      h->key.p, h->idx.p, h->cnt.p + 2);
-  int r = radix_sort_pairs(c, h->key.p, h->idx.p, h->key2.p, h->idx2.p, n, 3 * kGeoRes + 5);
+  int r = radix_sort_pairs(c, h->key.p, h->idx.p, h->key2.p, h->idx2.p, n, 3 * R + 5);
   if (r) return r;
   if (box) {
     hipLaunchKernelGGL(k_geo_boxkeys, dim3(g), dim3(256), 0, c->s, n, (const int *)h->idx.p, box,
@@ -1532,7 +1549,7 @@ static int aggregate_geo(apc_ctx *c, AmgHier *h, int l, int n, int up, AmgLevel 
     r = radix_sort_pairs(c, h->key.p, h->idx.p, h->key2.p, h->idx2.p, n, std::max(1, c->boxbits));
     if (r) return r;
     hipLaunchKernelGGL(k_geo_flags_box, dim3(g), dim3(256), 0, c->s, n, (const int *)h->idx.p,
-                       box, (const int4 *)h->gcell.p, up, h->flag.p);
+                       box, (const int4 *)h->gcell.p, up, R, h->flag.p);
   } else {
     hipLaunchKernelGGL(k_geo_flags, dim3(g), dim3(256), 0, c->s, n, (const uint64_t *)h->key.p,
                        h->flag.p);
