@@ -1512,7 +1512,8 @@ __global__ void k_geo_assign(int n, int nc, const int *__restrict__ idx,
 // Level l's aggregates by ancestor cells `up` levels above its rows' cells
 // (level 0: the computational nodes' cells; below: the cells the previous
 // call left in h->gcell).  Writes agg, the members (mp, ml) and the next
-// level's cells; *nc_out = the number of aggregates.  box (packed boxes):
+// level's cells; *nc_out = the number of aggregates, or -1 (nothing
+// written) where a cell lies too deep for the keys.  box (packed boxes):
 // each row's box; the aggregates are then box-major.
 static int aggregate_geo(apc_ctx *c, AmgHier *h, int l, int n, int up, AmgLevel *Lf,
                          int *nc_out, const int *box) {
@@ -1538,7 +1539,10 @@ static int aggregate_geo(apc_ctx *c, AmgHier *h, int l, int n, int up, AmgLevel 
   int R = 0;
   APC_HIP(c, hipMemcpyAsync(&R, h->cnt.p + 3, sizeof(int), hipMemcpyDeviceToHost, c->s));
   APC_HIP(c, hipStreamSynchronize(c->s));
-  if (R > kGeoRes) return fail(c, APC_ERR_UNSUPPORTED, "amg: cells too deep to aggregate");
+  if (R > kGeoRes) {  // cells deeper than the keys hold: the caller aggregates by sets
+    *nc_out = -1;
+    return 0;
+  }
   hipLaunchKernelGGL(k_geo_keys, dim3(g), dim3(256), 0, c->s, n, (const int4 *)h->gcell.p, up, R,
                      h->key.p, h->idx.p, h->cnt.p + 2);
   int r = radix_sort_pairs(c, h->key.p, h->idx.p, h->key2.p, h->idx2.p, n, 3 * R + 5);
@@ -2202,12 +2206,17 @@ int amg_setup(apc_ctx *c, AmgHier **hp, const Op<double> &A, bool periodic, bool
     // geometric aggregation where asked for and the rows are the context's
     // computational nodes (single box); a coarse level only below a
     // geometric one (its rows are then ancestor cells)
-    const bool geo = l == 0 ? (o.geo0 > 0 && geo_rows) : (!packed && o.geoc > 0 && h->geo_prev);
+    bool geo = l == 0 ? (o.geo0 > 0 && geo_rows) : (!packed && o.geoc > 0 && h->geo_prev);
     {
       KScope ks1(c, "amg_setup_aggregate");
-      if (geo) r = aggregate_geo(c, h, l, n, l == 0 ? o.geo0 : o.geoc, Lf, &nc, boxf);
-      else if (l == 0) r = aggregate_level(c, h, EllView{A}, n, Lf, &nc, boxf);
-      else r = aggregate_level(c, h, CsrView{Csr<double>{Lf->rp.p, Lf->col.p, Lf->val.p, n}}, n, Lf, &nc, boxf);
+      if (geo) {
+        r = aggregate_geo(c, h, l, n, l == 0 ? o.geo0 : o.geoc, Lf, &nc, boxf);
+        if (!r && nc < 0) geo = false;  // too deep: independent sets instead
+      }
+      if (!r && !geo)
+        r = (l == 0) ? aggregate_level(c, h, EllView{A}, n, Lf, &nc, boxf)
+                     : aggregate_level(c, h, CsrView{Csr<double>{Lf->rp.p, Lf->col.p, Lf->val.p, n}},
+                                       n, Lf, &nc, boxf);
     }
     if (r) return r;
     h->geo_prev = geo;
